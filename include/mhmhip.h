@@ -1,0 +1,254 @@
+/*
+ * mhmhip.h -- C-ABI of the MI355X (gfx950) implementation of mHM's per-step
+ * hot path: the meteo getters + per-L1-cell process chain of mHM() and the
+ * mRM Muskingum routing.
+ *
+ * The boundary is plain C: pointers, sizes and status codes, no device or
+ * framework types.  Host arrays follow the reference's Fortran layout
+ * (column-major, cells contiguous, 1-based network indices), so a Fortran
+ * caller passes c_loc(X(s1,1)) and size(X,1) through iso_c_binding.
+ *
+ * Threading: one context per host thread.  Calls on one context are
+ * serialised; kernels run asynchronously on the context's own stream and
+ * every entry point that returns host data synchronises that stream.
+ *
+ * Errors: every entry point returns MHMHIP_OK (0) or a negative code;
+ * mhmhip_last_error() gives the message.  The reference has no return codes
+ * (FORCES error_message aborts); the Fortran shim turns a negative status
+ * into error_message, which keeps the reference's abort semantics.
+ */
+#ifndef MHMHIP_H
+#define MHMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHMHIP_OK 0
+#define MHMHIP_EINVAL (-1) /* bad argument or shape                        */
+#define MHMHIP_EHIP (-2)   /* HIP runtime error (no device, launch failure) */
+#define MHMHIP_ESTATE (-3) /* call out of order (e.g. step before bind)     */
+#define MHMHIP_ENOMEM (-4) /* device memory exhausted                       */
+
+#define MHMHIP_MAX_HORIZONS 6
+
+/* Cell variables (mhmhip_set_cell_var / mhmhip_get_cell_var).  ncols is 1,
+ * nH (horizon arrays) or 12 (monthly arrays, indexed by iLAI = month). */
+enum mhmhip_var {
+  /* L1 effective parameters, the MPR output read by mHM() and the meteo
+   * getters (src/mHM/mo_mhm_interface_run.f90:366-457) -- set only */
+  MHMHIP_P_FSEALED = 0,   /* L1_fSealed(:,1,yId)                  */
+  MHMHIP_P_DEGDAY_INC,    /* L1_degDayInc                          */
+  MHMHIP_P_DEGDAY_MAX,    /* L1_degDayMax                          */
+  MHMHIP_P_DEGDAY_NOPRE,  /* L1_degDayNoPre                        */
+  MHMHIP_P_TEMP_THRESH,   /* L1_tempThresh                         */
+  MHMHIP_P_SEALED_THRESH, /* L1_sealedThresh                       */
+  MHMHIP_P_JARVIS_C1,     /* L1_jarvis_thresh_c1                   */
+  MHMHIP_P_KFASTFLOW,     /* L1_kFastFlow  (k0) [d]                */
+  MHMHIP_P_KSLOWFLOW,     /* L1_kSlowFlow  (k1) [d]                */
+  MHMHIP_P_KBASEFLOW,     /* L1_kBaseFlow  (k2) [d]                */
+  MHMHIP_P_KPERCO,        /* L1_kPerco     (kp) [d]                */
+  MHMHIP_P_ALPHA,         /* L1_alpha                              */
+  MHMHIP_P_KARSTLOSS,     /* L1_karstLoss                          */
+  MHMHIP_P_UNSAT_THRESH,  /* L1_unsatThresh                        */
+  MHMHIP_P_FASP,          /* L1_fAsp                               */
+  MHMHIP_P_HARSAMCOEFF,   /* L1_HarSamCoeff                        */
+  MHMHIP_P_LATITUDE,      /* level1%y packed [deg]                 */
+  MHMHIP_P_SOILMOIST_SAT, /* L1_soilMoistSat  (nH)                 */
+  MHMHIP_P_FROOTS,        /* L1_fRoots        (nH)                 */
+  MHMHIP_P_SOILMOIST_FC,  /* L1_soilMoistFC   (nH)                 */
+  MHMHIP_P_WILTINGPOINT,  /* L1_wiltingPoint  (nH)                 */
+  MHMHIP_P_SOILMOIST_EXP, /* L1_soilMoistExp  (nH)                 */
+  MHMHIP_P_MAXINTER,      /* L1_maxInter      (12)                 */
+  MHMHIP_P_PETLAICOR,     /* L1_petLAIcorFactor (12)               */
+  MHMHIP_P_COUNT,
+  /* states (src/mHM/mo_global_variables.f90) -- set and get */
+  MHMHIP_S_INTER = 32,    /* L1_inter                              */
+  MHMHIP_S_SNOWPACK,      /* L1_snowPack                           */
+  MHMHIP_S_SEALSTW,       /* L1_sealSTW                            */
+  MHMHIP_S_UNSATSTW,      /* L1_unsatSTW                           */
+  MHMHIP_S_SATSTW,        /* L1_satSTW                             */
+  MHMHIP_S_SOILMOIST,     /* L1_soilMoist (nH)                     */
+  MHMHIP_S_END,
+  /* fluxes of the last step -- get only */
+  MHMHIP_X_PET_CALC = 64, /* L1_pet_calc                           */
+  MHMHIP_X_TEMP_CALC,     /* L1_temp_calc                          */
+  MHMHIP_X_PREC_CALC,     /* L1_prec_calc                          */
+  MHMHIP_X_THROUGHFALL,   /* L1_Throughfall                        */
+  MHMHIP_X_AETCANOPY,     /* L1_aETCanopy                          */
+  MHMHIP_X_DEGDAY,        /* L1_degDay                             */
+  MHMHIP_X_MELT,          /* L1_melt                               */
+  MHMHIP_X_PREEFFECT,     /* L1_preEffect                          */
+  MHMHIP_X_RAIN,          /* L1_rain                               */
+  MHMHIP_X_SNOW,          /* L1_snow                               */
+  MHMHIP_X_RUNOFFSEAL,    /* L1_runoffSeal                         */
+  MHMHIP_X_AETSEALED,     /* L1_aETSealed                          */
+  MHMHIP_X_FASTRUNOFF,    /* L1_fastRunoff                         */
+  MHMHIP_X_SLOWRUNOFF,    /* L1_slowRunoff                         */
+  MHMHIP_X_PERCOL,        /* L1_percol                             */
+  MHMHIP_X_BASEFLOW,      /* L1_baseflow                           */
+  MHMHIP_X_TOTAL_RUNOFF,  /* L1_total_runoff                       */
+  MHMHIP_X_INFILSOIL,     /* L1_infilSoil (nH)                     */
+  MHMHIP_X_AETSOIL,       /* L1_aETSoil   (nH)                     */
+  MHMHIP_X_END
+};
+
+/* Daily (or hourly) forcing variables: L1_pre, L1_temp, L1_tmin, L1_tmax,
+ * L1_pet of the meteo handler (src/meteo/mo_meteo_handler.f90:120-160). */
+enum mhmhip_forcing { MHMHIP_F_PRE = 0, MHMHIP_F_TAVG, MHMHIP_F_TMIN, MHMHIP_F_TMAX,
+                      MHMHIP_F_PET, MHMHIP_F_COUNT };
+
+typedef struct mhmhip_ctx mhmhip_ctx;
+
+/* Model configuration of one context: the process switches, time step and
+ * the namelist tables the hot path reads. */
+typedef struct {
+  int64_t n_cells;        /* L1 cells over all packed domains (sum of nCells1)   */
+  int32_t n_domains;      /* domains packed in this context (>= 1)              */
+  int32_t n_horizons;     /* nSoilHorizons_mHM, 1..MHMHIP_MAX_HORIZONS          */
+  int32_t n_sets;         /* parameter sets run side by side on shared forcing  */
+  int32_t timestep;       /* model time step [h], a divisor of 24               */
+  int32_t process_soil;   /* processMatrix(3,1): 1,4 Feddes; 2,3 Jarvis         */
+  int32_t process_pet;    /* processMatrix(5,1): -1, 0 or 1                     */
+  int32_t process_route;  /* processMatrix(8,1): 0 (off) or 1 (Muskingum)       */
+  int32_t hourly_forcing; /* meteo_handler%is_hourly_forcing                    */
+  int32_t n_forcing_days; /* forcing columns held on the device                 */
+  int32_t device;         /* HIP device ordinal                                 */
+  const int64_t *domain_cell_start; /* n_domains+1 offsets into cells (NULL: 1 domain) */
+  double evap_coeff[12];  /* panEvapo evap_coeff (mhm.nml:788)                  */
+  double fnight_prec[12]; /* nightDayRatio (mhm.nml:794-813)                    */
+  double fnight_pet[12];
+  double fnight_temp[12];
+} mhmhip_config;
+
+/* Per-step scalars the reference derives on the host before mHM():
+ * tt (run_cfg%time_step), month/hour/doy from dec2date(newTime - 0.5),
+ * iMeteoTS from meteo_handler%update_timestep, read_restart. */
+typedef struct {
+  int32_t tt;          /* 1-based step counter                          */
+  int32_t month;       /* 1..12                                         */
+  int32_t hour;        /* 0..23                                         */
+  int32_t doy;         /* 1..366                                        */
+  int32_t iMeteoTS;    /* 1-based forcing column (relative to day0 = 1) */
+  int32_t read_states; /* read_restart                                  */
+} mhmhip_step;
+
+/* The L11 network of every packed domain, concatenated like the
+ * reference's module arrays (src/mRM/mo_mrm_global_variables.f90:190-218):
+ * per-domain slices with per-domain 1-based indices.  L1 = L11 is required
+ * (L1_L11_Id(k) == k within each domain, map_flag = .true.). */
+typedef struct {
+  int64_t n_nodes;               /* total L11 nodes (= cells)                    */
+  const int64_t *domain_node_start; /* n_domains+1 offsets (NULL: one domain)    */
+  const int32_t *n_outlets;      /* per domain L11_nOutlets                      */
+  const int32_t *netPerm;        /* per domain nLinks entries, 1-based, packed   */
+  const int32_t *fromN;          /* like netPerm                                 */
+  const int32_t *toN;
+  const double *length;          /* L11_length [m], per link                     */
+  const double *slope;           /* L11_slope, per link                          */
+  const double *fFPimp;          /* L11_nLinkFracFPimp, per link                 */
+  const double *L1_area;         /* level1%CellArea * 1e-6 [km2], per cell       */
+  const int32_t *L1_L11_Id;      /* per cell, 1-based within its domain          */
+  int32_t n_gauges;              /* gauges over all domains                      */
+  int32_t n_inflow;              /* inflow gauges over all domains               */
+  const int32_t *gauge_domain;   /* 0-based domain of each gauge                 */
+  const int32_t *gauge_node;     /* 1-based node within that domain              */
+  const int32_t *inflow_domain;  /* 0-based domain of each inflow gauge          */
+  const int32_t *inflow_node;    /* 1-based node within that domain              */
+  const int32_t *inflow_headwater; /* InflowGaugeHeadwater (0/1)                 */
+} mhmhip_network;
+
+/* Options of mhmhip_run. */
+typedef struct {
+  int32_t mode;          /* MHMHIP_RUN_PLAIN or MHMHIP_RUN_FUSED                 */
+  int32_t steps_per_launch; /* fused path: S (0 = autotuned)                   */
+  int32_t block;         /* cell-kernel block size (0 = autotuned)               */
+  int32_t use_graph;     /* capture the tile sequence in a hipGraph              */
+  int32_t route_mode;    /* 0 auto, 1 grid diagonals, 2 workgroup-resident       */
+  int32_t reserved[3];
+} mhmhip_run_opts;
+
+#define MHMHIP_RUN_PLAIN 0 /* one launch per step, every flux written each step */
+#define MHMHIP_RUN_FUSED 1 /* S steps per launch, states held in registers      */
+
+/* ---- lifecycle ------------------------------------------------------------ */
+
+/* Replaces the allocation part of mhm_initialize / variables_alloc
+ * (src/mHM/mo_startup.f90:68-122, src/mHM/mo_init_states.f90:49). */
+int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out);
+int mhmhip_destroy(mhmhip_ctx *ctx);
+/* Message of the last failed call on ctx (ctx NULL: last failed create). */
+const char *mhmhip_last_error(const mhmhip_ctx *ctx);
+/* Bytes of device memory held by ctx (admission control). */
+int64_t mhmhip_device_bytes(const mhmhip_ctx *ctx);
+/* Joins the context's stream. */
+int mhmhip_sync(mhmhip_ctx *ctx);
+
+/* ---- cell data -------------------------------------------------------------
+ * host points at X(1,1) of a Fortran array X(ld, ncols) holding n_cells rows
+ * of parameter set `set`.  Replaces the module-array reads/writes of mHM()
+ * (src/mHM/mo_mhm.f90:229-242) and get.L1_variable (pybind/src/wrapper.f90:569-661). */
+int mhmhip_set_cell_var(mhmhip_ctx *ctx, int32_t var, int32_t set, const double *host,
+                        int64_t ld, int32_t ncols);
+int mhmhip_get_cell_var(mhmhip_ctx *ctx, int32_t var, int32_t set, double *host, int64_t ld,
+                        int32_t ncols);
+/* fluxes_states_default_init (src/mHM/mo_init_states.f90:258-345) for every set. */
+int mhmhip_default_states(mhmhip_ctx *ctx, const double *horizon_depth);
+
+/* Forcing columns day0..day0+ndays-1 (1-based) of one variable, host X(ld, ndays).
+ * Replaces the meteo handler's L1_* arrays and the set_meteo seam
+ * (src/meteo/mo_meteo_handler.f90:1462-1605). */
+int mhmhip_set_forcing(mhmhip_ctx *ctx, int32_t var, int32_t day0, int32_t ndays,
+                       const double *host, int64_t ld);
+
+/* ---- one step ----------------------------------------------------------------
+ * get_corrected_pet + get_temp + get_prec + mHM() for every cell and set:
+ * src/mHM/mo_mhm_interface_run.f90:357-457 calling
+ * src/meteo/mo_meteo_handler.f90:915-1285 and src/mHM/mo_mhm.f90:229-521. */
+int mhmhip_mhm_step(mhmhip_ctx *ctx, const mhmhip_step *st);
+
+/* ---- routing -------------------------------------------------------------- */
+
+/* L11_routing_order (src/mRM/mo_mrm_net_startup.f90:728-859) in
+ * O(nLinks log nLinks): the same rOrder/netPerm as the reference's sweeps.
+ * Host-only; fromN/toN/rOrder/netPerm have n_nodes entries, 1-based. */
+int mhmhip_routing_order(int64_t n_nodes, int64_t n_links, const int32_t *fromN,
+                         const int32_t *toN, int32_t *rOrder, int32_t *netPerm);
+
+/* Network of every packed domain (input of mRM_routing,
+ * src/mRM/mo_mrm_routing.f90:104-110); resets routing states to 0
+ * (src/mRM/mo_mrm_init.f90:453-494). */
+int mhmhip_bind_network(mhmhip_ctx *ctx, const mhmhip_network *net);
+/* The five routing1 parameters of one parameter set (mhm_parameter.nml:193-201);
+ * C1/C2 follow from reg_rout (src/mRM/mo_mrm_mpr.f90:61-119). */
+int mhmhip_set_routing_params(mhmhip_ctx *ctx, int32_t set, const double gamma[5]);
+/* Inflow discharge [m3 s-1] of every inflow gauge for the next mrm step(s):
+ * q[step][n_inflow] for nsteps steps (InflowDischarge). */
+int mhmhip_set_inflow(mhmhip_ctx *ctx, int32_t nsteps, const double *q);
+/* L11 routing states of one set, per node in reference order (1-based node k
+ * at index domain_node_start[d] + k - 1): qOUT, qTIN(:,IT1), qTR(:,IT1), Qmod,
+ * C1, C2.  Any pointer may be NULL. */
+int mhmhip_get_routing_state(mhmhip_ctx *ctx, int32_t set, double *qOUT, double *qTIN,
+                             double *qTR, double *Qmod, double *C1, double *C2);
+int mhmhip_set_routing_state(mhmhip_ctx *ctx, int32_t set, const double *qTIN,
+                             const double *qTR);
+
+/* One mRM_routing() call for every domain and set after mhmhip_mhm_step:
+ * src/mRM/mo_mrm_routing.f90:104-303, called at
+ * src/mHM/mo_mhm_interface_run.f90:552-590.  gauge_out (host, may be NULL)
+ * receives GaugeDischarge as [set][gauge]. */
+int mhmhip_mrm_step(mhmhip_ctx *ctx, const mhmhip_step *st, double *gauge_out);
+
+/* ---- the time loop ------------------------------------------------------------
+ * nsteps consecutive steps of do_time_step's hot part (mHM + mRM_routing,
+ * src/mHM/mo_mhm_interface_run.f90:341-638) with the routing pipelined
+ * across time.  gauge_out (host, may be NULL) receives [step][set][gauge]. */
+int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
+               const mhmhip_run_opts *opts, double *gauge_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

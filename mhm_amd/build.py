@@ -1,0 +1,87 @@
+"""Builds the native libraries of mhm_amd in-tree.
+
+  mhm_amd/libmhmhip.so  HIP kernels + C-ABI (include/mhmhip.h), gfx950
+  mhm_amd/libmhmsyn.so  host-only synthetic-domain tools (priority-flood D8)
+
+Run ``python -m mhm_amd.build`` (or ``__graft_entry__.build()``).
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+OBJ = os.path.join(PKG, "_build")
+
+HIP_LIB = os.path.join(PKG, "libmhmhip.so")
+SYN_LIB = os.path.join(PKG, "libmhmsyn.so")
+
+HIP_SOURCES = ["kernels.hip", "api.hip", "network.cpp"]
+ARCH = os.environ.get("MHMHIP_ARCH", "gfx950")
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def hip_flags():
+    return [
+        f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+        # every + - * / rounds on its own, as in the reference (no FMA contraction)
+        "-ffp-contract=off",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC,
+    ]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout)
+        raise RuntimeError(f"build failed: {os.path.basename(cmd[-1])}")
+    return r.stdout
+
+
+def build_hip(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    jobs = []
+    for src in HIP_SOURCES:
+        o = os.path.join(OBJ, src + ".o")
+        objs.append(o)
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        jobs.append([hipcc] + hip_flags() + ["-c"] + lang + [os.path.join(CSRC, src), "-o", o])
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out.strip():
+                print(out)
+    tmp = HIP_LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def build_syn():
+    tmp = SYN_LIB + ".tmp"
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-o", tmp,
+          os.path.join(CSRC, "synth.cpp")])
+    os.replace(tmp, SYN_LIB)
+    return SYN_LIB
+
+
+def build_all(verbose=False):
+    build_syn()
+    build_hip(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)
+    print("built", HIP_LIB, SYN_LIB)

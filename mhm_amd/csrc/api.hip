@@ -1,0 +1,968 @@
+// api.hip -- host side of the mhmhip C-ABI: device memory, data layout,
+// argument checking and the time loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mhmhip.h"
+#include "internal.h"
+#include "launch.h"
+#include "network.h"
+
+using namespace mhmhip;
+
+namespace {
+
+// FORCES mo_constants values used by the host-evaluated terms.
+const double kTwoPiD = 6.283185307179586476925286766559005768394;
+const double kDeg2Rad = 3.141592653589793238462643383279502884197 / 180.0;
+const double kYearDays = 365.0;
+const double kHourSecs = 3600.0;
+
+thread_local std::string g_create_error;
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct mhmhip_ctx {
+  mhmhip_config cfg{};
+  int nh = 0;
+  int n_sets = 1;
+  int n_domains = 1;
+  int64_t n_cells = 0, sstride = 0, ld = 0;
+  double c2TSTu = 1.0, nT = 24.0;
+  std::vector<int64_t> dom_cell_start;
+  MonthTables mt{};
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  double *P = nullptr, *S = nullptr, *X = nullptr, *F = nullptr;
+  int64_t n_prows = 0, n_srows = 0, n_xrows = 0;
+  std::vector<int64_t> pos;  // reference cell -> device index (identity until a network is bound)
+  double *stage = nullptr;   // pinned staging row (sstride doubles)
+
+  // routing
+  bool net_bound = false;
+  LayoutOut lay;
+  NodeMeta *d_meta = nullptr;
+  double *d_area = nullptr, *d_C1 = nullptr, *d_C2 = nullptr, *d_qTIN = nullptr,
+         *d_pub = nullptr, *d_qOUT = nullptr;
+  int64_t *d_dom_start = nullptr;
+  int32_t *d_dom_dmax = nullptr;
+  std::vector<double> h_length, h_slope, h_fFPimp, h_area;
+  std::vector<int64_t> dom_link_start;
+  std::vector<int32_t> h_nout;
+  int32_t n_gauges = 0, n_inflow = 0;
+  int32_t *d_gauge_node = nullptr, *d_inflow_node = nullptr, *d_inflow_hw = nullptr;
+  std::vector<int32_t> h_inflow_node_dev, h_inflow_hw;
+  double *d_inflow_q = nullptr;
+  int64_t inflow_q_rows = 0;
+  double *d_ring = nullptr;
+  int64_t ring_len = 0;
+  double *d_gauge_buf = nullptr;
+  int64_t gauge_buf_rows = 0;
+  StepDev *d_steps = nullptr;
+  int64_t steps_cap = 0;
+
+  int64_t t_cell = 0, t_route = 0;  // global step counters
+  int64_t device_bytes = 0;
+
+  int fail(int code, const std::string &m) {
+    err = m;
+    return code;
+  }
+  int hipfail(hipError_t e, const char *what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? MHMHIP_ENOMEM : MHMHIP_EHIP;
+  }
+  template <typename T>
+  int dalloc(T **p, int64_t n, const char *what) {
+    if (n <= 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
+    if (e != hipSuccess) return hipfail(e, what);
+    device_bytes += sizeof(T) * n;
+    e = hipMemsetAsync(*p, 0, sizeof(T) * n, stream);
+    if (e != hipSuccess) return hipfail(e, what);
+    return MHMHIP_OK;
+  }
+  template <typename T>
+  void dfree(T *&p, int64_t n) {
+    if (p) {
+      (void)hipFree(p);
+      device_bytes -= sizeof(T) * (n <= 0 ? 1 : n);
+    }
+    p = nullptr;
+  }
+};
+
+#define CHECK_HIP(ctx, expr, what)                  \
+  do {                                              \
+    hipError_t e_ = (expr);                         \
+    if (e_ != hipSuccess) return (ctx)->hipfail(e_, what); \
+  } while (0)
+
+#define CHECK_RC(expr)         \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != MHMHIP_OK) return rc_; \
+  } while (0)
+
+// Copies are ordered on the context's (non-blocking) stream behind the
+// clears and kernels queued there, then joined.
+static hipError_t sync_copy(mhmhip_ctx *x, void *dst, const void *src, size_t n,
+                            hipMemcpyKind k) {
+  hipError_t e = hipMemcpyAsync(dst, src, n, k, x->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(x->stream);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
+  if (!cfg || !out) {
+    g_create_error = "create: null argument";
+    return MHMHIP_EINVAL;
+  }
+  *out = nullptr;
+  const mhmhip_config &c = *cfg;
+  if (c.n_cells < 1 || c.n_horizons < 1 || c.n_horizons > MHMHIP_MAX_HORIZONS || c.n_sets < 1 ||
+      c.timestep < 1 || c.timestep > 24 || 24 % c.timestep != 0 || c.process_soil < 1 ||
+      c.process_soil > 4 || c.process_pet < -1 || c.process_pet > 1 || c.process_route < 0 ||
+      c.process_route > 1 || c.n_forcing_days < 1 || c.n_domains < 1) {
+    g_create_error = "create: configuration out of range";
+    return MHMHIP_EINVAL;
+  }
+  std::vector<int64_t> ds(c.n_domains + 1);
+  if (c.domain_cell_start) {
+    for (int d = 0; d <= c.n_domains; ++d) ds[d] = c.domain_cell_start[d];
+    if (ds[0] != 0 || ds[c.n_domains] != c.n_cells) {
+      g_create_error = "create: domain_cell_start must run from 0 to n_cells";
+      return MHMHIP_EINVAL;
+    }
+    for (int d = 0; d < c.n_domains; ++d)
+      if (ds[d + 1] <= ds[d]) {
+        g_create_error = "create: empty or unordered domain";
+        return MHMHIP_EINVAL;
+      }
+  } else {
+    if (c.n_domains != 1) {
+      g_create_error = "create: domain_cell_start required for several domains";
+      return MHMHIP_EINVAL;
+    }
+    ds[0] = 0;
+    ds[1] = c.n_cells;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    g_create_error = "create: no HIP device";
+    return MHMHIP_EHIP;
+  }
+  if (c.device < 0 || c.device >= ndev) {
+    g_create_error = "create: device ordinal out of range";
+    return MHMHIP_EINVAL;
+  }
+  if (hipSetDevice(c.device) != hipSuccess) {
+    g_create_error = "create: hipSetDevice failed";
+    return MHMHIP_EHIP;
+  }
+  mhmhip_ctx *x = new mhmhip_ctx();
+  x->cfg = c;
+  x->cfg.domain_cell_start = nullptr;
+  x->dom_cell_start = ds;
+  x->nh = c.n_horizons;
+  x->n_sets = c.n_sets;
+  x->n_domains = c.n_domains;
+  x->n_cells = c.n_cells;
+  x->sstride = round_up(c.n_cells, 64);
+  x->ld = x->sstride * c.n_sets;
+  x->c2TSTu = (double)c.timestep / 24.0;  // mo_startup.f90:168 c2TSTu = real(timeStep,dp)/24
+  x->nT = (double)(24 / c.timestep);      // mo_meteo_handler.f90:327
+  for (int m = 0; m < 12; ++m) {
+    x->mt.evap[m] = c.evap_coeff[m];
+    x->mt.fnight_prec[m] = c.fnight_prec[m];
+    x->mt.fnight_pet[m] = c.fnight_pet[m];
+    x->mt.fnight_temp[m] = c.fnight_temp[m];
+    // mo_meteo_handler.f90:404-406
+    x->mt.fday_prec[m] = 1.0 - c.fnight_prec[m];
+    x->mt.fday_pet[m] = 1.0 - c.fnight_pet[m];
+    x->mt.fday_temp[m] = -1.0 * c.fnight_temp[m];
+  }
+  x->pos.resize(c.n_cells);
+  for (int64_t i = 0; i < c.n_cells; ++i) x->pos[i] = i;
+  int rc = MHMHIP_OK;
+  if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
+    g_create_error = "create: stream creation failed";
+    delete x;
+    return MHMHIP_EHIP;
+  }
+  x->n_prows = pr_count(x->nh);
+  x->n_srows = sr_count(x->nh);
+  x->n_xrows = xr_count(x->nh);
+  if ((rc = x->dalloc(&x->P, x->n_prows * x->ld, "alloc params")) ||
+      (rc = x->dalloc(&x->S, x->n_srows * x->ld, "alloc states")) ||
+      (rc = x->dalloc(&x->X, x->n_xrows * x->ld, "alloc fluxes")) ||
+      (rc = x->dalloc(&x->F, (int64_t)MHMHIP_F_COUNT * c.n_forcing_days * x->sstride,
+                      "alloc forcing"))) {
+    g_create_error = x->err;
+    mhmhip_destroy(x);
+    return rc;
+  }
+  if (hipHostMalloc((void **)&x->stage, sizeof(double) * x->sstride, hipHostMallocDefault) !=
+      hipSuccess) {
+    g_create_error = "create: pinned staging allocation failed";
+    mhmhip_destroy(x);
+    return MHMHIP_ENOMEM;
+  }
+  if (hipStreamSynchronize(x->stream) != hipSuccess) {
+    g_create_error = "create: initial clear failed";
+    mhmhip_destroy(x);
+    return MHMHIP_EHIP;
+  }
+  *out = x;
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
+  if (!x) return MHMHIP_OK;
+  (void)hipSetDevice(x->cfg.device);
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  (void)hipFree(x->P);
+  (void)hipFree(x->S);
+  (void)hipFree(x->X);
+  (void)hipFree(x->F);
+  (void)hipFree(x->d_meta);
+  (void)hipFree(x->d_area);
+  (void)hipFree(x->d_C1);
+  (void)hipFree(x->d_C2);
+  (void)hipFree(x->d_qTIN);
+  (void)hipFree(x->d_pub);
+  (void)hipFree(x->d_qOUT);
+  (void)hipFree(x->d_dom_start);
+  (void)hipFree(x->d_dom_dmax);
+  (void)hipFree(x->d_gauge_node);
+  (void)hipFree(x->d_inflow_node);
+  (void)hipFree(x->d_inflow_hw);
+  (void)hipFree(x->d_inflow_q);
+  (void)hipFree(x->d_ring);
+  (void)hipFree(x->d_gauge_buf);
+  (void)hipFree(x->d_steps);
+  if (x->stage) (void)hipHostFree(x->stage);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+  return MHMHIP_OK;
+}
+
+extern "C" const char *mhmhip_last_error(const mhmhip_ctx *x) {
+  return x ? x->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int64_t mhmhip_device_bytes(const mhmhip_ctx *x) { return x ? x->device_bytes : 0; }
+
+extern "C" int mhmhip_sync(mhmhip_ctx *x) {
+  if (!x) return MHMHIP_EINVAL;
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "sync");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "sync");
+  return MHMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// cell data
+
+namespace {
+
+// upload one device row slice from host column col (reference cell order),
+// applying f to every value
+template <typename Fn>
+int put_row(mhmhip_ctx *x, double *dst, const double *src, Fn f) {
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "stage wait");
+  std::fill(x->stage, x->stage + x->sstride, 0.0);
+  for (int64_t c = 0; c < x->n_cells; ++c) x->stage[x->pos[c]] = f(src[c]);
+  CHECK_HIP(x, hipMemcpyAsync(dst, x->stage, sizeof(double) * x->sstride, hipMemcpyHostToDevice,
+                              x->stream),
+            "upload");
+  return MHMHIP_OK;
+}
+
+int get_row(mhmhip_ctx *x, const double *srcdev, double *dst) {
+  CHECK_HIP(x, hipMemcpyAsync(x->stage, srcdev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost,
+                              x->stream),
+            "download");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "download");
+  for (int64_t c = 0; c < x->n_cells; ++c) dst[c] = x->stage[x->pos[c]];
+  return MHMHIP_OK;
+}
+
+int var_cols(const mhmhip_ctx *x, int var) {
+  switch (var) {
+    case MHMHIP_P_SOILMOIST_SAT:
+    case MHMHIP_P_FROOTS:
+    case MHMHIP_P_SOILMOIST_FC:
+    case MHMHIP_P_WILTINGPOINT:
+    case MHMHIP_P_SOILMOIST_EXP:
+    case MHMHIP_S_SOILMOIST:
+    case MHMHIP_X_INFILSOIL:
+    case MHMHIP_X_AETSOIL:
+      return x->nh;
+    case MHMHIP_P_MAXINTER:
+    case MHMHIP_P_PETLAICOR:
+      return 12;
+    default:
+      return 1;
+  }
+}
+
+}  // namespace
+
+extern "C" int mhmhip_set_cell_var(mhmhip_ctx *x, int32_t var, int32_t set, const double *host,
+                                   int64_t ld, int32_t ncols) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!host || set < 0 || set >= x->n_sets || ld < x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "set_cell_var: bad pointer, set or leading dimension");
+  const bool is_param = var >= 0 && var < MHMHIP_P_COUNT;
+  const bool is_state = var >= MHMHIP_S_INTER && var < MHMHIP_S_END;
+  if (!is_param && !is_state) return x->fail(MHMHIP_EINVAL, "set_cell_var: unknown or read-only variable");
+  if (ncols != var_cols(x, var)) return x->fail(MHMHIP_EINVAL, "set_cell_var: wrong column count");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_cell_var");
+  const int64_t off = (int64_t)set * x->sstride;
+  auto P = [&](int row) { return x->P + row * x->ld + off; };
+  auto id = [](double v) { return v; };
+  const int nh = x->nh;
+  const double c2 = x->c2TSTu;
+  switch (var) {
+    case MHMHIP_P_FSEALED: return put_row(x, P(PR_FS), host, id);
+    case MHMHIP_P_DEGDAY_INC: return put_row(x, P(PR_INCR), host, id);
+    // mo_mhm.f90:463-464 pass deg_day_max*c2TSTu and deg_day_noprec*c2TSTu
+    case MHMHIP_P_DEGDAY_MAX: return put_row(x, P(PR_DMAXS), host, [c2](double v) { return v * c2; });
+    case MHMHIP_P_DEGDAY_NOPRE: return put_row(x, P(PR_DNPS), host, [c2](double v) { return v * c2; });
+    case MHMHIP_P_TEMP_THRESH: return put_row(x, P(PR_TTH), host, id);
+    case MHMHIP_P_SEALED_THRESH: return put_row(x, P(PR_SEALTHR), host, id);
+    case MHMHIP_P_JARVIS_C1: return put_row(x, P(PR_JARVIS), host, id);
+    // mo_mhm.f90:484,488 pass c2TSTu/k0, c2TSTu/k1, c2TSTu/kp, c2TSTu/k2
+    case MHMHIP_P_KFASTFLOW: return put_row(x, P(PR_K0S), host, [c2](double v) { return c2 / v; });
+    case MHMHIP_P_KSLOWFLOW: return put_row(x, P(PR_K1S), host, [c2](double v) { return c2 / v; });
+    case MHMHIP_P_KBASEFLOW: return put_row(x, P(PR_K2S), host, [c2](double v) { return c2 / v; });
+    case MHMHIP_P_KPERCO: return put_row(x, P(PR_KPS), host, [c2](double v) { return c2 / v; });
+    case MHMHIP_P_ALPHA: return put_row(x, P(PR_ALPHA), host, id);
+    case MHMHIP_P_KARSTLOSS: return put_row(x, P(PR_KARST), host, id);
+    case MHMHIP_P_UNSAT_THRESH: return put_row(x, P(PR_UNSATTHR), host, id);
+    case MHMHIP_P_FASP: return put_row(x, P(PR_FASP), host, id);
+    case MHMHIP_P_HARSAMCOEFF: return put_row(x, P(PR_HARSAM), host, id);
+    case MHMHIP_P_LATITUDE:
+      // mo_pet.f90:113 deg2rad_dp * latitude, then tan/sin/cos of it (331-353)
+      CHECK_RC(put_row(x, P(PR_TANLAT), host, [](double v) { return std::tan(kDeg2Rad * v); }));
+      CHECK_RC(put_row(x, P(PR_SINLAT), host, [](double v) { return std::sin(kDeg2Rad * v); }));
+      return put_row(x, P(PR_COSLAT), host, [](double v) { return std::cos(kDeg2Rad * v); });
+    default: break;
+  }
+  int row0 = -1;
+  double *base = x->P;
+  switch (var) {
+    case MHMHIP_P_SOILMOIST_SAT: row0 = pr_sat(nh); break;
+    case MHMHIP_P_FROOTS: row0 = pr_froots(nh); break;
+    case MHMHIP_P_SOILMOIST_FC: row0 = pr_fc(nh); break;
+    case MHMHIP_P_WILTINGPOINT: row0 = pr_wp(nh); break;
+    case MHMHIP_P_SOILMOIST_EXP: row0 = pr_exp(nh); break;
+    case MHMHIP_P_MAXINTER: row0 = pr_maxinter(nh); break;
+    case MHMHIP_P_PETLAICOR: row0 = pr_petlai(nh); break;
+    case MHMHIP_S_INTER: row0 = SR_INTER; base = x->S; break;
+    case MHMHIP_S_SNOWPACK: row0 = SR_SNOW; base = x->S; break;
+    case MHMHIP_S_SEALSTW: row0 = SR_SEAL; base = x->S; break;
+    case MHMHIP_S_UNSATSTW: row0 = SR_UNSAT; base = x->S; break;
+    case MHMHIP_S_SATSTW: row0 = SR_SAT; base = x->S; break;
+    case MHMHIP_S_SOILMOIST: row0 = SR_NSCALAR; base = x->S; break;
+    default: return x->fail(MHMHIP_EINVAL, "set_cell_var: unknown variable");
+  }
+  for (int col = 0; col < ncols; ++col)
+    CHECK_RC(put_row(x, base + (row0 + col) * x->ld + off, host + col * ld, id));
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_cell_var");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_get_cell_var(mhmhip_ctx *x, int32_t var, int32_t set, double *host,
+                                   int64_t ld, int32_t ncols) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!host || set < 0 || set >= x->n_sets || ld < x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "get_cell_var: bad pointer, set or leading dimension");
+  if (ncols != var_cols(x, var)) return x->fail(MHMHIP_EINVAL, "get_cell_var: wrong column count");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "get_cell_var");
+  const int nh = x->nh;
+  int row0 = -1;
+  double *base = nullptr;
+  if (var >= MHMHIP_S_INTER && var < MHMHIP_S_END) {
+    base = x->S;
+    row0 = (var == MHMHIP_S_SOILMOIST) ? SR_NSCALAR : (var - MHMHIP_S_INTER);
+  } else if (var >= MHMHIP_X_PET_CALC && var < MHMHIP_X_END) {
+    base = x->X;
+    if (var == MHMHIP_X_INFILSOIL)
+      row0 = xr_infil(nh);
+    else if (var == MHMHIP_X_AETSOIL)
+      row0 = xr_aet(nh);
+    else
+      row0 = var - MHMHIP_X_PET_CALC;
+  } else {
+    return x->fail(MHMHIP_EINVAL, "get_cell_var: only states and fluxes can be read");
+  }
+  const int64_t off = (int64_t)set * x->sstride;
+  for (int col = 0; col < ncols; ++col)
+    CHECK_RC(get_row(x, base + (row0 + col) * x->ld + off, host + col * ld));
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_default_states(mhmhip_ctx *x, const double *hd) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!hd && x->nh > 1) return x->fail(MHMHIP_EINVAL, "default_states: horizon depths required");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "default_states");
+  std::vector<double> v(x->n_cells);
+  auto fill = [&](double val) { std::fill(v.begin(), v.end(), val); };
+  auto id = [](double a) { return a; };
+  for (int s = 0; s < x->n_sets; ++s) {
+    const int64_t off = (int64_t)s * x->sstride;
+    // mo_init_states.f90:273-287 with mo_mpr_constants P1..P5, C1_InitStateSM
+    fill(0.0);
+    CHECK_RC(put_row(x, x->S + SR_INTER * x->ld + off, v.data(), id));
+    CHECK_RC(put_row(x, x->S + SR_SEAL * x->ld + off, v.data(), id));
+    fill(15.0);
+    CHECK_RC(put_row(x, x->S + SR_SNOW * x->ld + off, v.data(), id));
+    fill(10.0);
+    CHECK_RC(put_row(x, x->S + SR_UNSAT * x->ld + off, v.data(), id));
+    fill(75.0);
+    CHECK_RC(put_row(x, x->S + SR_SAT * x->ld + off, v.data(), id));
+    for (int h = 0; h < x->nh - 1; ++h) {
+      fill((h == 0 ? hd[0] : hd[h] - hd[h - 1]) * 0.25);
+      CHECK_RC(put_row(x, x->S + (SR_NSCALAR + h) * x->ld + off, v.data(), id));
+    }
+    fill((1500.0 - (x->nh >= 2 ? hd[x->nh - 2] : 0.0)) * 0.25);
+    CHECK_RC(put_row(x, x->S + (SR_NSCALAR + x->nh - 1) * x->ld + off, v.data(), id));
+    fill(0.0);
+    for (int r = 0; r < x->n_xrows; ++r)
+      CHECK_RC(put_row(x, x->X + r * x->ld + off, v.data(), id));
+  }
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "default_states");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_set_forcing(mhmhip_ctx *x, int32_t var, int32_t day0, int32_t ndays,
+                                  const double *host, int64_t ld) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!host || var < 0 || var >= MHMHIP_F_COUNT || day0 < 1 || ndays < 1 ||
+      (int64_t)day0 - 1 + ndays > x->cfg.n_forcing_days || ld < x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "set_forcing: variable or day range out of bounds");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_forcing");
+  auto id = [](double a) { return a; };
+  for (int32_t j = 0; j < ndays; ++j) {
+    double *dst = x->F + ((int64_t)var * x->cfg.n_forcing_days + (day0 - 1 + j)) * x->sstride;
+    CHECK_RC(put_row(x, dst, host + (int64_t)j * ld, id));
+  }
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_forcing");
+  return MHMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step scalars
+
+namespace {
+
+int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
+  if (s.tt < 1 || s.month < 1 || s.month > 12 || s.hour < 0 || s.hour > 23 || s.doy < 1 ||
+      s.doy > 366 || s.iMeteoTS < 1 || s.iMeteoTS > x->cfg.n_forcing_days)
+    return x->fail(MHMHIP_EINVAL, "step: tt/month/hour/doy/iMeteoTS out of range (iMeteoTS " +
+                                      std::to_string(s.iMeteoTS) + ", forcing days " +
+                                      std::to_string(x->cfg.n_forcing_days) + ")");
+  // mo_pet.f90:339-340, evaluated with the same libm as the reference
+  const double doy = (double)s.doy;
+  d.dr = 1.0 + 0.0330 * std::cos(kTwoPiD * doy / kYearDays);
+  const double delta = 0.4090 * std::sin(kTwoPiD * doy / kYearDays - 1.3900);
+  d.tand = std::tan(delta);
+  d.sind = std::sin(delta);
+  d.cosd = std::cos(delta);
+  d.day = s.iMeteoTS - 1;
+  d.month0 = s.month - 1;
+  d.isday = (s.hour > 6) && (s.hour <= 18);  // mo_meteo_handler.f90:1048
+  d.init_sm = (s.tt == 1) && !s.read_states;  // mo_mhm.f90:448
+  d.doy = s.doy;
+  d.pad = 0;
+  return MHMHIP_OK;
+}
+
+CellArgs cell_args(mhmhip_ctx *x, bool ring) {
+  CellArgs a{};
+  a.P = x->P;
+  a.S = x->S;
+  a.X = x->X;
+  a.F = x->F;
+  a.ring = ring ? x->d_ring : nullptr;
+  a.steps = x->d_steps;
+  a.n_cells = x->n_cells;
+  a.sstride = x->sstride;
+  a.ld = x->ld;
+  a.forcing_days = x->cfg.n_forcing_days;
+  a.ring_len = x->ring_len;
+  a.pet_case = x->cfg.process_pet;
+  a.jarvis = (x->cfg.process_soil == 2 || x->cfg.process_soil == 3);
+  a.hourly = x->cfg.hourly_forcing;
+  a.nT = x->nT;
+  return a;
+}
+
+bool routing_on(const mhmhip_ctx *x) { return x->cfg.process_route == 1 && x->net_bound; }
+
+int ensure_ring(mhmhip_ctx *x, int64_t len) {
+  if (x->ring_len >= len) return MHMHIP_OK;
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "ring");
+  x->dfree(x->d_ring, x->ring_len * x->ld);
+  x->ring_len = 0;
+  CHECK_RC(x->dalloc(&x->d_ring, len * x->ld, "alloc runoff ring"));
+  x->ring_len = len;
+  return MHMHIP_OK;
+}
+
+int ensure_steps(mhmhip_ctx *x, int64_t n) {
+  if (x->steps_cap >= n) return MHMHIP_OK;
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "steps");
+  x->dfree(x->d_steps, x->steps_cap);
+  x->steps_cap = 0;
+  CHECK_RC(x->dalloc(&x->d_steps, n, "alloc step table"));
+  x->steps_cap = n;
+  return MHMHIP_OK;
+}
+
+int ensure_gauge_buf(mhmhip_ctx *x, int64_t rows) {
+  const int64_t per = (int64_t)x->n_sets * std::max(1, x->n_gauges);
+  if (x->gauge_buf_rows >= rows) return MHMHIP_OK;
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "gauges");
+  x->dfree(x->d_gauge_buf, x->gauge_buf_rows * per);
+  x->gauge_buf_rows = 0;
+  CHECK_RC(x->dalloc(&x->d_gauge_buf, rows * per, "alloc gauge series"));
+  x->gauge_buf_rows = rows;
+  return MHMHIP_OK;
+}
+
+RouteArgs route_args(mhmhip_ctx *x, int64_t t_lo, int64_t t_hi, int64_t t_base) {
+  RouteArgs r{};
+  r.meta = x->d_meta;
+  r.area = x->d_area;
+  r.C1 = x->d_C1;
+  r.C2 = x->d_C2;
+  r.qTIN = x->d_qTIN;
+  r.pub = x->d_pub;
+  r.qOUT = x->d_qOUT;
+  r.ring = x->d_ring;
+  r.gauge_out = x->d_gauge_buf;
+  r.gauge_node = x->d_gauge_node;
+  r.inflow_node = x->d_inflow_node;
+  r.inflow_hw = x->d_inflow_hw;
+  r.inflow_q = x->d_inflow_q;
+  r.n_nodes = x->n_cells;
+  r.sstride = x->sstride;
+  r.ring_len = x->ring_len;
+  r.t_lo = t_lo;
+  r.t_hi = t_hi;
+  r.t_base = t_base;
+  r.n_sets = x->n_sets;
+  r.n_gauges = x->n_gauges;
+  r.n_inflow = x->n_inflow;
+  r.dmax = x->lay.dmax;
+  r.tst = kHourSecs * (double)x->cfg.timestep;  // mo_mrm_pre_routing.f90:110
+  return r;
+}
+
+// Largest tree (nodes of one domain) for the workgroup-resident routing.
+int64_t max_domain_nodes(const mhmhip_ctx *x) {
+  int64_t m = 0;
+  for (int d = 0; d < x->n_domains; ++d)
+    m = std::max(m, x->dom_cell_start[d + 1] - x->dom_cell_start[d]);
+  return m;
+}
+
+bool use_wg_routing(const mhmhip_ctx *x, int route_mode) {
+  if (route_mode == 1) return false;
+  if (route_mode == 2) return true;
+  return max_domain_nodes(x) <= 16384;
+}
+
+// routing diagonals [d_lo, d_hi) (tiles) or the drain after t_hi
+int route_diagonals(mhmhip_ctx *x, const RouteArgs &r, int64_t d_lo, int64_t d_hi, bool drain,
+                    bool wg) {
+  if (wg) {
+    CHECK_HIP(x,
+              launch_route_wg(r, x->d_dom_start, x->d_dom_dmax, x->n_domains, d_lo, d_hi,
+                              drain ? 1 : 0, 256, x->stream),
+              "route_wg");
+    return MHMHIP_OK;
+  }
+  const int64_t end = drain ? d_lo + x->lay.dmax : d_hi;
+  for (int64_t d = d_lo; d < end; ++d)
+    CHECK_HIP(x, launch_route_diag(r, d, x->stream), "route_diag");
+  return MHMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!st) return x->fail(MHMHIP_EINVAL, "mhm_step: null step");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "mhm_step");
+  StepDev d;
+  CHECK_RC(make_step(x, *st, d));
+  const bool route = routing_on(x);
+  if (route) {
+    if (x->t_route != x->t_cell)
+      return x->fail(MHMHIP_ESTATE, "mhm_step: previous step not routed (call mrm_step)");
+    CHECK_RC(ensure_ring(x, 2));
+  }
+  CellArgs a = cell_args(x, route);
+  CHECK_HIP(x, launch_cell_step(x->nh, a, d, x->mt, x->t_cell, 256, x->n_sets, x->stream),
+            "cell_step");
+  x->t_cell++;
+  if (!route) x->t_route = x->t_cell;
+  return MHMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// network and routing
+
+extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!net || !net->netPerm || !net->fromN || !net->toN || !net->length || !net->slope ||
+      !net->fFPimp || !net->L1_area || !net->L1_L11_Id || !net->n_outlets)
+    return x->fail(MHMHIP_EINVAL, "bind_network: missing arrays");
+  if (net->n_nodes != x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "bind_network: L1 = L11 required (n_nodes == n_cells)");
+  std::vector<int64_t> ds(x->n_domains + 1);
+  for (int d = 0; d <= x->n_domains; ++d)
+    ds[d] = net->domain_node_start ? net->domain_node_start[d] : x->dom_cell_start[d];
+  if (ds != x->dom_cell_start)
+    return x->fail(MHMHIP_EINVAL, "bind_network: domain node ranges differ from cell ranges");
+  for (int d = 0; d < x->n_domains; ++d)
+    for (int64_t k = ds[d]; k < ds[d + 1]; ++k)
+      if (net->L1_L11_Id[k] != k - ds[d] + 1)
+        return x->fail(MHMHIP_EINVAL, "bind_network: L1_L11_Id must map each cell to its own node");
+  if (net->n_gauges < 0 || net->n_inflow < 0 ||
+      (net->n_gauges > 0 && (!net->gauge_node || !net->gauge_domain)) ||
+      (net->n_inflow > 0 && (!net->inflow_node || !net->inflow_domain || !net->inflow_headwater)))
+    return x->fail(MHMHIP_EINVAL, "bind_network: gauge lists missing");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "bind_network");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "bind_network");
+
+  LayoutIn in{x->n_cells, x->n_domains, ds.data(), net->n_outlets, net->netPerm, net->fromN,
+              net->toN};
+  LayoutOut lay;
+  std::string e;
+  int rc = build_layout(in, lay, e);
+  if (rc != MHMHIP_OK) return x->fail(rc, e);
+
+  // gauges / inflow gauges -> device node indices and flags
+  std::vector<int32_t> gnode(std::max(1, net->n_gauges));
+  for (int g = 0; g < net->n_gauges; ++g) {
+    const int d = net->gauge_domain[g];
+    if (d < 0 || d >= x->n_domains || net->gauge_node[g] < 1 ||
+        net->gauge_node[g] > ds[d + 1] - ds[d])
+      return x->fail(MHMHIP_EINVAL, "bind_network: gauge node out of range");
+    const int64_t p = lay.pos[ds[d] + net->gauge_node[g] - 1];
+    gnode[g] = (int32_t)p;
+    lay.packed[p] |= NF_GAUGE;
+  }
+  std::vector<int32_t> inode(std::max(1, net->n_inflow)), ihw(std::max(1, net->n_inflow));
+  for (int g = 0; g < net->n_inflow; ++g) {
+    const int d = net->inflow_domain[g];
+    if (d < 0 || d >= x->n_domains || net->inflow_node[g] < 1 ||
+        net->inflow_node[g] > ds[d + 1] - ds[d])
+      return x->fail(MHMHIP_EINVAL, "bind_network: inflow node out of range");
+    const int64_t p = lay.pos[ds[d] + net->inflow_node[g] - 1];
+    inode[g] = (int32_t)p;
+    ihw[g] = net->inflow_headwater[g] ? 1 : 0;
+    lay.packed[p] |= NF_INFLOW;
+    if (!ihw[g]) {
+      // links draining into this node have their routed outflow cut (mo_mrm_routing.f90:452-457)
+      const int64_t c0 = lay.child[p];
+      const int nc = nm_nchild(lay.packed[p]);
+      for (int c = 0; c < nc; ++c) lay.packed[c0 + c] |= NF_CUT;
+    }
+  }
+
+  // re-lay any cell data already on the device into routing order
+  std::vector<int64_t> old_pos = x->pos;
+  bool moved = false;
+  for (int64_t c = 0; c < x->n_cells && !moved; ++c) moved = old_pos[c] != lay.pos[c];
+  if (moved) {
+    std::vector<double> a(x->sstride), b(x->sstride);
+    auto relayout = [&](double *base, int64_t rows, int64_t stride, int64_t nsl) -> int {
+      for (int64_t r = 0; r < rows; ++r)
+        for (int64_t s = 0; s < nsl; ++s) {
+          double *row = base + r * stride + s * x->sstride;
+          CHECK_HIP(x, sync_copy(x, a.data(), row, sizeof(double) * x->sstride, hipMemcpyDeviceToHost),
+                    "relayout");
+          std::fill(b.begin(), b.end(), 0.0);
+          for (int64_t c = 0; c < x->n_cells; ++c) b[lay.pos[c]] = a[old_pos[c]];
+          CHECK_HIP(x, sync_copy(x, row, b.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice),
+                    "relayout");
+        }
+      return MHMHIP_OK;
+    };
+    CHECK_RC(relayout(x->P, x->n_prows, x->ld, x->n_sets));
+    CHECK_RC(relayout(x->S, x->n_srows, x->ld, x->n_sets));
+    CHECK_RC(relayout(x->X, x->n_xrows, x->ld, x->n_sets));
+    CHECK_RC(relayout(x->F, (int64_t)MHMHIP_F_COUNT * x->cfg.n_forcing_days, x->sstride, 1));
+  }
+  x->pos = lay.pos;
+
+  // per-link host copies for reg_rout, packed by domain
+  x->h_nout.assign(net->n_outlets, net->n_outlets + x->n_domains);
+  x->dom_link_start.assign(x->n_domains + 1, 0);
+  for (int d = 0; d < x->n_domains; ++d)
+    x->dom_link_start[d + 1] = x->dom_link_start[d] + (ds[d + 1] - ds[d]) - x->h_nout[d];
+  const int64_t nLtot = x->dom_link_start[x->n_domains];
+  x->h_length.assign(net->length, net->length + nLtot);
+  x->h_slope.assign(net->slope, net->slope + nLtot);
+  x->h_fFPimp.assign(net->fFPimp, net->fFPimp + nLtot);
+
+  // free old routing buffers and allocate new ones
+  x->dfree(x->d_meta, x->n_cells);
+  x->dfree(x->d_area, x->n_cells);
+  x->dfree(x->d_C1, x->ld);
+  x->dfree(x->d_C2, x->ld);
+  x->dfree(x->d_qTIN, x->ld);
+  x->dfree(x->d_pub, 2 * x->ld);
+  x->dfree(x->d_qOUT, x->ld);
+  x->dfree(x->d_dom_start, x->n_domains + 1);
+  x->dfree(x->d_dom_dmax, x->n_domains);
+  x->dfree(x->d_gauge_node, std::max(1, x->n_gauges));
+  x->dfree(x->d_inflow_node, std::max(1, x->n_inflow));
+  x->dfree(x->d_inflow_hw, std::max(1, x->n_inflow));
+  CHECK_RC(x->dalloc(&x->d_meta, x->n_cells, "alloc meta"));
+  CHECK_RC(x->dalloc(&x->d_area, x->n_cells, "alloc area"));
+  CHECK_RC(x->dalloc(&x->d_C1, x->ld, "alloc C1"));
+  CHECK_RC(x->dalloc(&x->d_C2, x->ld, "alloc C2"));
+  CHECK_RC(x->dalloc(&x->d_qTIN, x->ld, "alloc qTIN"));
+  CHECK_RC(x->dalloc(&x->d_pub, 2 * x->ld, "alloc qTR"));
+  CHECK_RC(x->dalloc(&x->d_qOUT, x->ld, "alloc qOUT"));
+  CHECK_RC(x->dalloc(&x->d_dom_start, x->n_domains + 1, "alloc domains"));
+  CHECK_RC(x->dalloc(&x->d_dom_dmax, x->n_domains, "alloc domains"));
+  x->n_gauges = net->n_gauges;
+  x->n_inflow = net->n_inflow;
+  CHECK_RC(x->dalloc(&x->d_gauge_node, std::max(1, x->n_gauges), "alloc gauges"));
+  CHECK_RC(x->dalloc(&x->d_inflow_node, std::max(1, x->n_inflow), "alloc inflow"));
+  CHECK_RC(x->dalloc(&x->d_inflow_hw, std::max(1, x->n_inflow), "alloc inflow"));
+
+  std::vector<NodeMeta> meta(x->n_cells);
+  std::vector<double> area(x->n_cells);
+  for (int64_t p = 0; p < x->n_cells; ++p) {
+    meta[p].child = lay.child[p];
+    meta[p].packed = lay.packed[p];
+    area[p] = net->L1_area[lay.node_at[p]];
+  }
+  std::vector<int64_t> dstart(x->n_domains + 1);
+  for (int d = 0; d <= x->n_domains; ++d) dstart[d] = ds[d];  // BFS keeps domains contiguous
+  CHECK_HIP(x, sync_copy(x, x->d_meta, meta.data(), sizeof(NodeMeta) * x->n_cells, hipMemcpyHostToDevice), "upload meta");
+  CHECK_HIP(x, sync_copy(x, x->d_area, area.data(), sizeof(double) * x->n_cells, hipMemcpyHostToDevice), "upload area");
+  CHECK_HIP(x, sync_copy(x, x->d_dom_start, dstart.data(), sizeof(int64_t) * (x->n_domains + 1), hipMemcpyHostToDevice), "upload domains");
+  CHECK_HIP(x, sync_copy(x, x->d_dom_dmax, lay.dom_dmax.data(), sizeof(int32_t) * x->n_domains, hipMemcpyHostToDevice), "upload domains");
+  CHECK_HIP(x, sync_copy(x, x->d_gauge_node, gnode.data(), sizeof(int32_t) * gnode.size(), hipMemcpyHostToDevice), "upload gauges");
+  CHECK_HIP(x, sync_copy(x, x->d_inflow_node, inode.data(), sizeof(int32_t) * inode.size(), hipMemcpyHostToDevice), "upload inflow");
+  CHECK_HIP(x, sync_copy(x, x->d_inflow_hw, ihw.data(), sizeof(int32_t) * ihw.size(), hipMemcpyHostToDevice), "upload inflow");
+  x->h_inflow_node_dev = inode;
+  x->h_inflow_hw = ihw;
+  x->lay = std::move(lay);
+  x->net_bound = true;
+  x->t_route = x->t_cell;
+  x->ring_len = std::min<int64_t>(x->ring_len, 0);
+  x->dfree(x->d_ring, 0);
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "bind_network");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, const double *p) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!p || set < 0 || set >= x->n_sets) return x->fail(MHMHIP_EINVAL, "set_routing_params: bad set");
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_params: bind a network first");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_params");
+  // reg_rout (mo_mrm_mpr.f90:61-119) per domain, TS = real(timestep) [h].  The
+  // reference re-evaluates it every step with unchanged inputs (mo_mrm_routing.f90:211-218);
+  // evaluating it once gives the same C1, C2.
+  const double TS = (double)x->cfg.timestep;
+  const int64_t nLtot = x->dom_link_start[x->n_domains];
+  std::vector<double> c1(std::max<int64_t>(nLtot, 1), 0.0), c2(std::max<int64_t>(nLtot, 1), 0.0);
+  for (int d = 0; d < x->n_domains; ++d) {
+    const int64_t l0 = x->dom_link_start[d], l1 = x->dom_link_start[d + 1];
+    if (l1 <= l0) continue;
+    double ssMax = x->h_slope[l0];
+    for (int64_t i = l0 + 1; i < l1; ++i)
+      if (x->h_slope[i] > ssMax) ssMax = x->h_slope[i];
+    for (int64_t i = l0; i < l1; ++i) {
+      double K = p[0] + p[1] * (x->h_length[i] * 0.001) + p[2] * x->h_slope[i] + p[3] * x->h_fFPimp[i];
+      double xi = p[4] * (1.0 + x->h_slope[i] / ssMax);
+      if (xi > 0.5) xi = 0.5;
+      if (xi < 0.005) xi = 0.005;
+      if (K > 0.5 * TS / xi) K = 0.5 * TS / xi;
+      if (K < 0.5 * TS / (1.0 - xi)) K = 0.5 * TS / (1.0 - xi);
+      c1[i] = TS / (K * (1.0 - xi) + 0.5 * TS);
+      c2[i] = 1.0 - c1[i] * K / TS;
+    }
+  }
+  std::vector<double> n1(x->sstride, 0.0), n2(x->sstride, 0.0);
+  for (int64_t q = 0; q < x->n_cells; ++q) {
+    const int64_t l = x->lay.link_of_node[q];
+    if (l >= 0) {
+      n1[q] = c1[l];
+      n2[q] = c2[l];
+    }
+  }
+  const int64_t off = (int64_t)set * x->sstride;
+  CHECK_HIP(x, sync_copy(x, x->d_C1 + off, n1.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C1");
+  CHECK_HIP(x, sync_copy(x, x->d_C2 + off, n2.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C2");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_set_inflow(mhmhip_ctx *x, int32_t nsteps, const double *q) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_inflow: bind a network first");
+  if (x->n_inflow == 0) return MHMHIP_OK;
+  if (nsteps < 1 || !q) return x->fail(MHMHIP_EINVAL, "set_inflow: bad series");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_inflow");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_inflow");
+  x->dfree(x->d_inflow_q, x->inflow_q_rows * x->n_inflow);
+  x->inflow_q_rows = 0;
+  CHECK_RC(x->dalloc(&x->d_inflow_q, (int64_t)nsteps * x->n_inflow, "alloc inflow series"));
+  CHECK_HIP(x, sync_copy(x, x->d_inflow_q, q, sizeof(double) * nsteps * x->n_inflow, hipMemcpyHostToDevice), "upload inflow");
+  x->inflow_q_rows = nsteps;
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_get_routing_state(mhmhip_ctx *x, int32_t set, double *qOUT, double *qTIN,
+                                        double *qTR, double *Qmod, double *C1, double *C2) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "get_routing_state: no network bound");
+  if (set < 0 || set >= x->n_sets) return x->fail(MHMHIP_EINVAL, "get_routing_state: bad set");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "get_routing_state");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "get_routing_state");
+  const int64_t off = (int64_t)set * x->sstride;
+  std::vector<double> buf(x->sstride);
+  auto node_out = [&](const double *dev, double *dst) -> int {
+    if (!dst) return MHMHIP_OK;
+    CHECK_HIP(x, sync_copy(x, buf.data(), dev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download");
+    for (int64_t k = 0; k < x->n_cells; ++k) dst[k] = buf[x->pos[k]];
+    return MHMHIP_OK;
+  };
+  auto link_out = [&](const double *dev, double *dst) -> int {
+    if (!dst) return MHMHIP_OK;
+    CHECK_HIP(x, sync_copy(x, buf.data(), dev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download");
+    for (int64_t q = 0; q < x->n_cells; ++q)
+      if (x->lay.link_of_node[q] >= 0) dst[x->lay.link_of_node[q]] = buf[q];
+    return MHMHIP_OK;
+  };
+  CHECK_RC(node_out(x->d_qOUT + off, qOUT));
+  CHECK_RC(node_out(x->d_qTIN + off, qTIN));
+  CHECK_RC(node_out(x->d_qTIN + off, Qmod));
+  CHECK_RC(node_out(x->d_pub + ((x->t_route - 1) & 1) * x->ld + off, qTR));
+  CHECK_RC(link_out(x->d_C1 + off, C1));
+  CHECK_RC(link_out(x->d_C2 + off, C2));
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double *qTIN,
+                                        const double *qTR) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_state: no network bound");
+  if (set < 0 || set >= x->n_sets) return x->fail(MHMHIP_EINVAL, "set_routing_state: bad set");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_state");
+  const int64_t off = (int64_t)set * x->sstride;
+  auto id = [](double a) { return a; };
+  if (qTIN) CHECK_RC(put_row(x, x->d_qTIN + off, qTIN, id));
+  if (qTR) CHECK_RC(put_row(x, x->d_pub + ((x->t_route - 1) & 1) * x->ld + off, qTR, id));
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_routing_state");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gauge_out) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!st) return x->fail(MHMHIP_EINVAL, "mrm_step: null step");
+  if (!routing_on(x)) return x->fail(MHMHIP_ESTATE, "mrm_step: routing is off or no network bound");
+  if (x->t_route != x->t_cell - 1)
+    return x->fail(MHMHIP_ESTATE, "mrm_step: call mhm_step for this step first");
+  if (st->read_states)
+    return x->fail(MHMHIP_EINVAL, "mrm_step: read_states (C1/C2 from a restart) is not supported");
+  if (x->n_inflow > 0 && x->inflow_q_rows < 1)
+    return x->fail(MHMHIP_ESTATE, "mrm_step: inflow discharge not set");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "mrm_step");
+  CHECK_RC(ensure_gauge_buf(x, 1));
+  const int64_t t = x->t_route;
+  RouteArgs r = route_args(x, t, t + 1, t);
+  const bool wg = use_wg_routing(x, 0);
+  CHECK_RC(route_diagonals(x, r, t, t + 1, false, wg));
+  CHECK_RC(route_diagonals(x, r, t + 1, 0, true, wg));
+  x->t_route++;
+  if (gauge_out && x->n_gauges > 0) {
+    CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf,
+                                sizeof(double) * x->n_sets * x->n_gauges, hipMemcpyDeviceToHost,
+                                x->stream),
+              "download gauges");
+  }
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "mrm_step");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
+                          const mhmhip_run_opts *opts, double *gauge_out) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!steps || nsteps < 1) return x->fail(MHMHIP_EINVAL, "run: no steps");
+  mhmhip_run_opts o{};
+  if (opts) o = *opts;
+  if (o.mode != MHMHIP_RUN_PLAIN && o.mode != MHMHIP_RUN_FUSED)
+    return x->fail(MHMHIP_EINVAL, "run: unknown mode");
+  if (o.block != 0 && o.block != 64 && o.block != 128 && o.block != 256)
+    return x->fail(MHMHIP_EINVAL, "run: block must be 64, 128 or 256");
+  if (o.route_mode < 0 || o.route_mode > 2) return x->fail(MHMHIP_EINVAL, "run: bad route_mode");
+  if (o.steps_per_launch < 0) return x->fail(MHMHIP_EINVAL, "run: bad steps_per_launch");
+  const bool route = routing_on(x);
+  if (route && x->t_route != x->t_cell)
+    return x->fail(MHMHIP_ESTATE, "run: previous step not routed (call mrm_step)");
+  for (int32_t k = 0; k < nsteps; ++k)
+    if (steps[k].read_states && route)
+      return x->fail(MHMHIP_EINVAL, "run: read_states with routing is not supported");
+  if (route && x->n_inflow > 0 && x->inflow_q_rows < nsteps)
+    return x->fail(MHMHIP_ESTATE, "run: inflow discharge series shorter than the run");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "run");
+  std::vector<StepDev> hs(nsteps);
+  for (int32_t k = 0; k < nsteps; ++k) CHECK_RC(make_step(x, steps[k], hs[k]));
+  const int S = o.mode == MHMHIP_RUN_PLAIN ? 1 : (o.steps_per_launch > 0 ? o.steps_per_launch : 24);
+  const int block = o.block > 0 ? o.block : 256;
+  const bool wg = route && use_wg_routing(x, o.route_mode);
+  CHECK_RC(ensure_steps(x, nsteps));
+  CHECK_HIP(x, hipMemcpyAsync(x->d_steps, hs.data(), sizeof(StepDev) * nsteps,
+                              hipMemcpyHostToDevice, x->stream),
+            "upload steps");
+  if (route) {
+    CHECK_RC(ensure_ring(x, (int64_t)S + x->lay.dmax + 2));
+    CHECK_RC(ensure_gauge_buf(x, nsteps));
+  }
+  const int64_t t_start = x->t_cell;
+  CellArgs a = cell_args(x, route);
+  RouteArgs r = route_args(x, t_start, t_start + nsteps, t_start);
+  for (int32_t k = 0; k < nsteps; k += S) {
+    const int n = std::min<int32_t>(S, nsteps - k);
+    if (o.mode == MHMHIP_RUN_PLAIN) {
+      CHECK_HIP(x, launch_cell_step(x->nh, a, hs[k], x->mt, t_start + k, block, x->n_sets, x->stream),
+                "cell_step");
+    } else {
+      CHECK_HIP(x, launch_cell_fused(x->nh, a, x->mt, t_start + k, n, k, block, x->n_sets, x->stream),
+                "cell_fused");
+    }
+    if (route) CHECK_RC(route_diagonals(x, r, t_start + k, t_start + k + n, false, wg));
+  }
+  if (route) CHECK_RC(route_diagonals(x, r, t_start + nsteps, 0, true, wg));
+  x->t_cell += nsteps;
+  x->t_route = x->t_cell;
+  if (route && gauge_out && x->n_gauges > 0) {
+    CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf,
+                                sizeof(double) * nsteps * x->n_sets * x->n_gauges,
+                                hipMemcpyDeviceToHost, x->stream),
+              "download gauges");
+  }
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "run");
+  return MHMHIP_OK;
+}

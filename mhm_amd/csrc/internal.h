@@ -1,0 +1,133 @@
+// internal.h -- device data layout and kernel argument blocks shared by the
+// kernels (kernels.hip) and the host side of the C-ABI (api.hip).
+//
+// Layout in HBM (one context):
+//   cells are stored in routing order (breadth-first from each outlet,
+//   domain after domain), so a cell and its L11 node share one index p.
+//   A parameter set s occupies the slice [s*sstride, s*sstride + n_cells)
+//   of every row; sstride = n_cells rounded up to 64 doubles.
+//   P[row][ld]   effective parameters          (ld = n_sets * sstride)
+//   S[row][ld]   states
+//   X[row][ld]   fluxes of the last step
+//   F[var][day][sstride]  forcing, shared by all parameter sets
+//   ring[R][ld]  L1 total runoff per step, consumed by the routing
+#pragma once
+
+#include <stdint.h>
+
+namespace mhmhip {
+
+constexpr int kMaxH = 6;
+
+// parameter rows (device).  Divisions and scalings the reference performs
+// every step on step-invariant operands are done once on the host
+// (bit-identical): DMAXS = degDayMax*c2TSTu, DNPS = degDayNoPre*c2TSTu,
+// K*S = c2TSTu/k*, and the latitude terms of extraterr_rad_approx.
+enum PRow {
+  PR_FS = 0, PR_SEALTHR, PR_JARVIS, PR_K0S, PR_K1S, PR_K2S, PR_KPS, PR_ALPHA, PR_KARST,
+  PR_UNSATTHR, PR_FASP, PR_HARSAM, PR_TANLAT, PR_SINLAT, PR_COSLAT, PR_DMAXS, PR_DNPS,
+  PR_INCR, PR_TTH, PR_NSCALAR
+};
+__host__ __device__ constexpr int pr_sat(int nh) { return PR_NSCALAR; }
+__host__ __device__ constexpr int pr_froots(int nh) { return PR_NSCALAR + nh; }
+__host__ __device__ constexpr int pr_fc(int nh) { return PR_NSCALAR + 2 * nh; }
+__host__ __device__ constexpr int pr_wp(int nh) { return PR_NSCALAR + 3 * nh; }
+__host__ __device__ constexpr int pr_exp(int nh) { return PR_NSCALAR + 4 * nh; }
+__host__ __device__ constexpr int pr_maxinter(int nh) { return PR_NSCALAR + 5 * nh; }
+__host__ __device__ constexpr int pr_petlai(int nh) { return PR_NSCALAR + 5 * nh + 12; }
+__host__ __device__ constexpr int pr_count(int nh) { return PR_NSCALAR + 5 * nh + 24; }
+
+enum SRow { SR_INTER = 0, SR_SNOW, SR_SEAL, SR_UNSAT, SR_SAT, SR_NSCALAR };
+__host__ __device__ constexpr int sr_count(int nh) { return SR_NSCALAR + nh; }
+
+enum XRow {
+  XR_PET = 0, XR_TEMP, XR_PREC, XR_THROUGHFALL, XR_AETCAN, XR_DEGDAY, XR_MELT, XR_PEFF,
+  XR_RAIN, XR_SNOW, XR_RUNOFFSEAL, XR_AETSEAL, XR_FAST, XR_SLOW, XR_PERC, XR_BASEFLOW,
+  XR_TOTAL, XR_NSCALAR
+};
+__host__ __device__ constexpr int xr_infil(int nh) { return XR_NSCALAR; }
+__host__ __device__ constexpr int xr_aet(int nh) { return XR_NSCALAR + nh; }
+__host__ __device__ constexpr int xr_count(int nh) { return XR_NSCALAR + 2 * nh; }
+
+// Per-step scalars on the device (host-evaluated from the step clock).
+struct StepDev {
+  double dr, tand, sind, cosd;  // extraterr_rad_approx day-of-year terms
+  int32_t day;                  // 0-based forcing column
+  int32_t month0;               // 0..11
+  int32_t isday;                // 6 < hour <= 18
+  int32_t init_sm;              // tt == 1 .and. .not. read_states
+  int32_t doy;
+  int32_t pad;
+};
+
+struct MonthTables {
+  double evap[12];
+  double fday_pet[12], fnight_pet[12];
+  double fday_prec[12], fnight_prec[12];
+  double fday_temp[12], fnight_temp[12];
+};
+
+struct CellArgs {
+  const double *P;
+  double *S;
+  double *X;
+  const double *F;
+  double *ring;            // nullptr: no routing input wanted
+  const StepDev *steps;    // device step table (fused kernel)
+  int64_t n_cells;
+  int64_t sstride;
+  int64_t ld;
+  int64_t forcing_days;
+  int64_t ring_len;
+  int32_t pet_case;
+  int32_t jarvis;
+  int32_t hourly;
+  int32_t pad;
+  double nT;               // steps per day as real (nTstepDay_dp)
+};
+
+// Routing node metadata, packed in 8 bytes per node.
+//   x: index of the first upstream node (children are contiguous, in the
+//      reference's routing rank)
+//   y: depth (bits 0..19) | nchild (20..23) | flags (24..28)
+enum NodeFlag {
+  NF_LINK = 1 << 24,    // node has an outgoing link (computes qTR)
+  NF_QOUT = 1 << 25,    // node adds its own qOUT to qTIN
+  NF_CUT = 1 << 26,     // downstream node is a non-headwater inflow gauge
+  NF_GAUGE = 1 << 27,   // node carries one or more gauges
+  NF_INFLOW = 1 << 28,  // node is an inflow gauge node
+};
+struct NodeMeta {
+  int32_t child;
+  int32_t packed;
+};
+__host__ __device__ inline int nm_depth(int32_t y) { return y & 0xFFFFF; }
+__host__ __device__ inline int nm_nchild(int32_t y) { return (y >> 20) & 0xF; }
+
+struct RouteArgs {
+  const NodeMeta *meta;    // [n_nodes]
+  const double *area;      // [n_nodes] km2
+  const double *C1;        // [ld]
+  const double *C2;        // [ld]
+  double *qTIN;            // [ld]   qTIN(:,IT1)
+  double *pub;             // [2][ld] qTR by step parity
+  double *qOUT;            // [ld]   last qOUT (reference L11_qOUT), may be null
+  const double *ring;      // [ring_len][ld]
+  double *gauge_out;       // [nsteps][n_sets][n_gauges] relative to t_base
+  const int32_t *gauge_node;  // [n_gauges] device node index
+  const int32_t *inflow_node; // [n_inflow] device node index
+  const int32_t *inflow_hw;   // [n_inflow]
+  const double *inflow_q;     // [nsteps][n_inflow] relative to t_base
+  int64_t n_nodes;
+  int64_t sstride;
+  int64_t ring_len;
+  int64_t t_lo, t_hi;      // active global steps [t_lo, t_hi)
+  int64_t t_base;          // global step of gauge_out/inflow_q row 0
+  int32_t n_sets;
+  int32_t n_gauges;
+  int32_t n_inflow;
+  int32_t dmax;            // maximum depth over the nodes this launch covers
+  double tst;              // HourSecs * timestep [s]
+};
+
+}  // namespace mhmhip

@@ -1,0 +1,35 @@
+// network.h -- breadth-first device layout of the packed L11 networks.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace mhmhip {
+
+struct LayoutIn {
+  int64_t n_nodes;
+  int32_t n_domains;
+  const int64_t *dom_node_start;  // n_domains + 1
+  const int32_t *n_outlets;       // per domain
+  const int32_t *netPerm;         // packed per-domain links, 1-based local
+  const int32_t *fromN;
+  const int32_t *toN;
+};
+
+struct LayoutOut {
+  std::vector<int64_t> pos;           // reference node (global 0-based) -> device index
+  std::vector<int64_t> node_at;       // device index -> reference node
+  std::vector<int32_t> child;         // NodeMeta.child
+  std::vector<int32_t> packed;        // NodeMeta.packed
+  std::vector<int64_t> link_of_node;  // device index -> packed link index (or -1)
+  std::vector<int32_t> dom_dmax;
+  int32_t dmax;
+  int64_t n_links_total;
+};
+
+int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err);
+
+}  // namespace mhmhip
