@@ -221,9 +221,11 @@ int mhmhip_routing_order(int64_t n_nodes, int64_t n_links, const int32_t *fromN,
  * src/mRM/mo_mrm_routing.f90:104-110); resets routing states to 0
  * (src/mRM/mo_mrm_init.f90:453-494). */
 int mhmhip_bind_network(mhmhip_ctx *ctx, const mhmhip_network *net);
-/* The five routing1 parameters of one parameter set (mhm_parameter.nml:193-201);
- * C1/C2 follow from reg_rout (src/mRM/mo_mrm_mpr.f90:61-119). */
-int mhmhip_set_routing_params(mhmhip_ctx *ctx, int32_t set, const double gamma[5]);
+/* The five routing1 parameters (mhm_parameter.nml:193-201) of one parameter
+ * set for one domain (domain < 0: every domain); C1/C2 follow from reg_rout
+ * (src/mRM/mo_mrm_mpr.f90:61-119) over that domain's links. */
+int mhmhip_set_routing_params(mhmhip_ctx *ctx, int32_t set, int32_t domain,
+                              const double gamma[5]);
 /* Inflow discharge [m3 s-1] of every inflow gauge for the next mrm step(s):
  * q[step][n_inflow] for nsteps steps (InflowDischarge). */
 int mhmhip_set_inflow(mhmhip_ctx *ctx, int32_t nsteps, const double *q);
@@ -247,6 +249,17 @@ int mhmhip_mrm_step(mhmhip_ctx *ctx, const mhmhip_step *st, double *gauge_out);
  * across time.  gauge_out (host, may be NULL) receives [step][set][gauge]. */
 int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
                const mhmhip_run_opts *opts, double *gauge_out);
+
+/* ---- measurement (no counterpart in the reference, which has only wall
+ * timers, mo_mhm_interface.F90:157-162) -------------------------------------
+ * With profiling on, every kernel launch is bracketed by HIP events on the
+ * context's stream; the device time, launch count and work units (cell-steps
+ * for the cell kernels, node-steps for the routing) accumulate per kernel. */
+enum mhmhip_kernel { MHMHIP_K_CELL_STEP = 0, MHMHIP_K_CELL_FUSED, MHMHIP_K_ROUTE_DIAG,
+                     MHMHIP_K_ROUTE_WG, MHMHIP_K_COUNT };
+int mhmhip_set_profiling(mhmhip_ctx *ctx, int32_t on); /* on != 0 also resets the totals */
+int mhmhip_get_profile(mhmhip_ctx *ctx, int32_t kernel, double *ms, int64_t *launches,
+                       double *units);
 
 #ifdef __cplusplus
 }

@@ -84,13 +84,16 @@ SIGNATURES = {
     "mhmhip_mhm_step": (C.c_int, [C.c_void_p, C.POINTER(Step)]),
     "mhmhip_routing_order": (C.c_int, [C.c_int64, C.c_int64, _ip, _ip, _ip, _ip]),
     "mhmhip_bind_network": (C.c_int, [C.c_void_p, C.POINTER(Network)]),
-    "mhmhip_set_routing_params": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "mhmhip_set_routing_params": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
     "mhmhip_set_inflow": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mhmhip_get_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mhmhip_set_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
     "mhmhip_mrm_step": (C.c_int, [C.c_void_p, C.POINTER(Step), _dp]),
     "mhmhip_run": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts), _dp]),
+    "mhmhip_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mhmhip_get_profile": (C.c_int, [C.c_void_p, C.c_int32, _dp, _lp, _dp]),
 }
+K_NAMES = ["cell_step", "cell_fused", "route_diag", "route_wg"]
 
 
 class MhmHipError(RuntimeError):

@@ -73,6 +73,19 @@ struct mhmhip_ctx {
   int64_t t_cell = 0, t_route = 0;  // global step counters
   int64_t device_bytes = 0;
 
+  // kernel timing (mhmhip_set_profiling)
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+    double units;
+  };
+  bool prof = false;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms[MHMHIP_K_COUNT] = {0, 0, 0, 0};
+  int64_t prof_n[MHMHIP_K_COUNT] = {0, 0, 0, 0};
+  double prof_units[MHMHIP_K_COUNT] = {0, 0, 0, 0};
+
   int fail(int code, const std::string &m) {
     err = m;
     return code;
@@ -130,7 +143,8 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
   }
   *out = nullptr;
   const mhmhip_config &c = *cfg;
-  if (c.n_cells < 1 || c.n_horizons < 1 || c.n_horizons > MHMHIP_MAX_HORIZONS || c.n_sets < 1 ||
+  if (c.n_cells < 1 || c.n_cells > ((int64_t)1 << 31) - 64 || c.n_horizons < 1 ||
+      c.n_horizons > MHMHIP_MAX_HORIZONS || c.n_sets < 1 || c.n_sets > 65535 ||
       c.timestep < 1 || c.timestep > 24 || 24 % c.timestep != 0 || c.process_soil < 1 ||
       c.process_soil > 4 || c.process_pet < -1 || c.process_pet > 1 || c.process_route < 0 ||
       c.process_route > 1 || c.n_forcing_days < 1 || c.n_domains < 1) {
@@ -252,6 +266,11 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_gauge_buf);
   (void)hipFree(x->d_steps);
   if (x->stage) (void)hipHostFree(x->stage);
+  for (auto &p : x->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (hipEvent_t e : x->ev_pool) (void)hipEventDestroy(e);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return MHMHIP_OK;
@@ -585,20 +604,68 @@ bool use_wg_routing(const mhmhip_ctx *x, int route_mode) {
   return max_domain_nodes(x) <= 16384;
 }
 
+hipError_t ev_get(mhmhip_ctx *x, hipEvent_t *e) {
+  if (!x->ev_pool.empty()) {
+    *e = x->ev_pool.back();
+    x->ev_pool.pop_back();
+    return hipSuccess;
+  }
+  return hipEventCreate(e);
+}
+
+// Launches f() on the context stream; with profiling on, brackets it with events.
+template <typename Fn>
+hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
+  if (!x->prof) return f();
+  mhmhip_ctx::Pending p{kind, nullptr, nullptr, units};
+  hipError_t e;
+  if ((e = ev_get(x, &p.a)) != hipSuccess || (e = ev_get(x, &p.b)) != hipSuccess) return e;
+  if ((e = hipEventRecord(p.a, x->stream)) != hipSuccess) return e;
+  if ((e = f()) != hipSuccess) return e;
+  if ((e = hipEventRecord(p.b, x->stream)) != hipSuccess) return e;
+  x->pending.push_back(p);
+  return hipSuccess;
+}
+
+// After a stream join: fold the recorded event pairs into the totals.
+int prof_collect(mhmhip_ctx *x) {
+  for (auto &p : x->pending) {
+    float ms = 0.f;
+    CHECK_HIP(x, hipEventElapsedTime(&ms, p.a, p.b), "profile");
+    x->prof_ms[p.kind] += ms;
+    x->prof_n[p.kind] += 1;
+    x->prof_units[p.kind] += p.units;
+    x->ev_pool.push_back(p.a);
+    x->ev_pool.push_back(p.b);
+  }
+  x->pending.clear();
+  return MHMHIP_OK;
+}
+
 // routing diagonals [d_lo, d_hi) (tiles) or the drain after t_hi
 int route_diagonals(mhmhip_ctx *x, const RouteArgs &r, int64_t d_lo, int64_t d_hi, bool drain,
                     bool wg) {
   if (wg) {
-    CHECK_HIP(x,
-              launch_route_wg(r, x->d_dom_start, x->d_dom_dmax, x->n_domains, d_lo, d_hi,
-                              drain ? 1 : 0, 256, x->stream),
+    CHECK_HIP(x, timed(x, MHMHIP_K_ROUTE_WG, 0.0, [&] {
+                return launch_route_wg(r, x->d_dom_start, x->d_dom_dmax, x->n_domains, d_lo,
+                                       d_hi, drain ? 1 : 0, 256, x->stream);
+              }),
               "route_wg");
     return MHMHIP_OK;
   }
   const int64_t end = drain ? d_lo + x->lay.dmax : d_hi;
   for (int64_t d = d_lo; d < end; ++d)
-    CHECK_HIP(x, launch_route_diag(r, d, x->stream), "route_diag");
+    CHECK_HIP(x, timed(x, MHMHIP_K_ROUTE_DIAG, 0.0,
+                       [&] { return launch_route_diag(r, d, x->stream); }),
+              "route_diag");
   return MHMHIP_OK;
+}
+
+// node-steps routed, credited to the routing kernel that did them
+void credit_route(mhmhip_ctx *x, bool wg, int64_t nsteps) {
+  if (x->prof)
+    x->prof_units[wg ? MHMHIP_K_ROUTE_WG : MHMHIP_K_ROUTE_DIAG] +=
+        (double)x->n_cells * x->n_sets * (double)nsteps;
 }
 
 }  // namespace
@@ -616,7 +683,9 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
     CHECK_RC(ensure_ring(x, 2));
   }
   CellArgs a = cell_args(x, route);
-  CHECK_HIP(x, launch_cell_step(x->nh, a, d, x->mt, x->t_cell, 256, x->n_sets, x->stream),
+  CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
+              return launch_cell_step(x->nh, a, d, x->mt, x->t_cell, 256, x->n_sets, x->stream);
+            }),
             "cell_step");
   x->t_cell++;
   if (!route) x->t_route = x->t_cell;
@@ -770,15 +839,17 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->lay = std::move(lay);
   x->net_bound = true;
   x->t_route = x->t_cell;
-  x->ring_len = std::min<int64_t>(x->ring_len, 0);
-  x->dfree(x->d_ring, 0);
+  x->dfree(x->d_ring, x->ring_len * x->ld);
+  x->ring_len = 0;
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "bind_network");
   return MHMHIP_OK;
 }
 
-extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, const double *p) {
+extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, int32_t domain,
+                                         const double *p) {
   if (!x) return MHMHIP_EINVAL;
-  if (!p || set < 0 || set >= x->n_sets) return x->fail(MHMHIP_EINVAL, "set_routing_params: bad set");
+  if (!p || set < 0 || set >= x->n_sets || domain >= x->n_domains)
+    return x->fail(MHMHIP_EINVAL, "set_routing_params: bad set or domain");
   if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_params: bind a network first");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_params");
   // reg_rout (mo_mrm_mpr.f90:61-119) per domain, TS = real(timestep) [h].  The
@@ -786,9 +857,17 @@ extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, const doubl
   // evaluating it once gives the same C1, C2.
   const double TS = (double)x->cfg.timestep;
   const int64_t nLtot = x->dom_link_start[x->n_domains];
+  const int64_t off = (int64_t)set * x->sstride;
+  // start from the set's current C1/C2 so other domains keep theirs
+  std::vector<double> n1(x->sstride, 0.0), n2(x->sstride, 0.0);
+  CHECK_HIP(x, sync_copy(x, n1.data(), x->d_C1 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C1");
+  CHECK_HIP(x, sync_copy(x, n2.data(), x->d_C2 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C2");
   std::vector<double> c1(std::max<int64_t>(nLtot, 1), 0.0), c2(std::max<int64_t>(nLtot, 1), 0.0);
+  std::vector<char> touched(std::max<int64_t>(nLtot, 1), 0);
   for (int d = 0; d < x->n_domains; ++d) {
+    if (domain >= 0 && d != domain) continue;
     const int64_t l0 = x->dom_link_start[d], l1 = x->dom_link_start[d + 1];
+    for (int64_t i = l0; i < l1; ++i) touched[i] = 1;
     if (l1 <= l0) continue;
     double ssMax = x->h_slope[l0];
     for (int64_t i = l0 + 1; i < l1; ++i)
@@ -804,15 +883,13 @@ extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, const doubl
       c2[i] = 1.0 - c1[i] * K / TS;
     }
   }
-  std::vector<double> n1(x->sstride, 0.0), n2(x->sstride, 0.0);
   for (int64_t q = 0; q < x->n_cells; ++q) {
     const int64_t l = x->lay.link_of_node[q];
-    if (l >= 0) {
+    if (l >= 0 && touched[l]) {
       n1[q] = c1[l];
       n2[q] = c2[l];
     }
   }
-  const int64_t off = (int64_t)set * x->sstride;
   CHECK_HIP(x, sync_copy(x, x->d_C1 + off, n1.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C1");
   CHECK_HIP(x, sync_copy(x, x->d_C2 + off, n2.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C2");
   return MHMHIP_OK;
@@ -895,6 +972,7 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   const bool wg = use_wg_routing(x, 0);
   CHECK_RC(route_diagonals(x, r, t, t + 1, false, wg));
   CHECK_RC(route_diagonals(x, r, t + 1, 0, true, wg));
+  credit_route(x, wg, 1);
   x->t_route++;
   if (gauge_out && x->n_gauges > 0) {
     CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf,
@@ -903,7 +981,7 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
               "download gauges");
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "mrm_step");
-  return MHMHIP_OK;
+  return prof_collect(x);
 }
 
 extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
@@ -946,15 +1024,24 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   for (int32_t k = 0; k < nsteps; k += S) {
     const int n = std::min<int32_t>(S, nsteps - k);
     if (o.mode == MHMHIP_RUN_PLAIN) {
-      CHECK_HIP(x, launch_cell_step(x->nh, a, hs[k], x->mt, t_start + k, block, x->n_sets, x->stream),
+      CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
+                  return launch_cell_step(x->nh, a, hs[k], x->mt, t_start + k, block, x->n_sets,
+                                          x->stream);
+                }),
                 "cell_step");
     } else {
-      CHECK_HIP(x, launch_cell_fused(x->nh, a, x->mt, t_start + k, n, k, block, x->n_sets, x->stream),
+      CHECK_HIP(x, timed(x, MHMHIP_K_CELL_FUSED, (double)x->n_cells * x->n_sets * n, [&] {
+                  return launch_cell_fused(x->nh, a, x->mt, t_start + k, n, k, block, x->n_sets,
+                                           x->stream);
+                }),
                 "cell_fused");
     }
     if (route) CHECK_RC(route_diagonals(x, r, t_start + k, t_start + k + n, false, wg));
   }
-  if (route) CHECK_RC(route_diagonals(x, r, t_start + nsteps, 0, true, wg));
+  if (route) {
+    CHECK_RC(route_diagonals(x, r, t_start + nsteps, 0, true, wg));
+    credit_route(x, wg, nsteps);
+  }
   x->t_cell += nsteps;
   x->t_route = x->t_cell;
   if (route && gauge_out && x->n_gauges > 0) {
@@ -964,5 +1051,30 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
               "download gauges");
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "run");
+  return prof_collect(x);
+}
+
+extern "C" int mhmhip_set_profiling(mhmhip_ctx *x, int32_t on) {
+  if (!x) return MHMHIP_EINVAL;
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_profiling");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_profiling");
+  CHECK_RC(prof_collect(x));
+  x->prof = on != 0;
+  if (x->prof)
+    for (int k = 0; k < MHMHIP_K_COUNT; ++k) {
+      x->prof_ms[k] = 0.0;
+      x->prof_n[k] = 0;
+      x->prof_units[k] = 0.0;
+    }
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_get_profile(mhmhip_ctx *x, int32_t kernel, double *ms, int64_t *launches,
+                                  double *units) {
+  if (!x) return MHMHIP_EINVAL;
+  if (kernel < 0 || kernel >= MHMHIP_K_COUNT) return x->fail(MHMHIP_EINVAL, "get_profile: bad kernel");
+  if (ms) *ms = x->prof_ms[kernel];
+  if (launches) *launches = x->prof_n[kernel];
+  if (units) *units = x->prof_units[kernel];
   return MHMHIP_OK;
 }

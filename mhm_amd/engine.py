@@ -1,0 +1,225 @@
+"""Host-side owner of one device context (include/mhmhip.h).
+
+A `Model` packs one or more domains (basins) and one or more parameter sets
+into one context: cells of all domains are concatenated like the reference's
+module arrays, parameter sets run side by side on the shared forcing.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from . import synth
+
+
+def _fortran_cols(a):
+    """(n,) or (n, k) array -> C-contiguous (k, n) block = Fortran X(n, k)."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        return np.ascontiguousarray(a[None, :]), 1
+    return np.ascontiguousarray(a.T), a.shape[1]
+
+
+class Model:
+    def __init__(self, domains, n_sets=1, timestep=1, process_soil=1, process_pet=1,
+                 route=True, n_forcing_days=1, hourly_forcing=False, device=0,
+                 evap_coeff=synth.EVAP_COEFF, fnight_prec=synth.FNIGHT_PREC,
+                 fnight_pet=synth.FNIGHT_PET, fnight_temp=synth.FNIGHT_TEMP):
+        if isinstance(domains, synth.Domain):
+            domains = [domains]
+        self.domains = list(domains)
+        nh = {d.horizon_depth.shape[0] for d in self.domains}
+        if len(nh) != 1:
+            raise ValueError("all packed domains need the same number of horizons")
+        self.n_horizons = nh.pop()
+        self.n_sets = int(n_sets)
+        self.timestep = int(timestep)
+        self.cell_start = np.zeros(len(self.domains) + 1, np.int64)
+        self.cell_start[1:] = np.cumsum([d.n_cells for d in self.domains])
+        self.n_cells = int(self.cell_start[-1])
+        cfg = N.Config()
+        cfg.n_cells = self.n_cells
+        cfg.n_domains = len(self.domains)
+        cfg.n_horizons = self.n_horizons
+        cfg.n_sets = self.n_sets
+        cfg.timestep = self.timestep
+        cfg.process_soil = process_soil
+        cfg.process_pet = process_pet
+        cfg.process_route = 1 if route else 0
+        cfg.hourly_forcing = 1 if hourly_forcing else 0
+        cfg.n_forcing_days = int(n_forcing_days)
+        cfg.device = device
+        cfg.domain_cell_start = N.lptr(self.cell_start)
+        tables = {"evap_coeff": evap_coeff, "fnight_prec": fnight_prec,
+                  "fnight_pet": fnight_pet, "fnight_temp": fnight_temp}
+        for name, vals in tables.items():
+            arr = getattr(cfg, name)
+            for m in range(12):
+                arr[m] = float(vals[m])
+        self.n_forcing_days = int(n_forcing_days)
+        self.route = bool(route)
+        self._ctx = C.c_void_p()
+        N.check(N.lib().mhmhip_create(C.byref(cfg), C.byref(self._ctx)), None)
+        self.n_gauges = 0
+
+    # -- lifecycle -------------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            N.lib().mhmhip_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        return N.check(rc, self._ctx)
+
+    @property
+    def device_bytes(self):
+        return int(N.lib().mhmhip_device_bytes(self._ctx))
+
+    def sync(self):
+        self._chk(N.lib().mhmhip_sync(self._ctx))
+
+    # -- cell data -------------------------------------------------------------
+    def set_var(self, name, values, set_=0):
+        """One parameter or state field over all packed cells ((n,), (n, nH) or (n, 12))."""
+        var = N.P[name] if name in N.P else N.S[name]
+        blk, ncols = _fortran_cols(values)
+        if blk.shape[1] != self.n_cells:
+            raise ValueError(f"{name}: expected {self.n_cells} cells, got {blk.shape[1]}")
+        self._chk(N.lib().mhmhip_set_cell_var(self._ctx, var, set_, N.dptr(blk), self.n_cells,
+                                              ncols))
+
+    def set_params(self, params, set_=0):
+        for k, v in params.items():
+            self.set_var(k, v, set_)
+
+    def get(self, name, set_=0):
+        var = N.S[name] if name in N.S else N.X[name]
+        ncols = self.n_horizons if name in N.HORIZON_VARS else 1
+        out = np.zeros((ncols, self.n_cells))
+        self._chk(N.lib().mhmhip_get_cell_var(self._ctx, var, set_, N.dptr(out), self.n_cells,
+                                              ncols))
+        return out[0] if ncols == 1 else out.T.copy()
+
+    def default_states(self):
+        hd = np.ascontiguousarray(self.domains[0].horizon_depth, dtype=np.float64)
+        self._chk(N.lib().mhmhip_default_states(self._ctx, N.dptr(hd)))
+
+    def set_forcing(self, name, data, day0=1):
+        """data: (n_days, n_cells) = Fortran L1_x(n_cells, n_days) of all packed cells."""
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        self._chk(N.lib().mhmhip_set_forcing(self._ctx, N.F[name], day0, data.shape[0],
+                                             N.dptr(data), data.shape[1]))
+
+    def set_domain_forcing(self, day0=1):
+        """Upload every domain's own forcing (synth.make_forcing) for all packed cells."""
+        for name in ("PRE", "TAVG", "TMIN", "TMAX"):
+            if all(name in d.forcing for d in self.domains):
+                data = np.concatenate([d.forcing[name] for d in self.domains], axis=1)
+                self.set_forcing(name, data, day0)
+
+    # -- network -----------------------------------------------------------------
+    def bind_network(self, inflow=None):
+        doms = self.domains
+        cat = lambda f, dt: np.ascontiguousarray(np.concatenate([f(d) for d in doms]).astype(dt))
+        self._net_arrays = arrs = dict(
+            start=np.ascontiguousarray(self.cell_start),
+            nout=np.array([d.n_outlets for d in doms], np.int32),
+            netPerm=cat(lambda d: d.netPerm, np.int32), fromN=cat(lambda d: d.fromN, np.int32),
+            toN=cat(lambda d: d.toN, np.int32), length=cat(lambda d: d.length, np.float64),
+            slope=cat(lambda d: d.slope, np.float64), fFPimp=cat(lambda d: d.fFPimp, np.float64),
+            area=cat(lambda d: d.area_km2, np.float64),
+            l1l11=cat(lambda d: np.arange(1, d.n_cells + 1), np.int32),
+            gdom=cat(lambda d: np.full(d.gauge_nodes.shape[0], doms.index(d)), np.int32),
+            gnode=cat(lambda d: d.gauge_nodes, np.int32))
+        net = N.Network()
+        net.n_nodes = self.n_cells
+        net.domain_node_start = N.lptr(arrs["start"])
+        net.n_outlets = N.iptr(arrs["nout"])
+        net.netPerm = N.iptr(arrs["netPerm"])
+        net.fromN = N.iptr(arrs["fromN"])
+        net.toN = N.iptr(arrs["toN"])
+        net.length = N.dptr(arrs["length"])
+        net.slope = N.dptr(arrs["slope"])
+        net.fFPimp = N.dptr(arrs["fFPimp"])
+        net.L1_area = N.dptr(arrs["area"])
+        net.L1_L11_Id = N.iptr(arrs["l1l11"])
+        net.n_gauges = arrs["gnode"].shape[0]
+        net.gauge_domain = N.iptr(arrs["gdom"])
+        net.gauge_node = N.iptr(arrs["gnode"])
+        if inflow:
+            arrs["idom"] = np.ascontiguousarray(inflow["domain"], np.int32)
+            arrs["inode"] = np.ascontiguousarray(inflow["node"], np.int32)
+            arrs["ihw"] = np.ascontiguousarray(inflow["headwater"], np.int32)
+            net.n_inflow = arrs["inode"].shape[0]
+            net.inflow_domain = N.iptr(arrs["idom"])
+            net.inflow_node = N.iptr(arrs["inode"])
+            net.inflow_headwater = N.iptr(arrs["ihw"])
+        self._chk(N.lib().mhmhip_bind_network(self._ctx, C.byref(net)))
+        self.n_gauges = net.n_gauges
+        self.n_inflow = net.n_inflow
+
+    def set_routing_params(self, gamma5, set_=0, domain=-1):
+        g = np.ascontiguousarray(gamma5, dtype=np.float64)
+        self._chk(N.lib().mhmhip_set_routing_params(self._ctx, set_, domain, N.dptr(g)))
+
+    def set_inflow(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self._chk(N.lib().mhmhip_set_inflow(self._ctx, q.shape[0], N.dptr(q)))
+
+    def routing_state(self, set_=0):
+        n = self.n_cells
+        nl = self._net_arrays["netPerm"].shape[0]
+        out = {k: np.zeros(n) for k in ("qOUT", "qTIN", "qTR", "Qmod")}
+        out["C1"] = np.zeros(max(nl, 1))
+        out["C2"] = np.zeros(max(nl, 1))
+        self._chk(N.lib().mhmhip_get_routing_state(self._ctx, set_, *[N.dptr(out[k]) for k in
+                                                   ("qOUT", "qTIN", "qTR", "Qmod", "C1", "C2")]))
+        return out
+
+    def set_routing_state(self, qTIN, qTR, set_=0):
+        a = np.ascontiguousarray(qTIN, dtype=np.float64)
+        b = np.ascontiguousarray(qTR, dtype=np.float64)
+        self._chk(N.lib().mhmhip_set_routing_state(self._ctx, set_, N.dptr(a), N.dptr(b)))
+
+    # -- stepping ----------------------------------------------------------------
+    def mhm_step(self, step):
+        self._chk(N.lib().mhmhip_mhm_step(self._ctx, C.byref(step)))
+
+    def mrm_step(self, step):
+        g = np.zeros((self.n_sets, max(self.n_gauges, 1)))
+        self._chk(N.lib().mhmhip_mrm_step(self._ctx, C.byref(step), N.dptr(g)))
+        return g[:, :self.n_gauges]
+
+    def run(self, steps, mode=N.RUN_FUSED, steps_per_launch=0, block=0, route_mode=0,
+            want_gauges=True):
+        """steps: ctypes array of mhmhip_step.  Returns gauges (n_steps, n_sets, n_gauges)."""
+        n = len(steps)
+        if not isinstance(steps, C.Array):
+            steps = (N.Step * n)(*steps)
+        opts = N.RunOpts()
+        opts.mode = mode
+        opts.steps_per_launch = steps_per_launch
+        opts.block = block
+        opts.route_mode = route_mode
+        g = np.zeros((n, self.n_sets, max(self.n_gauges, 1))) if want_gauges else None
+        self._chk(N.lib().mhmhip_run(self._ctx, steps, n, C.byref(opts), N.dptr(g)))
+        return None if g is None else g[:, :, :self.n_gauges]
+
+    # -- measurement ---------------------------------------------------------------
+    def set_profiling(self, on=True):
+        self._chk(N.lib().mhmhip_set_profiling(self._ctx, 1 if on else 0))
+
+    def profile(self):
+        """{kernel: (device ms, launches, work units)} accumulated since set_profiling(True)."""
+        out = {}
+        for k, name in enumerate(N.K_NAMES):
+            ms, n, u = C.c_double(), C.c_int64(), C.c_double()
+            self._chk(N.lib().mhmhip_get_profile(self._ctx, k, C.byref(ms), C.byref(n), C.byref(u)))
+            out[name] = (ms.value, n.value, u.value)
+        return out

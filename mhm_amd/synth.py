@@ -116,8 +116,11 @@ def make_domain(nrows, ncols, seed, cellsize=1000.0, horizon_depth=HORIZON_DEPTH
     cls = rng.integers(0, 3, size=(n, 4))
     fractions = np.stack([(cls == c).mean(axis=1) for c in range(3)])
     texture = np.clip(_fbm(rng, nrows, ncols, 0.5).ravel(order="F"), 0.0, 0.999999)
-    gy, gx = np.gradient(z, cellsize)
-    aspect = (np.degrees(np.arctan2(-gx, gy)) % 360.0).ravel(order="F")
+    if nrows >= 2 and ncols >= 2:
+        gy, gx = np.gradient(z, cellsize)
+        aspect = (np.degrees(np.arctan2(-gx, gy)) % 360.0).ravel(order="F")
+    else:
+        aspect = np.full(n, 180.0)
     lat = lat0 + (nrows / 2.0 - ii.ravel(order="F")) * cellsize / 111000.0
     area = np.full(n, cellsize * cellsize * 1.0e-6)
     return Domain(nrows=nrows, ncols=ncols, cellsize=cellsize, seed=seed,

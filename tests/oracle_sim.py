@@ -1,0 +1,222 @@
+"""Drives the oracle (CPU restatement) through the same time loop as the
+HIP path: per domain and parameter set, the meteo getters + mHM() step and
+one mRM_routing call per step, in the order of
+src/mHM/mo_mhm_interface_run.f90:341-638.  Test infrastructure only.
+"""
+import ctypes as C
+
+import numpy as np
+
+import oracle_lib as O
+
+# ABI variable name -> oracle parameter row (scalar rows)
+P_ROW = {"FSEALED": 0, "DEGDAY_INC": 1, "DEGDAY_MAX": 2, "DEGDAY_NOPRE": 3, "TEMP_THRESH": 4,
+         "SEALED_THRESH": 5, "JARVIS_C1": 6, "KFASTFLOW": 7, "KSLOWFLOW": 8, "KBASEFLOW": 9,
+         "KPERCO": 10, "ALPHA": 11, "KARSTLOSS": 12, "UNSAT_THRESH": 13, "FASP": 14,
+         "HARSAMCOEFF": 15, "LATITUDE": 16}
+H_BLOCK = {"SOILMOIST_SAT": 0, "FROOTS": 1, "SOILMOIST_FC": 2, "WILTINGPOINT": 3,
+           "SOILMOIST_EXP": 4}
+S_ROW = {"INTER": 0, "SNOWPACK": 1, "SEALSTW": 2, "UNSATSTW": 3, "SATSTW": 4}
+X_ROW = {"PET_CALC": 0, "TEMP_CALC": 1, "PREC_CALC": 2, "THROUGHFALL": 3, "AETCANOPY": 4,
+         "DEGDAY": 5, "MELT": 6, "PREEFFECT": 7, "RAIN": 8, "SNOW": 9, "RUNOFFSEAL": 10,
+         "AETSEALED": 11, "FASTRUNOFF": 12, "SLOWRUNOFF": 13, "PERCOL": 14, "BASEFLOW": 15,
+         "TOTAL_RUNOFF": 16}
+F_IDX = {"PRE": 0, "TAVG": 1, "TMIN": 2, "TMAX": 3, "PET": 4}
+
+
+class _DomSet:
+    pass
+
+
+class OracleSim:
+    def __init__(self, domains, n_sets=1, timestep=1, process_soil=1, process_pet=1,
+                 n_forcing_days=1, hourly_forcing=False, route=True, evap_coeff=None,
+                 fnight_prec=None, fnight_pet=None, fnight_temp=None, nthreads=0):
+        from mhm_amd import synth
+        self.domains = list(domains)
+        self.n_sets = n_sets
+        self.nh = self.domains[0].horizon_depth.shape[0]
+        self.timestep = timestep
+        self.route = route
+        self.nthreads = nthreads
+        self.n_forcing_days = n_forcing_days
+        self.cfgs = []
+        self.ds = []
+        for d in self.domains:
+            n = d.n_cells
+            cfg = O.CellsCfg()
+            cfg.n_cells = n
+            cfg.ld = n
+            cfg.n_h = self.nh
+            cfg.process_soil = process_soil
+            cfg.process_pet = process_pet
+            cfg.hourly_forcing = 1 if hourly_forcing else 0
+            cfg.c2TSTu = float(timestep) / 24.0
+            cfg.nTstepDay = float(24 // timestep)
+            cfg.forcing_ld = n
+            cfg.forcing_days = n_forcing_days
+            tabs = dict(evap_coeff=synth.EVAP_COEFF if evap_coeff is None else evap_coeff,
+                        fnight_prec=synth.FNIGHT_PREC if fnight_prec is None else fnight_prec,
+                        fnight_pet=synth.FNIGHT_PET if fnight_pet is None else fnight_pet,
+                        fnight_temp=synth.FNIGHT_TEMP if fnight_temp is None else fnight_temp)
+            for k, v in tabs.items():
+                arr = getattr(cfg, k)
+                for m in range(12):
+                    arr[m] = float(v[m])
+            self.cfgs.append(cfg)
+            sets = []
+            for s in range(n_sets):
+                o = _DomSet()
+                o.P = np.zeros((O.lib().or_n_param_rows(self.nh), n))
+                o.S = np.zeros((O.lib().or_n_state_rows(self.nh), n))
+                o.X = np.zeros((O.lib().or_n_flux_rows(self.nh), n))
+                nl = d.n_links
+                o.C1 = np.zeros(max(nl, 1))
+                o.C2 = np.zeros(max(nl, 1))
+                o.qOUT = np.zeros(n)
+                o.qTIN = np.zeros(2 * n)
+                o.qTR = np.zeros(2 * n)
+                o.Qmod = np.zeros(n)
+                o.gamma = np.zeros(5)
+                sets.append(o)
+            self.ds.append(sets)
+            d._oracle_F = np.zeros((5, n_forcing_days, n))
+        self.gauge_series = []
+
+    # -- data -------------------------------------------------------------------------
+    def set_params(self, params, set_=0, domain=None):
+        """params: arrays over the cells of `domain` (or all packed cells if None)."""
+        doms = range(len(self.domains)) if domain is None else [domain]
+        off = 0
+        starts = np.cumsum([0] + [d.n_cells for d in self.domains])
+        for di in doms:
+            o = self.ds[di][set_]
+            lo, hi = (starts[di], starts[di + 1]) if domain is None else (0, self.domains[di].n_cells)
+            for k, v in params.items():
+                v = np.asarray(v, dtype=np.float64)
+                if k in P_ROW:
+                    o.P[P_ROW[k]] = v[lo:hi]
+                elif k in H_BLOCK:
+                    for h in range(self.nh):
+                        o.P[17 + H_BLOCK[k] * self.nh + h] = v[lo:hi, h]
+                elif k == "MAXINTER":
+                    for m in range(12):
+                        o.P[17 + 5 * self.nh + m] = v[lo:hi, m]
+                elif k == "PETLAICOR":
+                    for m in range(12):
+                        o.P[17 + 5 * self.nh + 12 + m] = v[lo:hi, m]
+                else:
+                    raise KeyError(k)
+
+    def default_states(self):
+        for d, sets in zip(self.domains, self.ds):
+            hd = np.ascontiguousarray(d.horizon_depth)
+            for o in sets:
+                O.lib().or_default_states(d.n_cells, d.n_cells, self.nh, O.dptr(hd), O.dptr(o.S),
+                                          O.dptr(o.X))
+
+    def set_domain_forcing(self, day0=1):
+        for d in self.domains:
+            for k, v in d.forcing.items():
+                d._oracle_F[F_IDX[k], day0 - 1:day0 - 1 + v.shape[0]] = v
+
+    def set_forcing_domain(self, di, name, data, day0=1):
+        d = self.domains[di]
+        d._oracle_F[F_IDX[name], day0 - 1:day0 - 1 + data.shape[0]] = data
+
+    def set_routing_params(self, gamma5, set_=0, domain=-1):
+        for di in range(len(self.domains)):
+            if domain < 0 or di == domain:
+                self.ds[di][set_].gamma[:] = gamma5
+
+    # -- stepping ----------------------------------------------------------------------
+    def step(self, st):
+        """One do_time_step for every domain and set; returns gauges [set][gauge]."""
+        si = O.StepInfo(int(st["tt"]), int(st["month"]), int(st["hour"]), int(st["doy"]),
+                        int(st["iMeteoTS"]), int(st["read_states"]))
+        gauges = [[] for _ in range(self.n_sets)]
+        for di, (d, cfg, sets) in enumerate(zip(self.domains, self.cfgs, self.ds)):
+            F = np.ascontiguousarray(d._oracle_F)
+            for s, o in enumerate(sets):
+                O.lib().or_mhm_step(C.byref(cfg), C.byref(si), O.dptr(o.P), O.dptr(F), O.dptr(o.S),
+                                    O.dptr(o.X), self.nthreads)
+                if not self.route:
+                    continue
+                rc = O.RouteCfg()
+                rc.n_nodes = d.n_cells
+                rc.n_outlets = d.n_outlets
+                rc.timestep = self.timestep
+                rc.process_case = 1
+                rc.read_states = int(st["read_states"])
+                rc.map_flag = 1
+                rc.n_inflow = 0
+                rc.n_gauges = d.gauge_nodes.shape[0]
+                n = d.n_cells
+                l1 = np.arange(1, n + 1, dtype=np.int32)
+                g = np.zeros(max(rc.n_gauges, 1))
+                tot = np.ascontiguousarray(o.X[16])
+                inf = getattr(d, "_oracle_inflow", None)
+                if inf is not None:
+                    rc.n_inflow = inf["node"].shape[0]
+                    iidx = np.arange(1, rc.n_inflow + 1, dtype=np.int32)
+                    ihw = np.ascontiguousarray(inf["headwater"], np.int32)
+                    inode = np.ascontiguousarray(inf["node"], np.int32)
+                    iq = np.ascontiguousarray(inf["q"][int(st["tt"]) - 1 - inf["tt0"]])
+                else:
+                    iidx = ihw = inode = iq = None
+                O.lib().or_mrm_routing(C.byref(rc), O.dptr(o.gamma), n, O.dptr(tot),
+                                       O.dptr(np.ascontiguousarray(d.area_km2)), O.iptr(l1),
+                                       O.dptr(np.ascontiguousarray(d.area_km2)), O.iptr(l1),
+                                       O.iptr(d.netPerm), O.iptr(d.fromN), O.iptr(d.toN),
+                                       O.dptr(d.length), O.dptr(d.slope), O.dptr(d.fFPimp),
+                                       O.iptr(iidx), O.iptr(ihw), O.iptr(inode), O.dptr(iq),
+                                       O.iptr(np.ascontiguousarray(d.gauge_nodes)), O.dptr(o.C1),
+                                       O.dptr(o.C2), O.dptr(o.qOUT), O.dptr(o.qTIN), O.dptr(o.qTR),
+                                       O.dptr(o.Qmod), O.dptr(g))
+                gauges[s].extend(g[:rc.n_gauges].tolist())
+        return np.array(gauges)
+
+    # -- views ---------------------------------------------------------------------------
+    def get(self, name, set_=0):
+        parts = []
+        for sets in self.ds:
+            o = sets[set_]
+            if name in S_ROW:
+                parts.append(o.S[S_ROW[name]])
+            elif name == "SOILMOIST":
+                parts.append(o.S[5:5 + self.nh].T)
+            elif name in X_ROW:
+                parts.append(o.X[X_ROW[name]])
+            elif name == "INFILSOIL":
+                parts.append(o.X[17:17 + self.nh].T)
+            elif name == "AETSOIL":
+                parts.append(o.X[17 + self.nh:17 + 2 * self.nh].T)
+            else:
+                raise KeyError(name)
+        return np.concatenate(parts, axis=0)
+
+    def routing_state(self, set_=0):
+        out = {}
+        for k in ("qOUT", "Qmod"):
+            out[k] = np.concatenate([sets[set_].__dict__[k] for sets in self.ds])
+        out["qTIN"] = np.concatenate([sets[set_].qTIN[:d.n_cells] for d, sets in zip(self.domains, self.ds)])
+        out["qTR"] = np.concatenate([sets[set_].qTR[:d.n_cells] for d, sets in zip(self.domains, self.ds)])
+        out["C1"] = np.concatenate([sets[set_].C1[:d.n_links] for d, sets in zip(self.domains, self.ds)])
+        out["C2"] = np.concatenate([sets[set_].C2[:d.n_links] for d, sets in zip(self.domains, self.ds)])
+        return out
+
+
+STATE_NAMES = ["INTER", "SNOWPACK", "SEALSTW", "UNSATSTW", "SATSTW", "SOILMOIST"]
+FLUX_NAMES = list(X_ROW) + ["INFILSOIL", "AETSOIL"]
+
+
+def rel_close(a, b, rtol=1e-10, atol=1e-12):
+    """|a - b| <= rtol * max(|a|, |b|) + atol, elementwise; returns (ok, worst)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    err = np.abs(a - b)
+    lim = rtol * np.maximum(np.abs(a), np.abs(b)) + atol
+    both_nan = np.isnan(a) & np.isnan(b)
+    bad = (err > lim) & ~both_nan
+    worst = float(np.max(np.where(lim > 0, err / np.where(lim > 0, lim, 1), 0))) if a.size else 0.0
+    return (not bad.any()), worst

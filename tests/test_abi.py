@@ -1,0 +1,54 @@
+"""The C-ABI library loads and exports every entry point include/mhmhip.h
+declares; host-only entry points behave as documented without a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from mhm_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "mhmhip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(?:int|int64_t|const char \*)\s*\**\s*(mhmhip_\w+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "mhmhip_create" in names and "mhmhip_run" in names
+    assert len(names) == len(N.SIGNATURES), sorted(set(names) ^ set(N.SIGNATURES))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(N.HIP_LIB)
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert set(_declared()) == set(N.SIGNATURES)
+
+
+def test_create_rejects_bad_config_without_touching_a_device():
+    L = N.lib()
+    cfg = N.Config()
+    cfg.n_cells = 0  # invalid
+    ctx = C.c_void_p()
+    assert L.mhmhip_create(C.byref(cfg), C.byref(ctx)) == N.EINVAL
+    assert b"out of range" in L.mhmhip_last_error(None)
+    cfg.n_cells, cfg.n_horizons, cfg.n_sets, cfg.timestep = 10, 3, 1, 5  # 24 % 5 != 0
+    cfg.process_soil, cfg.process_pet, cfg.n_forcing_days, cfg.n_domains = 1, 1, 1, 1
+    assert L.mhmhip_create(C.byref(cfg), C.byref(ctx)) == N.EINVAL
+
+
+def test_routing_order_rejects_cycles_and_double_outlinks():
+    fr = np.array([1, 2, 3, 0], np.int32)
+    to = np.array([2, 3, 1, 0], np.int32)  # 1->2->3->1
+    ro = np.zeros(4, np.int32)
+    pm = np.zeros(4, np.int32)
+    assert N.lib().mhmhip_routing_order(4, 3, N.iptr(fr), N.iptr(to), N.iptr(ro), N.iptr(pm)) == N.EINVAL
+    fr = np.array([1, 1, 0], np.int32)
+    to = np.array([2, 3, 0], np.int32)
+    assert N.lib().mhmhip_routing_order(3, 2, N.iptr(fr), N.iptr(to), N.iptr(ro[:3]), N.iptr(pm[:3])) == N.EINVAL

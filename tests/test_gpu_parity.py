@@ -1,0 +1,103 @@
+"""HIP path against the oracle on identical seeded inputs (FP64).
+
+Bar: every per-cell state and flux, every routing state and every gauge
+discharge within 1e-10 relative (|a-b| <= 1e-10*max(|a|,|b|) + 1e-12).
+The per-step entry points (mhmhip_mhm_step + mhmhip_mrm_step, the drop-in
+for mHM() + mRM_routing()) and the pipelined time loop (mhmhip_run, plain
+and fused, both routing schedules) are checked.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from mhm_amd import synth
+from oracle_sim import rel_close
+from pairs import compare_cells, compare_routing, make_pair
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+ATOL = 1e-12
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_series(sim, recs):
+    return np.stack([sim.step(r) for r in recs])
+
+
+def test_step_by_step_matches_oracle():
+    _log("case: step_by_step 24x24 sets=2 hourly")
+    dom = synth.make_domain(24, 24, seed=11)
+    model, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=30, seed=3)
+    for k in range(len(recs)):
+        model.mhm_step(steps[k])
+        g = model.mrm_step(steps[k])
+        go = sim.step(recs[k])
+        ok, worst = rel_close(g, go, RTOL, ATOL)
+        assert ok, (k, worst, g, go)
+        if k in (0, 1, 7, 29):
+            assert compare_cells(model, sim, RTOL, ATOL) == [], k
+            assert compare_routing(model, sim, RTOL, ATOL) == [], k
+    model.close()
+
+
+@pytest.mark.parametrize("mode,S,route_mode", [(0, 0, 1), (1, 24, 1), (1, 5, 2), (1, 48, 2)])
+def test_run_matches_oracle(mode, S, route_mode):
+    _log(f"case: run mode={mode} S={S} route_mode={route_mode}")
+    dom = synth.make_domain(20, 28, seed=5)
+    model, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=60, seed=7)
+    g = model.run(steps, mode=mode, steps_per_launch=S, route_mode=route_mode)
+    go = _oracle_series(sim, recs)
+    ok, worst = rel_close(g, go, RTOL, ATOL)
+    assert ok, worst
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    model.close()
+
+
+def test_run_in_pieces_equals_one_run():
+    """Splitting the time loop over several calls changes nothing (states persist on device)."""
+    _log("case: run pieces")
+    dom = synth.make_domain(16, 16, seed=8)
+    m1, _, steps, _ = make_pair([dom], n_sets=1, n_steps=50, seed=2)
+    g1 = m1.run(steps, mode=1, steps_per_launch=24)
+    m2, _, _, _ = make_pair([dom], n_sets=1, n_steps=50, seed=2)
+    parts = [m2.run(steps[a:b], mode=1, steps_per_launch=7) for a, b in ((0, 13), (13, 14), (14, 50))]
+    assert np.array_equal(np.concatenate(parts), g1)
+    for name in ("SOILMOIST", "SATSTW", "TOTAL_RUNOFF"):
+        assert np.array_equal(m1.get(name), m2.get(name))
+    m1.close()
+    m2.close()
+
+
+def test_packed_domains_match_oracle():
+    """Uneven basins with their own parameters packed in one context."""
+    _log("case: packed domains")
+    doms = [synth.make_domain(12, 20, seed=21), synth.make_domain(30, 30, seed=22),
+            synth.make_domain(5, 7, seed=23)]
+    model, sim, steps, recs = make_pair(doms, n_sets=1, n_steps=40, seed=9)
+    g = model.run(steps, mode=1, steps_per_launch=16, route_mode=1)
+    go = _oracle_series(sim, recs)
+    assert rel_close(g, go, RTOL, ATOL)[0]
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    model.close()
+
+
+@pytest.mark.parametrize("soil,pet,timestep", [(2, 0, 1), (3, -1, 24), (4, 1, 24), (1, 0, 3)])
+def test_process_variants_match_oracle(soil, pet, timestep):
+    _log(f"case: variant soil={soil} pet={pet} timestep={timestep}")
+    dom = synth.make_domain(10, 14, seed=31)
+    n_steps = 40
+    model, sim, steps, recs = make_pair([dom], n_sets=1, timestep=timestep, n_steps=n_steps,
+                                        soil=soil, pet=pet, seed=4, pet_forcing=(pet <= 0))
+    g = model.run(steps, mode=1, steps_per_launch=8)
+    go = _oracle_series(sim, recs)
+    assert rel_close(g, go, RTOL, ATOL)[0]
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    model.close()
