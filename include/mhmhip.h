@@ -257,7 +257,9 @@ int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
  * for the cell kernels, node-steps for the routing) accumulate per kernel. */
 enum mhmhip_kernel { MHMHIP_K_CELL_STEP = 0, MHMHIP_K_CELL_FUSED, MHMHIP_K_ROUTE_DIAG,
                      MHMHIP_K_ROUTE_WG, MHMHIP_K_COUNT };
-int mhmhip_set_profiling(mhmhip_ctx *ctx, int32_t on); /* on != 0 also resets the totals */
+/* on > 0 enables timing and resets the totals; routing-diagonal launches are
+ * then timed one in `on` (1: all), the totals scaled to the timed ones. */
+int mhmhip_set_profiling(mhmhip_ctx *ctx, int32_t on);
 int mhmhip_get_profile(mhmhip_ctx *ctx, int32_t kernel, double *ms, int64_t *launches,
                        double *units);
 

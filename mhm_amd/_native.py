@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-HIP_LIB = os.path.join(PKG, "libmhmhip.so")
+HIP_LIB = os.environ.get("MHMHIP_LIB", os.path.join(PKG, "libmhmhip.so"))
 SYN_LIB = os.path.join(PKG, "libmhmsyn.so")
 
 OK, EINVAL, EHIP, ESTATE, ENOMEM = 0, -1, -2, -3, -4

@@ -31,8 +31,8 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def hip_flags():
-    return [
+def hip_flags(defines=()):
+    return [f"-D{d}" for d in defines] + [
         f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
         # every + - * / rounds on its own, as in the reference (no FMA contraction)
         "-ffp-contract=off",
@@ -49,24 +49,27 @@ def _run(cmd):
     return r.stdout
 
 
-def build_hip(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build_hip(verbose=False, out=None, defines=(), obj_dir=None):
+    """Compile and link the HIP library (out: other path, defines: -D macros)."""
+    obj = obj_dir or OBJ
+    os.makedirs(obj, exist_ok=True)
     hipcc = _hipcc()
     objs = []
     jobs = []
     for src in HIP_SOURCES:
-        o = os.path.join(OBJ, src + ".o")
+        o = os.path.join(obj, src + ".o")
         objs.append(o)
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        jobs.append([hipcc] + hip_flags() + ["-c"] + lang + [os.path.join(CSRC, src), "-o", o])
+        jobs.append([hipcc] + hip_flags(defines) + ["-c"] + lang + [os.path.join(CSRC, src), "-o", o])
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
-        for out in ex.map(_run, jobs):
-            if verbose and out.strip():
-                print(out)
-    tmp = HIP_LIB + ".tmp"
+        for log in ex.map(_run, jobs):
+            if verbose and log.strip():
+                print(log)
+    target = out or HIP_LIB
+    tmp = target + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
-    os.replace(tmp, HIP_LIB)
-    return HIP_LIB
+    os.replace(tmp, target)
+    return target
 
 
 def build_syn():

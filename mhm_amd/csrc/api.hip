@@ -72,6 +72,27 @@ struct mhmhip_ctx {
 
   int64_t t_cell = 0, t_route = 0;  // global step counters
   int64_t device_bytes = 0;
+  // Kernels index steps relative to the start of each call; the parity slot
+  // of pub holding qTR(:,IT1) is normalised to 1 before a call (step -1).
+  int qtr_slot = 1;
+  // positions [lev_start[k], lev_start[k+1]) hold the nodes at depth k
+  // (single-domain networks; lets a diagonal launch only its active band)
+  std::vector<int64_t> lev_start;
+
+  // time loop captured as a hipGraph, replayed while its key matches
+  struct GraphKey {
+    int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1;
+    const void *ring = nullptr, *gauge = nullptr, *steps = nullptr, *inflow = nullptr;
+    int64_t ring_len = 0;
+    bool operator==(const GraphKey &o) const {
+      return nsteps == o.nsteps && mode == o.mode && S == o.S && block == o.block && wg == o.wg &&
+             ring == o.ring && gauge == o.gauge && steps == o.steps && inflow == o.inflow &&
+             ring_len == o.ring_len;
+    }
+  };
+  GraphKey gkey;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
 
   // kernel timing (mhmhip_set_profiling)
   struct Pending {
@@ -80,10 +101,13 @@ struct mhmhip_ctx {
     double units;
   };
   bool prof = false;
+  int prof_stride = 1;         // time every prof_stride-th routing launch
+  int64_t route_launches = 0;  // routing launches seen while profiling
   std::vector<Pending> pending;
   std::vector<hipEvent_t> ev_pool;
   double prof_ms[MHMHIP_K_COUNT] = {0, 0, 0, 0};
-  int64_t prof_n[MHMHIP_K_COUNT] = {0, 0, 0, 0};
+  int64_t prof_n[MHMHIP_K_COUNT] = {0, 0, 0, 0};      // timed launches
+  int64_t prof_all[MHMHIP_K_COUNT] = {0, 0, 0, 0};    // all launches
   double prof_units[MHMHIP_K_COUNT] = {0, 0, 0, 0};
 
   int fail(int code, const std::string &m) {
@@ -125,6 +149,10 @@ struct mhmhip_ctx {
     int rc_ = (expr);          \
     if (rc_ != MHMHIP_OK) return rc_; \
   } while (0)
+
+namespace {
+void drop_graph(mhmhip_ctx *x);
+}
 
 // Copies are ordered on the context's (non-blocking) stream behind the
 // clears and kernels queued there, then joined.
@@ -271,6 +299,7 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
     (void)hipEventDestroy(p.b);
   }
   for (hipEvent_t e : x->ev_pool) (void)hipEventDestroy(e);
+  drop_graph(x);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return MHMHIP_OK;
@@ -502,6 +531,11 @@ int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
   d.day = s.iMeteoTS - 1;
   d.month0 = s.month - 1;
   d.isday = (s.hour > 6) && (s.hour <= 18);  // mo_meteo_handler.f90:1048
+  const int m = s.month - 1;
+  d.f_pet = d.isday ? x->mt.fday_pet[m] : x->mt.fnight_pet[m];
+  d.f_prec = d.isday ? x->mt.fday_prec[m] : x->mt.fnight_prec[m];
+  d.f_temp = d.isday ? x->mt.fday_temp[m] : x->mt.fnight_temp[m];
+  d.evap = x->mt.evap[m];
   d.init_sm = (s.tt == 1) && !s.read_states;  // mo_mhm.f90:448
   d.doy = s.doy;
   d.pad = 0;
@@ -515,6 +549,8 @@ CellArgs cell_args(mhmhip_ctx *x, bool ring) {
   a.X = x->X;
   a.F = x->F;
   a.ring = ring ? x->d_ring : nullptr;
+  a.area = x->d_area;
+  a.tst = kHourSecs * (double)x->cfg.timestep;  // mo_mrm_pre_routing.f90:110
   a.steps = x->d_steps;
   a.n_cells = x->n_cells;
   a.sstride = x->sstride;
@@ -564,7 +600,6 @@ int ensure_gauge_buf(mhmhip_ctx *x, int64_t rows) {
 RouteArgs route_args(mhmhip_ctx *x, int64_t t_lo, int64_t t_hi, int64_t t_base) {
   RouteArgs r{};
   r.meta = x->d_meta;
-  r.area = x->d_area;
   r.C1 = x->d_C1;
   r.C2 = x->d_C2;
   r.qTIN = x->d_qTIN;
@@ -613,10 +648,13 @@ hipError_t ev_get(mhmhip_ctx *x, hipEvent_t *e) {
   return hipEventCreate(e);
 }
 
-// Launches f() on the context stream; with profiling on, brackets it with events.
+// Launches f() on the context stream; with profiling on, brackets it with
+// events (routing launches: every prof_stride-th one).
 template <typename Fn>
 hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
   if (!x->prof) return f();
+  x->prof_all[kind] += 1;
+  if (kind == MHMHIP_K_ROUTE_DIAG && (x->route_launches++ % x->prof_stride) != 0) return f();
   mhmhip_ctx::Pending p{kind, nullptr, nullptr, units};
   hipError_t e;
   if ((e = ev_get(x, &p.a)) != hipSuccess || (e = ev_get(x, &p.b)) != hipSuccess) return e;
@@ -642,6 +680,24 @@ int prof_collect(mhmhip_ctx *x) {
   return MHMHIP_OK;
 }
 
+// Node positions that can be active on diagonal d (relative clock): nodes at
+// depths with t = d - (dmax - depth) in [t_lo, t_hi).  Single-domain layouts
+// keep depths contiguous; packed domains use the full range.
+void active_range(const mhmhip_ctx *x, const RouteArgs &r, int64_t d, int64_t &lo, int64_t &hi) {
+  lo = 0;
+  hi = x->n_cells;
+  if (x->lev_start.empty()) return;
+  const int64_t dm = x->lay.dmax;
+  const int64_t dep_lo = std::max<int64_t>(0, r.t_lo - d + dm);
+  const int64_t dep_hi = std::min<int64_t>(dm, r.t_hi - 1 - d + dm);
+  if (dep_lo > dep_hi) {
+    lo = hi = 0;
+    return;
+  }
+  lo = x->lev_start[dep_lo];
+  hi = x->lev_start[dep_hi + 1];
+}
+
 // routing diagonals [d_lo, d_hi) (tiles) or the drain after t_hi
 int route_diagonals(mhmhip_ctx *x, const RouteArgs &r, int64_t d_lo, int64_t d_hi, bool drain,
                     bool wg) {
@@ -654,10 +710,14 @@ int route_diagonals(mhmhip_ctx *x, const RouteArgs &r, int64_t d_lo, int64_t d_h
     return MHMHIP_OK;
   }
   const int64_t end = drain ? d_lo + x->lay.dmax : d_hi;
-  for (int64_t d = d_lo; d < end; ++d)
+  for (int64_t d = d_lo; d < end; ++d) {
+    int64_t lo, hi;
+    active_range(x, r, d, lo, hi);
+    if (hi <= lo) continue;
     CHECK_HIP(x, timed(x, MHMHIP_K_ROUTE_DIAG, 0.0,
-                       [&] { return launch_route_diag(r, d, x->stream); }),
+                       [&] { return launch_route_diag(r, d, lo, hi, x->stream); }),
               "route_diag");
+  }
   return MHMHIP_OK;
 }
 
@@ -666,6 +726,24 @@ void credit_route(mhmhip_ctx *x, bool wg, int64_t nsteps) {
   if (x->prof)
     x->prof_units[wg ? MHMHIP_K_ROUTE_WG : MHMHIP_K_ROUTE_DIAG] +=
         (double)x->n_cells * x->n_sets * (double)nsteps;
+}
+
+// Moves qTR(:,IT1) into parity slot 1, where a call's step -1 reads it.
+int normalize_qtr(mhmhip_ctx *x) {
+  if (x->qtr_slot == 1) return MHMHIP_OK;
+  CHECK_HIP(x, hipMemcpyAsync(x->d_pub + x->ld, x->d_pub, sizeof(double) * x->ld,
+                              hipMemcpyDeviceToDevice, x->stream),
+            "qTR slot");
+  x->qtr_slot = 1;
+  return MHMHIP_OK;
+}
+
+void drop_graph(mhmhip_ctx *x) {
+  if (x->graph_exec) (void)hipGraphExecDestroy(x->graph_exec);
+  if (x->graph) (void)hipGraphDestroy(x->graph);
+  x->graph_exec = nullptr;
+  x->graph = nullptr;
+  x->gkey = mhmhip_ctx::GraphKey();
 }
 
 }  // namespace
@@ -684,7 +762,7 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
   }
   CellArgs a = cell_args(x, route);
   CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
-              return launch_cell_step(x->nh, a, d, x->mt, x->t_cell, 256, x->n_sets, x->stream);
+              return launch_cell_step(x->nh, a, d, 0, 256, x->n_sets, x->stream);
             }),
             "cell_step");
   x->t_cell++;
@@ -836,8 +914,16 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   CHECK_HIP(x, sync_copy(x, x->d_inflow_hw, ihw.data(), sizeof(int32_t) * ihw.size(), hipMemcpyHostToDevice), "upload inflow");
   x->h_inflow_node_dev = inode;
   x->h_inflow_hw = ihw;
+  x->lev_start.clear();
+  if (x->n_domains == 1) {
+    x->lev_start.assign(lay.dmax + 2, 0);
+    for (int64_t q = 0; q < x->n_cells; ++q) x->lev_start[nm_depth(lay.packed[q]) + 1]++;
+    for (int k = 0; k <= lay.dmax; ++k) x->lev_start[k + 1] += x->lev_start[k];
+  }
   x->lay = std::move(lay);
   x->net_bound = true;
+  x->qtr_slot = 1;
+  drop_graph(x);
   x->t_route = x->t_cell;
   x->dfree(x->d_ring, x->ring_len * x->ld);
   x->ring_len = 0;
@@ -935,7 +1021,7 @@ extern "C" int mhmhip_get_routing_state(mhmhip_ctx *x, int32_t set, double *qOUT
   CHECK_RC(node_out(x->d_qOUT + off, qOUT));
   CHECK_RC(node_out(x->d_qTIN + off, qTIN));
   CHECK_RC(node_out(x->d_qTIN + off, Qmod));
-  CHECK_RC(node_out(x->d_pub + ((x->t_route - 1) & 1) * x->ld + off, qTR));
+  CHECK_RC(node_out(x->d_pub + x->qtr_slot * x->ld + off, qTR));
   CHECK_RC(link_out(x->d_C1 + off, C1));
   CHECK_RC(link_out(x->d_C2 + off, C2));
   return MHMHIP_OK;
@@ -950,7 +1036,7 @@ extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double
   const int64_t off = (int64_t)set * x->sstride;
   auto id = [](double a) { return a; };
   if (qTIN) CHECK_RC(put_row(x, x->d_qTIN + off, qTIN, id));
-  if (qTR) CHECK_RC(put_row(x, x->d_pub + ((x->t_route - 1) & 1) * x->ld + off, qTR, id));
+  if (qTR) CHECK_RC(put_row(x, x->d_pub + x->qtr_slot * x->ld + off, qTR, id));
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_routing_state");
   return MHMHIP_OK;
 }
@@ -967,12 +1053,14 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
     return x->fail(MHMHIP_ESTATE, "mrm_step: inflow discharge not set");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "mrm_step");
   CHECK_RC(ensure_gauge_buf(x, 1));
-  const int64_t t = x->t_route;
-  RouteArgs r = route_args(x, t, t + 1, t);
+  CHECK_RC(normalize_qtr(x));
+  // this step is step 0 of the call: mhm_step wrote its qOUT to ring slot 0
+  RouteArgs r = route_args(x, 0, 1, 0);
   const bool wg = use_wg_routing(x, 0);
-  CHECK_RC(route_diagonals(x, r, t, t + 1, false, wg));
-  CHECK_RC(route_diagonals(x, r, t + 1, 0, true, wg));
+  CHECK_RC(route_diagonals(x, r, 0, 1, false, wg));
+  CHECK_RC(route_diagonals(x, r, 1, 0, true, wg));
   credit_route(x, wg, 1);
+  x->qtr_slot = 0;
   x->t_route++;
   if (gauge_out && x->n_gauges > 0) {
     CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf,
@@ -1018,29 +1106,64 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     CHECK_RC(ensure_ring(x, (int64_t)S + x->lay.dmax + 2));
     CHECK_RC(ensure_gauge_buf(x, nsteps));
   }
-  const int64_t t_start = x->t_cell;
+  if (route) CHECK_RC(normalize_qtr(x));
   CellArgs a = cell_args(x, route);
-  RouteArgs r = route_args(x, t_start, t_start + nsteps, t_start);
-  for (int32_t k = 0; k < nsteps; k += S) {
-    const int n = std::min<int32_t>(S, nsteps - k);
-    if (o.mode == MHMHIP_RUN_PLAIN) {
-      CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
-                  return launch_cell_step(x->nh, a, hs[k], x->mt, t_start + k, block, x->n_sets,
-                                          x->stream);
-                }),
-                "cell_step");
-    } else {
-      CHECK_HIP(x, timed(x, MHMHIP_K_CELL_FUSED, (double)x->n_cells * x->n_sets * n, [&] {
-                  return launch_cell_fused(x->nh, a, x->mt, t_start + k, n, k, block, x->n_sets,
-                                           x->stream);
-                }),
-                "cell_fused");
+  RouteArgs r = route_args(x, 0, nsteps, 0);
+  // the whole loop, in steps relative to this call (replayable as a graph)
+  auto enqueue = [&]() -> int {
+    for (int32_t k = 0; k < nsteps; k += S) {
+      const int n = std::min<int32_t>(S, nsteps - k);
+      if (o.mode == MHMHIP_RUN_PLAIN) {
+        CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
+                    return launch_cell_step(x->nh, a, hs[k], k, block, x->n_sets, x->stream);
+                  }),
+                  "cell_step");
+      } else {
+        CHECK_HIP(x, timed(x, MHMHIP_K_CELL_FUSED, (double)x->n_cells * x->n_sets * n, [&] {
+                    return launch_cell_fused(x->nh, a, k, n, k, block, x->n_sets, x->stream);
+                  }),
+                  "cell_fused");
+      }
+      if (route) CHECK_RC(route_diagonals(x, r, k, k + n, false, wg));
     }
-    if (route) CHECK_RC(route_diagonals(x, r, t_start + k, t_start + k + n, false, wg));
+    if (route) CHECK_RC(route_diagonals(x, r, nsteps, 0, true, wg));
+    return MHMHIP_OK;
+  };
+  const bool graph = o.use_graph && !x->prof && o.mode == MHMHIP_RUN_FUSED;
+  if (graph) {
+    mhmhip_ctx::GraphKey key;
+    key.nsteps = nsteps;
+    key.mode = o.mode;
+    key.S = S;
+    key.block = block;
+    key.wg = wg ? 1 : 0;
+    key.ring = route ? x->d_ring : nullptr;
+    key.gauge = route ? x->d_gauge_buf : nullptr;
+    key.steps = x->d_steps;
+    key.inflow = x->d_inflow_q;
+    key.ring_len = x->ring_len;
+    if (!x->graph_exec || !(key == x->gkey)) {
+      drop_graph(x);
+      CHECK_HIP(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal), "capture");
+      const int rc = enqueue();
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(x->stream, &g);
+      if (rc != MHMHIP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      CHECK_HIP(x, e, "capture");
+      x->graph = g;
+      CHECK_HIP(x, hipGraphInstantiate(&x->graph_exec, g, nullptr, nullptr, 0), "instantiate");
+      x->gkey = key;
+    }
+    CHECK_HIP(x, hipGraphLaunch(x->graph_exec, x->stream), "graph launch");
+  } else {
+    CHECK_RC(enqueue());
   }
   if (route) {
-    CHECK_RC(route_diagonals(x, r, t_start + nsteps, 0, true, wg));
     credit_route(x, wg, nsteps);
+    x->qtr_slot = (int)((nsteps - 1) & 1);
   }
   x->t_cell += nsteps;
   x->t_route = x->t_cell;
@@ -1059,11 +1182,14 @@ extern "C" int mhmhip_set_profiling(mhmhip_ctx *x, int32_t on) {
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_profiling");
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_profiling");
   CHECK_RC(prof_collect(x));
-  x->prof = on != 0;
+  x->prof = on > 0;
+  x->prof_stride = on > 0 ? on : 1;
+  x->route_launches = 0;
   if (x->prof)
     for (int k = 0; k < MHMHIP_K_COUNT; ++k) {
       x->prof_ms[k] = 0.0;
       x->prof_n[k] = 0;
+      x->prof_all[k] = 0;
       x->prof_units[k] = 0.0;
     }
   return MHMHIP_OK;
@@ -1073,8 +1199,10 @@ extern "C" int mhmhip_get_profile(mhmhip_ctx *x, int32_t kernel, double *ms, int
                                   double *units) {
   if (!x) return MHMHIP_EINVAL;
   if (kernel < 0 || kernel >= MHMHIP_K_COUNT) return x->fail(MHMHIP_EINVAL, "get_profile: bad kernel");
+  // units scaled to the timed launches when routing launches were sampled
+  const double frac = x->prof_all[kernel] ? (double)x->prof_n[kernel] / x->prof_all[kernel] : 1.0;
   if (ms) *ms = x->prof_ms[kernel];
   if (launches) *launches = x->prof_n[kernel];
-  if (units) *units = x->prof_units[kernel];
+  if (units) *units = x->prof_units[kernel] * frac;
   return MHMHIP_OK;
 }

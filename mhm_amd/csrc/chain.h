@@ -26,6 +26,30 @@ constexpr double kDuffieDelta2 = 1.3900;
 constexpr double kPi = 3.141592653589793238462643383279502884197;
 constexpr double kRadConst = 86400.0 / kPi / 2.45e06 * 1367.0;
 
+// x**y for y > 0 as exp(y*log(x)).  For the chain's arguments (|y*log(x)| of
+// order 10 at most) this is within a few 1e-16 relative of the correctly
+// rounded power, at a fraction of pow()'s cost; x <= 0 gives pow()'s result
+// for non-integer y (0 for x == 0, NaN below).
+// exp(y * log(x)), kept out of line: inlined at every call site inside the
+// time loop, the polynomial constants of exp and log are hoisted into
+// registers for the whole loop and cut the kernel's occupancy.
+#ifndef MHMHIP_OUTLINE_TRANSC
+#define MHMHIP_OUTLINE_TRANSC 1
+#endif
+#if MHMHIP_OUTLINE_TRANSC
+#define MHMHIP_OUTLINE __attribute__((noinline))
+#else
+#define MHMHIP_OUTLINE __forceinline__
+#endif
+__device__ MHMHIP_OUTLINE double exp_ylogx(double y, double x) {
+  return exp(y * log(x));
+}
+
+__device__ __forceinline__ double pow_pos(double x, double y) {
+  if (x > 0.0) return exp_ylogx(y, x);
+  return x == 0.0 ? 0.0 : __builtin_nan("");
+}
+
 // FORCES mo_utils LE: a <= b within eps*|b| (pre-proc/mtCLIM/mo_utils.f90:922-934)
 __device__ __forceinline__ bool le_eps(double a, double b) {
   return !(((DBL_EPSILON * fabs(b) - fabs(a - b)) < 0.0) && (a > b));
@@ -44,7 +68,7 @@ __device__ __forceinline__ void canopy_interc(double pet, double interc_max, dou
     interc = aux;
   }
   if (interc_max > kEps)
-    evap_canopy = pet * pow(interc / interc_max, kTwoThird);
+    evap_canopy = pet * pow_pos(interc / interc_max, kTwoThird);
   else
     evap_canopy = 0.0;
   if (evap_canopy < 0.0) evap_canopy = 0.0;
@@ -159,7 +183,7 @@ __device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double
     } else {
       double frac;
       if (sm[h] > kEps)
-        frac = exp(bexp[h] * log(sm[h] / sat[h]));
+        frac = exp_ylogx(bexp[h], sm[h] / sat[h]);
       else
         frac = 0.0;
       const double tmp = pin * (1.0 - frac);
@@ -208,7 +232,7 @@ __device__ __forceinline__ void runoff_unsat_zone(double k1, double kp, double k
   unsat = unsat - fast;
   slow = 0.0;
   if (unsat > kEps) {
-    const double a = k1 * pow(unsat, 1.0 + alpha), b = unsat - kEps;
+    const double a = k1 * pow_pos(unsat, 1.0 + alpha), b = unsat - kEps;
     slow = (b < a) ? b : a;
   }
   unsat = unsat - slow;
@@ -243,7 +267,7 @@ __device__ __forceinline__ double total_runoff(double fS, double fast, double sl
 // latitude terms tan/sin/cos(deg2rad*lat) are per-cell constants and the
 // day-of-year terms dr, tan/sin/cos(delta) per-step constants; both are
 // evaluated once on the host with the same libm calls the reference makes.
-__device__ __forceinline__ double pet_hargreaves(double coeff, double tavg, double tmax,
+__device__ MHMHIP_OUTLINE double pet_hargreaves(double coeff, double tavg, double tmax,
                                                  double tmin, double tanlat, double sinlat,
                                                  double coslat, double dr, double tand,
                                                  double sind, double cosd) {
@@ -257,17 +281,16 @@ __device__ __forceinline__ double pet_hargreaves(double coeff, double tavg, doub
   return coeff * ra * (tavg + kHarSamConst) * sqrt(dT);
 }
 
-// src/meteo/mo_meteo_temporal_tools.f90:54-79
-__device__ __forceinline__ double disagg_flux(bool isday, double nT, double v, double fday,
-                                              double fnight) {
-  if (nT > 1.0) return isday ? 2.0 * v * fday / nT : 2.0 * v * fnight / nT;
+// src/meteo/mo_meteo_temporal_tools.f90:54-79 with the day or night factor f
+// already chosen: 2 * v * f / ntimesteps_day
+__device__ __forceinline__ double disagg_flux(double nT, double v, double f) {
+  if (nT > 1.0) return 2.0 * v * f / nT;
   return v;
 }
 
 // src/meteo/mo_meteo_temporal_tools.f90:100-161 with add_correction = .true.
-__device__ __forceinline__ double disagg_state_add(bool isday, double nT, double v, double fday,
-                                                   double fnight) {
-  if (nT > 1.0) return isday ? v + fday : v + fnight;
+__device__ __forceinline__ double disagg_state_add(double nT, double v, double f) {
+  if (nT > 1.0) return v + f;
   return v;
 }
 

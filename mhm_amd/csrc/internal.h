@@ -10,7 +10,7 @@
 //   S[row][ld]   states
 //   X[row][ld]   fluxes of the last step
 //   F[var][day][sstride]  forcing, shared by all parameter sets
-//   ring[R][ld]  L1 total runoff per step, consumed by the routing
+//   ring[R][ld]  qOUT per step (L1 total runoff in m3/s), consumed by the routing
 #pragma once
 
 #include <stdint.h>
@@ -52,6 +52,9 @@ __host__ __device__ constexpr int xr_count(int nh) { return XR_NSCALAR + 2 * nh;
 // Per-step scalars on the device (host-evaluated from the step clock).
 struct StepDev {
   double dr, tand, sind, cosd;  // extraterr_rad_approx day-of-year terms
+  // monthly namelist factors of this step, day or night already selected
+  // (mo_meteo_handler.f90:1111-1117, 1192-1198, 1273-1279; mo_mhm.f90:473)
+  double f_pet, f_prec, f_temp, evap;
   int32_t day;                  // 0-based forcing column
   int32_t month0;               // 0..11
   int32_t isday;                // 6 < hour <= 18
@@ -73,6 +76,8 @@ struct CellArgs {
   double *X;
   const double *F;
   double *ring;            // nullptr: no routing input wanted
+  const double *area;      // [n_cells] L1 cell area [km2] (routing input)
+  double tst;              // HourSecs * timestep [s]
   const StepDev *steps;    // device step table (fused kernel)
   int64_t n_cells;
   int64_t sstride;
@@ -106,7 +111,6 @@ __host__ __device__ inline int nm_nchild(int32_t y) { return (y >> 20) & 0xF; }
 
 struct RouteArgs {
   const NodeMeta *meta;    // [n_nodes]
-  const double *area;      // [n_nodes] km2
   const double *C1;        // [ld]
   const double *C2;        // [ld]
   double *qTIN;            // [ld]   qTIN(:,IT1)

@@ -2,7 +2,10 @@
 //
 // Cell kernels run one lane per (cell, parameter set).  Every row of P, S, X
 // and F is a structure-of-arrays slab, so each of the ~70 loads and stores a
-// lane makes per step is a fully coalesced 512-byte wavefront access.
+// lane makes per step is a fully coalesced 512-byte wavefront access.  Rows
+// are addressed as a wave-uniform base (SGPRs) plus one 32-bit byte offset
+// per lane shared by all rows, which maps onto global_load/store with
+// saddr + voffset and keeps per-row 64-bit addresses out of the VGPRs.
 //
 // Routing is pipelined over (depth, time) diagonals: node p at depth delta
 // (distance to its outlet) handles step t on diagonal d = t + dmax - delta.
@@ -18,10 +21,28 @@
 
 namespace mhmhip {
 
+// row[boff / 8] with a uniform row base and a 32-bit per-lane byte offset
+__device__ __forceinline__ double ldr(const double *row, uint32_t boff) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(row) + boff);
+}
+__device__ __forceinline__ void str(double *row, uint32_t boff, double x) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + boff) = x;
+}
+// streaming store for outputs no kernel of the step reads back
+#ifndef MHMHIP_NT_STORE
+#define MHMHIP_NT_STORE 1
+#endif
+__device__ __forceinline__ void stn(double *row, uint32_t boff, double x) {
+#if MHMHIP_NT_STORE
+  __builtin_nontemporal_store(x, reinterpret_cast<double *>(reinterpret_cast<char *>(row) + boff));
+#else
+  str(row, boff, x);
+#endif
+}
+
 template <int NH>
 struct CellRegs {
-  double fS, thrS, c1, k0s, k1s, k2s, kps, alpha, karst, thrU, fasp, harsam, tanlat, sinlat,
-      coslat, dmaxs, dnps, incr, tth, ddthr;
+  double fS, thrS, c1, k0s, k1s, k2s, kps, alpha, karst, thrU, dmaxs, dnps, incr, tth, ddthr;
   double sat[NH], fr[NH], fc[NH], wp[NH], bexp[NH];
   double inter, pack, seal, unsat, satS;
   double sm[NH];
@@ -35,105 +56,107 @@ struct FluxRegs {
 };
 
 template <int NH>
-__device__ __forceinline__ void load_params(const CellArgs &a, int64_t v, CellRegs<NH> &c) {
-  const double *P = a.P + v;
-  const int64_t ld = a.ld;
-  c.fS = P[PR_FS * ld];
-  c.thrS = P[PR_SEALTHR * ld];
-  c.c1 = a.jarvis ? P[PR_JARVIS * ld] : 0.0;
-  c.k0s = P[PR_K0S * ld];
-  c.k1s = P[PR_K1S * ld];
-  c.k2s = P[PR_K2S * ld];
-  c.kps = P[PR_KPS * ld];
-  c.alpha = P[PR_ALPHA * ld];
-  c.karst = P[PR_KARST * ld];
-  c.thrU = P[PR_UNSATTHR * ld];
-  c.fasp = P[PR_FASP * ld];
-  c.harsam = P[PR_HARSAM * ld];
-  c.tanlat = P[PR_TANLAT * ld];
-  c.sinlat = P[PR_SINLAT * ld];
-  c.coslat = P[PR_COSLAT * ld];
-  c.dmaxs = P[PR_DMAXS * ld];
-  c.dnps = P[PR_DNPS * ld];
-  c.incr = P[PR_INCR * ld];
-  c.tth = P[PR_TTH * ld];
+__device__ __forceinline__ void load_params(const CellArgs &a, uint32_t bv, CellRegs<NH> &c) {
+  auto P = [&](int r) { return ldr(a.P + (int64_t)r * a.ld, bv); };
+  c.fS = P(PR_FS);
+  c.thrS = P(PR_SEALTHR);
+  c.c1 = a.jarvis ? P(PR_JARVIS) : 0.0;
+  c.k0s = P(PR_K0S);
+  c.k1s = P(PR_K1S);
+  c.k2s = P(PR_K2S);
+  c.kps = P(PR_KPS);
+  c.alpha = P(PR_ALPHA);
+  c.karst = P(PR_KARST);
+  c.thrU = P(PR_UNSATTHR);
+  c.dmaxs = P(PR_DMAXS);
+  c.dnps = P(PR_DNPS);
+  c.incr = P(PR_INCR);
+  c.tth = P(PR_TTH);
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
-    c.sat[h] = P[(pr_sat(NH) + h) * ld];
-    c.fr[h] = P[(pr_froots(NH) + h) * ld];
-    c.fc[h] = P[(pr_fc(NH) + h) * ld];
-    c.wp[h] = P[(pr_wp(NH) + h) * ld];
-    c.bexp[h] = P[(pr_exp(NH) + h) * ld];
+    c.sat[h] = P(pr_sat(NH) + h);
+    c.fr[h] = P(pr_froots(NH) + h);
+    c.fc[h] = P(pr_fc(NH) + h);
+    c.wp[h] = P(pr_wp(NH) + h);
+    c.bexp[h] = P(pr_exp(NH) + h);
   }
   // snow_accum_melt's branch threshold (mo_snow_accum_melt.f90:127)
   c.ddthr = (c.dmaxs - c.dnps) / c.incr;
 }
 
 template <int NH>
-__device__ __forceinline__ void load_states(const CellArgs &a, int64_t v, CellRegs<NH> &c) {
-  const double *S = a.S + v;
-  const int64_t ld = a.ld;
-  c.inter = S[SR_INTER * ld];
-  c.pack = S[SR_SNOW * ld];
-  c.seal = S[SR_SEAL * ld];
-  c.unsat = S[SR_UNSAT * ld];
-  c.satS = S[SR_SAT * ld];
+__device__ __forceinline__ void load_states(const CellArgs &a, uint32_t bv, CellRegs<NH> &c) {
+  auto S = [&](int r) { return ldr(a.S + (int64_t)r * a.ld, bv); };
+  c.inter = S(SR_INTER);
+  c.pack = S(SR_SNOW);
+  c.seal = S(SR_SEAL);
+  c.unsat = S(SR_UNSAT);
+  c.satS = S(SR_SAT);
 #pragma unroll
-  for (int h = 0; h < NH; ++h) c.sm[h] = S[(SR_NSCALAR + h) * ld];
+  for (int h = 0; h < NH; ++h) c.sm[h] = S(SR_NSCALAR + h);
 }
 
 template <int NH>
-__device__ __forceinline__ void store_states(const CellArgs &a, int64_t v, const CellRegs<NH> &c) {
-  double *S = a.S + v;
-  const int64_t ld = a.ld;
-  S[SR_INTER * ld] = c.inter;
-  S[SR_SNOW * ld] = c.pack;
-  S[SR_SEAL * ld] = c.seal;
-  S[SR_UNSAT * ld] = c.unsat;
-  S[SR_SAT * ld] = c.satS;
+__device__ __forceinline__ void store_states(const CellArgs &a, uint32_t bv, const CellRegs<NH> &c) {
+  auto S = [&](int r, double x) { str(a.S + (int64_t)r * a.ld, bv, x); };
+  S(SR_INTER, c.inter);
+  S(SR_SNOW, c.pack);
+  S(SR_SEAL, c.seal);
+  S(SR_UNSAT, c.unsat);
+  S(SR_SAT, c.satS);
 #pragma unroll
-  for (int h = 0; h < NH; ++h) S[(SR_NSCALAR + h) * ld] = c.sm[h];
+  for (int h = 0; h < NH; ++h) S(SR_NSCALAR + h, c.sm[h]);
 }
 
 template <int NH>
-__device__ __forceinline__ void store_fluxes(const CellArgs &a, int64_t v, const FluxRegs<NH> &x) {
-  double *X = a.X + v;
-  const int64_t ld = a.ld;
-  X[XR_PET * ld] = x.pet;
-  X[XR_TEMP * ld] = x.temp;
-  X[XR_PREC * ld] = x.prec;
-  X[XR_THROUGHFALL * ld] = x.thr;
-  X[XR_AETCAN * ld] = x.ecan;
-  X[XR_DEGDAY * ld] = x.ddf;
-  X[XR_MELT * ld] = x.melt;
-  X[XR_PEFF * ld] = x.peff;
-  X[XR_RAIN * ld] = x.rain;
-  X[XR_SNOW * ld] = x.snow;
-  X[XR_RUNOFFSEAL * ld] = x.rseal;
-  X[XR_AETSEAL * ld] = x.aseal;
-  X[XR_FAST * ld] = x.fast;
-  X[XR_SLOW * ld] = x.slow;
-  X[XR_PERC * ld] = x.perc;
-  X[XR_BASEFLOW * ld] = x.bflow;
-  X[XR_TOTAL * ld] = x.total;
+__device__ __forceinline__ void store_fluxes(const CellArgs &a, uint32_t bv, const FluxRegs<NH> &x) {
+  auto X = [&](int r, double y) { stn(a.X + (int64_t)r * a.ld, bv, y); };
+  X(XR_PET, x.pet);
+  X(XR_TEMP, x.temp);
+  X(XR_PREC, x.prec);
+  X(XR_THROUGHFALL, x.thr);
+  X(XR_AETCAN, x.ecan);
+  X(XR_DEGDAY, x.ddf);
+  X(XR_MELT, x.melt);
+  X(XR_PEFF, x.peff);
+  X(XR_RAIN, x.rain);
+  X(XR_SNOW, x.snow);
+  X(XR_RUNOFFSEAL, x.rseal);
+  X(XR_AETSEAL, x.aseal);
+  X(XR_FAST, x.fast);
+  X(XR_SLOW, x.slow);
+  X(XR_PERC, x.perc);
+  X(XR_BASEFLOW, x.bflow);
+  X(XR_TOTAL, x.total);
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
-    X[(xr_infil(NH) + h) * ld] = x.infil[h];
-    X[(xr_aet(NH) + h) * ld] = x.aet[h];
+    X(xr_infil(NH) + h, x.infil[h]);
+    X(xr_aet(NH) + h, x.aet[h]);
   }
 }
 
-// Daily PET before disaggregation (get_corrected_pet, mo_meteo_handler.f90:1050-1097)
+// Daily PET before disaggregation (get_corrected_pet, mo_meteo_handler.f90:1050-1097).
+// Its per-cell parameters are read here, when a new forcing day starts, rather
+// than held in registers across steps.  bc: byte offset of the cell in a
+// forcing row; bv: byte offset of (cell, set) in a parameter row.
 template <int NH>
-__device__ __forceinline__ double pet_day(const CellArgs &a, const CellRegs<NH> &c,
-                                          const StepDev &st, int64_t v, int64_t f, double tavg,
-                                          double tmax, double tmin) {
+__device__ __forceinline__ double pet_day(const CellArgs &a, const StepDev &st, uint32_t bv,
+                                          uint32_t bc, double tavg, double tmax, double tmin) {
+  auto P = [&](int r) { return ldr(a.P + (int64_t)r * a.ld, bv); };
   if (a.pet_case == 1)
-    return c.fasp * pet_hargreaves(c.harsam, tavg, tmax, tmin, c.tanlat, c.sinlat, c.coslat,
-                                   st.dr, st.tand, st.sind, st.cosd);
-  const double petin = a.F[((int64_t)MHMHIP_F_PET * a.forcing_days + st.day) * a.sstride + f];
-  if (a.pet_case == 0) return c.fasp * petin;
-  return a.P[(pr_petlai(NH) + st.month0) * a.ld + v] * petin;
+    return P(PR_FASP) * pet_hargreaves(P(PR_HARSAM), tavg, tmax, tmin, P(PR_TANLAT),
+                                       P(PR_SINLAT), P(PR_COSLAT), st.dr, st.tand, st.sind,
+                                       st.cosd);
+  const double petin =
+      ldr(a.F + ((int64_t)MHMHIP_F_PET * a.forcing_days + st.day) * a.sstride, bc);
+  if (a.pet_case == 0) return P(PR_FASP) * petin;
+  return P(pr_petlai(NH) + st.month0) * petin;
+}
+
+// L11_runoff_acc for L1 = L11 with map_flag (mo_mrm_pre_routing.f90:110-126):
+// qAcc = 0; qAcc = qAcc + q * area; qAcc = qAcc * 1000 / TST   [m3 s-1]
+__device__ __forceinline__ double runoff_acc(double q, double area, double tst) {
+  return ((0.0 + q * area) * 1000.0) / tst;
 }
 
 // One step of mHM() for one cell after the meteo getters (mo_mhm.f90:455-519)
@@ -154,106 +177,107 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
 // ---------------------------------------------------------------------------
 // Plain path: one launch per model step; every state and flux is written.
 template <int NH>
-__global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, MonthTables mt,
-                                                        int64_t t) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, int64_t t) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
-  const int64_t v = (int64_t)blockIdx.y * a.sstride + c;
+  const uint32_t bc = c * 8u;
+  const uint32_t bv = (uint32_t)(blockIdx.y * a.sstride + c) * 8u;
   CellRegs<NH> r;
   FluxRegs<NH> x;
-  load_params<NH>(a, v, r);
-  load_states<NH>(a, v, r);
-  const int m = st.month0;
-  const bool isday = st.isday != 0;
-  const double *F = a.F + (int64_t)st.day * a.sstride + c;
+  load_params<NH>(a, bv, r);
+  load_states<NH>(a, bv, r);
+  const double *F = a.F + (int64_t)st.day * a.sstride;
   const int64_t fv = a.forcing_days * a.sstride;
-  const double pre = F[MHMHIP_F_PRE * fv], tavg = F[MHMHIP_F_TAVG * fv];
-  const double tmax = a.pet_case == 1 ? F[MHMHIP_F_TMAX * fv] : 0.0;
-  const double tmin = a.pet_case == 1 ? F[MHMHIP_F_TMIN * fv] : 0.0;
-  const double pd = pet_day<NH>(a, r, st, v, c, tavg, tmax, tmin);
-  x.pet = a.hourly ? pd : disagg_flux(isday, a.nT, pd, mt.fday_pet[m], mt.fnight_pet[m]);
-  x.temp = a.hourly ? tavg : disagg_state_add(isday, a.nT, tavg, mt.fday_temp[m], mt.fnight_temp[m]);
-  x.prec = a.hourly ? pre : disagg_flux(isday, a.nT, pre, mt.fday_prec[m], mt.fnight_prec[m]);
+  const double pre = ldr(F + MHMHIP_F_PRE * fv, bc), tavg = ldr(F + MHMHIP_F_TAVG * fv, bc);
+  const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, bc) : 0.0;
+  const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, bc) : 0.0;
+  const double pd = pet_day<NH>(a, st, bv, bc, tavg, tmax, tmin);
+  x.pet = a.hourly ? pd : disagg_flux(a.nT, pd, st.f_pet);
+  x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
+  x.prec = a.hourly ? pre : disagg_flux(a.nT, pre, st.f_prec);
   if (st.init_sm) {
 #pragma unroll
     for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
   }
-  const double maxinter = a.P[(pr_maxinter(NH) + m) * a.ld + v];
-  chain_step<NH>(a, r, x, maxinter, mt.evap[m]);
-  store_states<NH>(a, v, r);
-  store_fluxes<NH>(a, v, x);
-  if (a.ring) a.ring[(t % a.ring_len) * a.ld + v] = x.total;
+  const double maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld, bv);
+  chain_step<NH>(a, r, x, maxinter, st.evap);
+  store_states<NH>(a, bv, r);
+  store_fluxes<NH>(a, bv, x);
+  if (a.ring) str(a.ring + (t % a.ring_len) * a.ld, bv, runoff_acc(x.total, ldr(a.area, bc), a.tst));
 }
 
 // ---------------------------------------------------------------------------
 // Fused path: nsteps model steps per launch with parameters and states held
-// in registers.  Forcing and Hargreaves PET are evaluated once per forcing day
-// (the reference recomputes the same value every hour).  Only L1 total runoff
-// leaves the lane each step; states and the last step's fluxes are written at
-// the end.
+// in registers.  Forcing and daily PET are read/evaluated once per forcing day
+// (the reference recomputes the same value every hour), maximum interception
+// once per month.  Only the routing input leaves the lane each step; states
+// and the last step's fluxes are written at the end.  Per-step fluxes are
+// scoped to the loop body so they do not stay live across steps.
 template <int NH>
-__global__ __launch_bounds__(256) void cell_fused_kernel(CellArgs a, MonthTables mt, int64_t t0,
-                                                         int32_t nsteps, int32_t step0) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
+                                                         int32_t step0) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
-  const int64_t v = (int64_t)blockIdx.y * a.sstride + c;
+  const uint32_t bc = c * 8u;
+  const uint32_t bv = (uint32_t)(blockIdx.y * a.sstride + c) * 8u;
   CellRegs<NH> r;
-  FluxRegs<NH> x;
-  load_params<NH>(a, v, r);
-  load_states<NH>(a, v, r);
+  load_params<NH>(a, bv, r);
+  load_states<NH>(a, bv, r);
+  const double area = a.ring ? ldr(a.area, bc) : 0.0;
   const int64_t fv = a.forcing_days * a.sstride;
-  int cached_day = -1, cached_doy = -1;
-  double pre = 0.0, tavg = 0.0, pd = 0.0;
+  int cached_day = -1, cached_doy = -1, cached_month = -1;
+  double pre = 0.0, tavg = 0.0, pd = 0.0, maxinter = 0.0;
+#pragma nounroll
   for (int32_t s = 0; s < nsteps; ++s) {
     const StepDev st = a.steps[step0 + s];
-    const int m = st.month0;
-    const bool isday = st.isday != 0;
     if (st.day != cached_day || st.doy != cached_doy) {
-      const double *F = a.F + (int64_t)st.day * a.sstride + c;
-      pre = F[MHMHIP_F_PRE * fv];
-      tavg = F[MHMHIP_F_TAVG * fv];
-      const double tmax = a.pet_case == 1 ? F[MHMHIP_F_TMAX * fv] : 0.0;
-      const double tmin = a.pet_case == 1 ? F[MHMHIP_F_TMIN * fv] : 0.0;
-      pd = pet_day<NH>(a, r, st, v, c, tavg, tmax, tmin);
+      const double *F = a.F + (int64_t)st.day * a.sstride;
+      pre = ldr(F + MHMHIP_F_PRE * fv, bc);
+      tavg = ldr(F + MHMHIP_F_TAVG * fv, bc);
+      const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, bc) : 0.0;
+      const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, bc) : 0.0;
+      pd = pet_day<NH>(a, st, bv, bc, tavg, tmax, tmin);
       cached_day = st.day;
       cached_doy = st.doy;
     }
-    x.pet = a.hourly ? pd : disagg_flux(isday, a.nT, pd, mt.fday_pet[m], mt.fnight_pet[m]);
-    x.temp = a.hourly ? tavg
-                      : disagg_state_add(isday, a.nT, tavg, mt.fday_temp[m], mt.fnight_temp[m]);
-    x.prec = a.hourly ? pre : disagg_flux(isday, a.nT, pre, mt.fday_prec[m], mt.fnight_prec[m]);
+    if (st.month0 != cached_month) {
+      maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld, bv);
+      cached_month = st.month0;
+    }
+    FluxRegs<NH> x;
+    x.pet = a.hourly ? pd : disagg_flux(a.nT, pd, st.f_pet);
+    x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
+    x.prec = a.hourly ? pre : disagg_flux(a.nT, pre, st.f_prec);
     if (st.init_sm) {
 #pragma unroll
       for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
     }
-    const double maxinter = a.P[(pr_maxinter(NH) + m) * a.ld + v];
-    chain_step<NH>(a, r, x, maxinter, mt.evap[m]);
-    if (a.ring) a.ring[((t0 + s) % a.ring_len) * a.ld + v] = x.total;
+    chain_step<NH>(a, r, x, maxinter, st.evap);
+    if (a.ring) str(a.ring + ((t0 + s) % a.ring_len) * a.ld, bv, runoff_acc(x.total, area, a.tst));
+    if (s == nsteps - 1) store_fluxes<NH>(a, bv, x);
   }
-  store_states<NH>(a, v, r);
-  store_fluxes<NH>(a, v, x);
+  store_states<NH>(a, bv, r);
 }
 
 // ---------------------------------------------------------------------------
 // One node, one step of L11_runoff_acc + add_inflow + L11_routing
 // (mo_mrm_pre_routing.f90:77-143,179-214; mo_mrm_routing.f90:425-472).
-__device__ __forceinline__ void route_node(const RouteArgs &r, int64_t p, int32_t set, int64_t t,
+__device__ __forceinline__ void route_node(const RouteArgs &r, uint32_t p, int32_t set, int64_t t,
                                            NodeMeta m) {
   const int64_t ld = (int64_t)r.n_sets * r.sstride;
-  const int64_t v = (int64_t)set * r.sstride + p;
+  const uint32_t bv = (uint32_t)(set * r.sstride + p) * 8u;
   const int32_t y = m.packed;
-  // L11_runoff_acc, map_flag with L1 = L11: qAcc = (0 + q*area) * 1000 / TST
-  const double q = r.ring[(t % r.ring_len) * ld + v];
-  double qout = ((0.0 + q * r.area[p]) * 1000.0) / r.tst;
+  // qOUT of this node and step (L11_runoff_acc, done by the cell kernel)
+  double qout = ldr(r.ring + (t % r.ring_len) * ld, bv);
   if (y & NF_INFLOW) {
     for (int32_t g = 0; g < r.n_inflow; ++g) {
-      if (r.inflow_node[g] == p) {
+      if (r.inflow_node[g] == (int32_t)p) {
         const double qi = r.inflow_q[(t - r.t_base) * r.n_inflow + g];
         qout = r.inflow_hw[g] ? qout + qi : qi;
       }
     }
   }
-  if (r.qOUT && t == r.t_hi - 1) r.qOUT[v] = qout;
+  if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qout);
   // qTIN(iNode,IT): upstream qTR in routing rank, then own qOUT
   const double *pub_t = r.pub + (t & 1) * ld + (int64_t)set * r.sstride + m.child;
   double s = 0.0;
@@ -261,23 +285,24 @@ __device__ __forceinline__ void route_node(const RouteArgs &r, int64_t p, int32_
   for (int c = 0; c < nc; ++c) s = s + pub_t[c];
   if (y & NF_QOUT) s = s + qout;
   if (y & NF_LINK) {
-    const double qtin1 = r.qTIN[v];
-    const double qtr1 = r.pub[((t - 1) & 1) * ld + v];
-    double qtr = qtr1 + r.C1[v] * (qtin1 - qtr1) + r.C2[v] * (s - qtin1);
+    const double qtin1 = ldr(r.qTIN, bv);
+    const double qtr1 = ldr(r.pub + ((t - 1) & 1) * ld, bv);
+    double qtr = qtr1 + ldr(r.C1, bv) * (qtin1 - qtr1) + ldr(r.C2, bv) * (s - qtin1);
     if (y & NF_CUT) qtr = 0.0;
-    r.pub[(t & 1) * ld + v] = qtr;
+    str(r.pub + (t & 1) * ld, bv, qtr);
   }
-  r.qTIN[v] = s;
+  str(r.qTIN, bv, s);
   if (y & NF_GAUGE) {
     for (int32_t g = 0; g < r.n_gauges; ++g)
-      if (r.gauge_node[g] == p)
+      if (r.gauge_node[g] == (int32_t)p)
         r.gauge_out[((t - r.t_base) * r.n_sets + set) * r.n_gauges + g] = s;
   }
 }
 
-__global__ __launch_bounds__(256) void route_diag_kernel(RouteArgs r, int64_t d) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= r.n_nodes) return;
+__global__ __launch_bounds__(256) void route_diag_kernel(RouteArgs r, int64_t d, uint32_t p_lo,
+                                                         uint32_t p_hi) {
+  const uint32_t p = p_lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p_hi) return;
   const NodeMeta m = r.meta[p];
   const int64_t t = d - (int64_t)(r.dmax - nm_depth(m.packed));
   if (t < r.t_lo || t >= r.t_hi) return;
@@ -294,11 +319,11 @@ __global__ __launch_bounds__(1024) void route_wg_kernel(RouteArgs r, const int64
                                                         int64_t d_hi, int32_t drain) {
   const int32_t dom = blockIdx.x;
   const int32_t set = blockIdx.y;
-  const int64_t lo = dom_start[dom], hi = dom_start[dom + 1];
+  const uint32_t lo = (uint32_t)dom_start[dom], hi = (uint32_t)dom_start[dom + 1];
   const int32_t dm = dom_dmax[dom];
   const int64_t d_end = drain ? d_lo + dm : d_hi;
   for (int64_t d = d_lo; d < d_end; ++d) {
-    for (int64_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
+    for (uint32_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
       const NodeMeta m = r.meta[p];
       const int64_t t = d - (int64_t)(dm - nm_depth(m.packed));
       if (t >= r.t_lo && t < r.t_hi) route_node(r, p, set, t, m);
@@ -310,50 +335,53 @@ __global__ __launch_bounds__(1024) void route_wg_kernel(RouteArgs r, const int64
 // ---------------------------------------------------------------------------
 // host launchers
 template <int NH>
-static hipError_t step_nh(const CellArgs &a, const StepDev &st, const MonthTables &mt, int64_t t,
-                          int block, int n_sets, hipStream_t s) {
+static hipError_t step_nh(const CellArgs &a, const StepDev &st, int64_t t, int block, int n_sets,
+                          hipStream_t s) {
   dim3 grid((unsigned)((a.n_cells + block - 1) / block), (unsigned)n_sets);
-  hipLaunchKernelGGL(cell_step_kernel<NH>, grid, dim3(block), 0, s, a, st, mt, t);
+  hipLaunchKernelGGL(cell_step_kernel<NH>, grid, dim3(block), 0, s, a, st, t);
   return hipGetLastError();
 }
 
 template <int NH>
-static hipError_t fused_nh(const CellArgs &a, const MonthTables &mt, int64_t t0, int nsteps,
-                           int step0, int block, int n_sets, hipStream_t s) {
+static hipError_t fused_nh(const CellArgs &a, int64_t t0, int nsteps, int step0, int block,
+                           int n_sets, hipStream_t s) {
   dim3 grid((unsigned)((a.n_cells + block - 1) / block), (unsigned)n_sets);
-  hipLaunchKernelGGL(cell_fused_kernel<NH>, grid, dim3(block), 0, s, a, mt, t0, nsteps, step0);
+  hipLaunchKernelGGL(cell_fused_kernel<NH>, grid, dim3(block), 0, s, a, t0, nsteps, step0);
   return hipGetLastError();
 }
 
-hipError_t launch_cell_step(int nh, const CellArgs &a, const StepDev &st, const MonthTables &mt,
-                            int64_t t, int block, int n_sets, hipStream_t s) {
+hipError_t launch_cell_step(int nh, const CellArgs &a, const StepDev &st, int64_t t, int block,
+                            int n_sets, hipStream_t s) {
   switch (nh) {
-    case 1: return step_nh<1>(a, st, mt, t, block, n_sets, s);
-    case 2: return step_nh<2>(a, st, mt, t, block, n_sets, s);
-    case 3: return step_nh<3>(a, st, mt, t, block, n_sets, s);
-    case 4: return step_nh<4>(a, st, mt, t, block, n_sets, s);
-    case 5: return step_nh<5>(a, st, mt, t, block, n_sets, s);
-    case 6: return step_nh<6>(a, st, mt, t, block, n_sets, s);
+    case 1: return step_nh<1>(a, st, t, block, n_sets, s);
+    case 2: return step_nh<2>(a, st, t, block, n_sets, s);
+    case 3: return step_nh<3>(a, st, t, block, n_sets, s);
+    case 4: return step_nh<4>(a, st, t, block, n_sets, s);
+    case 5: return step_nh<5>(a, st, t, block, n_sets, s);
+    case 6: return step_nh<6>(a, st, t, block, n_sets, s);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_cell_fused(int nh, const CellArgs &a, const MonthTables &mt, int64_t t0,
-                             int nsteps, int step0, int block, int n_sets, hipStream_t s) {
+hipError_t launch_cell_fused(int nh, const CellArgs &a, int64_t t0, int nsteps, int step0,
+                             int block, int n_sets, hipStream_t s) {
   switch (nh) {
-    case 1: return fused_nh<1>(a, mt, t0, nsteps, step0, block, n_sets, s);
-    case 2: return fused_nh<2>(a, mt, t0, nsteps, step0, block, n_sets, s);
-    case 3: return fused_nh<3>(a, mt, t0, nsteps, step0, block, n_sets, s);
-    case 4: return fused_nh<4>(a, mt, t0, nsteps, step0, block, n_sets, s);
-    case 5: return fused_nh<5>(a, mt, t0, nsteps, step0, block, n_sets, s);
-    case 6: return fused_nh<6>(a, mt, t0, nsteps, step0, block, n_sets, s);
+    case 1: return fused_nh<1>(a, t0, nsteps, step0, block, n_sets, s);
+    case 2: return fused_nh<2>(a, t0, nsteps, step0, block, n_sets, s);
+    case 3: return fused_nh<3>(a, t0, nsteps, step0, block, n_sets, s);
+    case 4: return fused_nh<4>(a, t0, nsteps, step0, block, n_sets, s);
+    case 5: return fused_nh<5>(a, t0, nsteps, step0, block, n_sets, s);
+    case 6: return fused_nh<6>(a, t0, nsteps, step0, block, n_sets, s);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_route_diag(const RouteArgs &r, int64_t d, hipStream_t s) {
-  dim3 grid((unsigned)((r.n_nodes + 255) / 256), (unsigned)r.n_sets);
-  hipLaunchKernelGGL(route_diag_kernel, grid, dim3(256), 0, s, r, d);
+hipError_t launch_route_diag(const RouteArgs &r, int64_t d, int64_t p_lo, int64_t p_hi,
+                             hipStream_t s) {
+  if (p_lo < 0 || p_hi > r.n_nodes || p_hi <= p_lo) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((p_hi - p_lo + 255) / 256), (unsigned)r.n_sets);
+  hipLaunchKernelGGL(route_diag_kernel, grid, dim3(256), 0, s, r, d, (uint32_t)p_lo,
+                     (uint32_t)p_hi);
   return hipGetLastError();
 }
 

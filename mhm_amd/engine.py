@@ -197,7 +197,7 @@ class Model:
         return g[:, :self.n_gauges]
 
     def run(self, steps, mode=N.RUN_FUSED, steps_per_launch=0, block=0, route_mode=0,
-            want_gauges=True):
+            want_gauges=True, use_graph=False):
         """steps: ctypes array of mhmhip_step.  Returns gauges (n_steps, n_sets, n_gauges)."""
         n = len(steps)
         if not isinstance(steps, C.Array):
@@ -207,13 +207,15 @@ class Model:
         opts.steps_per_launch = steps_per_launch
         opts.block = block
         opts.route_mode = route_mode
+        opts.use_graph = 1 if use_graph else 0
         g = np.zeros((n, self.n_sets, max(self.n_gauges, 1))) if want_gauges else None
         self._chk(N.lib().mhmhip_run(self._ctx, steps, n, C.byref(opts), N.dptr(g)))
         return None if g is None else g[:, :, :self.n_gauges]
 
     # -- measurement ---------------------------------------------------------------
-    def set_profiling(self, on=True):
-        self._chk(N.lib().mhmhip_set_profiling(self._ctx, 1 if on else 0))
+    def set_profiling(self, on=True, route_stride=1):
+        """Kernel timing on/off; routing diagonals timed one in route_stride."""
+        self._chk(N.lib().mhmhip_set_profiling(self._ctx, max(1, int(route_stride)) if on else 0))
 
     def profile(self):
         """{kernel: (device ms, launches, work units)} accumulated since set_profiling(True)."""

@@ -75,6 +75,27 @@ def test_run_in_pieces_equals_one_run():
     m2.close()
 
 
+def test_graph_replays_match_eager_and_oracle():
+    """The time loop captured as a hipGraph and replayed on later chunks of the
+    run (new step tables, eager reads in between) equals the eager loop and the
+    oracle on every chunk."""
+    _log("case: graph replays")
+    dom = synth.make_domain(18, 22, seed=12)
+    m1, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=48, seed=5)
+    m2, _, _, _ = make_pair([dom], n_sets=2, n_steps=48, seed=5)
+    for a in range(0, 48, 12):
+        g1 = m1.run(steps[a:a + 12], mode=1, steps_per_launch=5)
+        g2 = m2.run(steps[a:a + 12], mode=1, steps_per_launch=5, use_graph=True)
+        go = _oracle_series(sim, recs[a:a + 12])
+        assert np.array_equal(g1, g2), a
+        assert rel_close(g2, go, RTOL, ATOL)[0], a
+        for name in ("SOILMOIST", "UNSATSTW", "TOTAL_RUNOFF"):
+            assert np.array_equal(m1.get(name, 1), m2.get(name, 1)), (a, name)
+        assert compare_routing(m2, sim, RTOL, ATOL) == [], a
+    m1.close()
+    m2.close()
+
+
 def test_packed_domains_match_oracle():
     """Uneven basins with their own parameters packed in one context."""
     _log("case: packed domains")
