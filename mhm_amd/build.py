@@ -2,6 +2,7 @@
 
   mhm_amd/libmhmhip.so  HIP kernels + C-ABI (include/mhmhip.h), gfx950
   mhm_amd/libmhmsyn.so  host-only synthetic-domain tools (priority-flood D8)
+  mhm_amd/fortran/mhmhip_driver  Fortran host on the iso_c_binding module
 
 Run ``python -m mhm_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -80,9 +81,33 @@ def build_syn():
     return SYN_LIB
 
 
+FORTRAN_DIR = os.path.join(PKG, "fortran")
+FORTRAN_DRIVER = os.path.join(FORTRAN_DIR, "mhmhip_driver")
+
+
+def build_fortran():
+    """iso_c_binding module + the Fortran driver, linked to libmhmhip.so."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        return None
+    obj = os.path.join(OBJ, "fortran")
+    os.makedirs(obj, exist_ok=True)
+    mods = ["mo_mhmhip.f90", "mhmhip_driver.f90"]
+    objs = []
+    for m in mods:
+        o = os.path.join(obj, m + ".o")
+        _run([fc, "-O2", "-J", obj, "-c", os.path.join(FORTRAN_DIR, m), "-o", o])
+        objs.append(o)
+    tmp = FORTRAN_DRIVER + ".tmp"
+    _run([fc] + objs + ["-L" + PKG, "-lmhmhip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+    os.replace(tmp, FORTRAN_DRIVER)
+    return FORTRAN_DRIVER
+
+
 def build_all(verbose=False):
     build_syn()
     build_hip(verbose)
+    build_fortran()
 
 
 if __name__ == "__main__":
