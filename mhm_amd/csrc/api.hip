@@ -81,13 +81,13 @@ struct mhmhip_ctx {
 
   // time loop captured as a hipGraph, replayed while its key matches
   struct GraphKey {
-    int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1;
+    int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1, prof = -1;
     const void *ring = nullptr, *gauge = nullptr, *steps = nullptr, *inflow = nullptr;
     int64_t ring_len = 0;
     bool operator==(const GraphKey &o) const {
       return nsteps == o.nsteps && mode == o.mode && S == o.S && block == o.block && wg == o.wg &&
-             ring == o.ring && gauge == o.gauge && steps == o.steps && inflow == o.inflow &&
-             ring_len == o.ring_len;
+             prof == o.prof && ring == o.ring && gauge == o.gauge && steps == o.steps &&
+             inflow == o.inflow && ring_len == o.ring_len;
     }
   };
   GraphKey gkey;
@@ -100,6 +100,12 @@ struct mhmhip_ctx {
     hipEvent_t a, b;
     double units;
   };
+  // With profiling on, a captured graph holds event-record nodes around the
+  // timed launches; these events belong to the graph and are read after
+  // every replay.
+  bool capturing = false;
+  std::vector<Pending> graph_events;
+  int64_t graph_all[MHMHIP_K_COUNT] = {0, 0, 0, 0};
   bool prof = false;
   int prof_stride = 1;         // time every prof_stride-th routing launch
   int64_t route_launches = 0;  // routing launches seen while profiling
@@ -653,15 +659,24 @@ hipError_t ev_get(mhmhip_ctx *x, hipEvent_t *e) {
 template <typename Fn>
 hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
   if (!x->prof) return f();
-  x->prof_all[kind] += 1;
+  (x->capturing ? x->graph_all : x->prof_all)[kind] += 1;
   if (kind == MHMHIP_K_ROUTE_DIAG && (x->route_launches++ % x->prof_stride) != 0) return f();
   mhmhip_ctx::Pending p{kind, nullptr, nullptr, units};
   hipError_t e;
-  if ((e = ev_get(x, &p.a)) != hipSuccess || (e = ev_get(x, &p.b)) != hipSuccess) return e;
+  if (x->capturing) {
+    if ((e = hipEventCreate(&p.a)) != hipSuccess) return e;
+    if ((e = hipEventCreate(&p.b)) != hipSuccess) {
+      (void)hipEventDestroy(p.a);
+      return e;
+    }
+    x->graph_events.push_back(p);  // owned by the graph from here on
+  } else if ((e = ev_get(x, &p.a)) != hipSuccess || (e = ev_get(x, &p.b)) != hipSuccess) {
+    return e;
+  }
   if ((e = hipEventRecord(p.a, x->stream)) != hipSuccess) return e;
   if ((e = f()) != hipSuccess) return e;
   if ((e = hipEventRecord(p.b, x->stream)) != hipSuccess) return e;
-  x->pending.push_back(p);
+  if (!x->capturing) x->pending.push_back(p);
   return hipSuccess;
 }
 
@@ -677,6 +692,19 @@ int prof_collect(mhmhip_ctx *x) {
     x->ev_pool.push_back(p.b);
   }
   x->pending.clear();
+  return MHMHIP_OK;
+}
+
+// After a replay of a graph captured with profiling on and a stream join.
+int prof_collect_graph(mhmhip_ctx *x) {
+  for (auto &p : x->graph_events) {
+    float ms = 0.f;
+    CHECK_HIP(x, hipEventElapsedTime(&ms, p.a, p.b), "profile");
+    x->prof_ms[p.kind] += ms;
+    x->prof_n[p.kind] += 1;
+    x->prof_units[p.kind] += p.units;
+  }
+  for (int k = 0; k < MHMHIP_K_COUNT; ++k) x->prof_all[k] += x->graph_all[k];
   return MHMHIP_OK;
 }
 
@@ -741,6 +769,12 @@ int normalize_qtr(mhmhip_ctx *x) {
 void drop_graph(mhmhip_ctx *x) {
   if (x->graph_exec) (void)hipGraphExecDestroy(x->graph_exec);
   if (x->graph) (void)hipGraphDestroy(x->graph);
+  for (auto &p : x->graph_events) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  x->graph_events.clear();
+  for (int k = 0; k < MHMHIP_K_COUNT; ++k) x->graph_all[k] = 0;
   x->graph_exec = nullptr;
   x->graph = nullptr;
   x->gkey = mhmhip_ctx::GraphKey();
@@ -1129,7 +1163,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     if (route) CHECK_RC(route_diagonals(x, r, nsteps, 0, true, wg));
     return MHMHIP_OK;
   };
-  const bool graph = o.use_graph && !x->prof && o.mode == MHMHIP_RUN_FUSED;
+  const bool graph = o.use_graph && o.mode == MHMHIP_RUN_FUSED;
   if (graph) {
     mhmhip_ctx::GraphKey key;
     key.nsteps = nsteps;
@@ -1137,6 +1171,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     key.S = S;
     key.block = block;
     key.wg = wg ? 1 : 0;
+    key.prof = x->prof ? x->prof_stride : 0;
     key.ring = route ? x->d_ring : nullptr;
     key.gauge = route ? x->d_gauge_buf : nullptr;
     key.steps = x->d_steps;
@@ -1145,7 +1180,9 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     if (!x->graph_exec || !(key == x->gkey)) {
       drop_graph(x);
       CHECK_HIP(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal), "capture");
+      x->capturing = true;
       const int rc = enqueue();
+      x->capturing = false;
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(x->stream, &g);
       if (rc != MHMHIP_OK) {
@@ -1174,6 +1211,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
               "download gauges");
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "run");
+  if (graph && x->prof) CHECK_RC(prof_collect_graph(x));
   return prof_collect(x);
 }
 

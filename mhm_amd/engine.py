@@ -212,6 +212,23 @@ class Model:
         self._chk(N.lib().mhmhip_run(self._ctx, steps, n, C.byref(opts), N.dptr(g)))
         return None if g is None else g[:, :, :self.n_gauges]
 
+    # -- state snapshots (restart-style round trip of the device states) ------------
+    STATE_NAMES = ("INTER", "SNOWPACK", "SEALSTW", "UNSATSTW", "SATSTW", "SOILMOIST")
+
+    def snapshot(self):
+        """Host copy of every state (cells and routing) of every parameter set."""
+        snap = {"cells": [{k: self.get(k, s) for k in self.STATE_NAMES} for s in range(self.n_sets)]}
+        if self.route and getattr(self, "_net_arrays", None) is not None:
+            snap["routing"] = [self.routing_state(s) for s in range(self.n_sets)]
+        return snap
+
+    def restore(self, snap):
+        for s, cells in enumerate(snap["cells"]):
+            for k, v in cells.items():
+                self.set_var(k, v, s)
+        for s, rs in enumerate(snap.get("routing", [])):
+            self.set_routing_state(rs["qTIN"], rs["qTR"], s)
+
     # -- measurement ---------------------------------------------------------------
     def set_profiling(self, on=True, route_stride=1):
         """Kernel timing on/off; routing diagonals timed one in route_stride."""

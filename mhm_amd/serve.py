@@ -1,0 +1,229 @@
+"""Serving loop: simulation requests of different basin sizes, horizons and
+parameter-set counts arrive over time and are run with continuous batching.
+
+* Admission by HBM footprint: a request is admitted when a slot is free and
+  its estimated device footprint fits the budget next to the requests on the
+  device; otherwise it waits (first-fit over the queue, so a small request
+  is not held behind a large one).
+* Each admitted request owns a device context (its states stay resident for
+  its whole horizon) and a worker thread; the C-ABI releases the GIL and each
+  context runs on its own HIP stream, so the work of requests in flight
+  overlaps on the GPU.  Requests advance in chunks of `chunk_steps`, and a
+  request that arrives while others run is admitted at once when memory
+  allows -- not after the batch drains.
+* No state or reduction is shared between requests, so a result is
+  bitwise the same alone and in the mix.
+
+Latency = completion - arrival; p50/p99 are reported beside cell-steps/s.
+"""
+import math
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from . import calendar, engine, parameters as PM, synth
+
+
+@dataclass
+class Request:
+    rid: int
+    domain: synth.Domain            # with forcing (synth.make_forcing)
+    gammas: np.ndarray              # (n_sets, n_gamma)
+    n_steps: int
+    timestep: int = 24
+    start: object = None
+    arrival: float = 0.0            # seconds after the stream starts
+
+    @property
+    def cell_steps(self):
+        return self.domain.n_cells * self.gammas.shape[0] * self.n_steps
+
+
+@dataclass
+class Result:
+    rid: int
+    gauges: np.ndarray              # (n_steps, n_sets, n_gauges)
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, dmax, steps_per_launch,
+                   n_steps, n_gauges=1):
+    """Device footprint of one context (mirrors the allocations of api.hip)."""
+    sstride = (n_cells + 63) // 64 * 64
+    ld = sstride * n_sets
+    rows = (19 + 5 * n_horizons + 24) + (5 + n_horizons) + (17 + 2 * n_horizons)
+    routing = 8 * n_cells + 8 * n_cells + 5 * 8 * ld  # meta, area, C1, C2, qTIN, qTR x2, qOUT
+    ring = (steps_per_launch + dmax + 2) * ld * 8
+    return (rows * ld * 8 + 5 * n_forcing_days * sstride * 8 + routing + ring
+            + n_steps * 40 + n_steps * n_sets * n_gauges * 8)
+
+
+def prepare_model(req, device=0):
+    """Context with network, parameters, default states and forcing of a request."""
+    dom = req.domain
+    n_days = calendar.n_forcing_days_needed(req.n_steps)
+    m = engine.Model([dom], n_sets=req.gammas.shape[0], timestep=req.timestep, route=True,
+                     n_forcing_days=n_days, device=device)
+    m.bind_network()
+    for s, g in enumerate(req.gammas):
+        m.set_params(synth.cell_params(dom, g), s)
+        m.set_routing_params(PM.routing_gamma(g), s)
+    m.default_states()
+    m.set_domain_forcing()
+    return m
+
+
+def run_alone(req, chunk_steps=73, device=0):
+    """The request on its own (reference result for the mix)."""
+    m = prepare_model(req, device)
+    steps = calendar.step_table(req.start, req.n_steps, req.timestep)
+    parts = []
+    for a in range(0, req.n_steps, chunk_steps):
+        parts.append(m.run(steps[a:a + chunk_steps], mode=N.RUN_FUSED,
+                           steps_per_launch=chunk_steps))
+    m.close()
+    return np.concatenate(parts)
+
+
+class Server:
+    def __init__(self, hbm_budget_bytes, max_slots=8, chunk_steps=73, device=0, runner=None,
+                 footprint=None):
+        """runner(req) -> gauges runs one request (default: its own device
+        context, chunked fused runs); footprint(req) -> bytes (default:
+        estimate_bytes of its context)."""
+        self.runner = runner or (lambda req: run_alone(req, chunk_steps, device))
+        self.footprint = footprint or self._footprint
+        self.budget = int(hbm_budget_bytes)
+        self.max_slots = int(max_slots)
+        self.chunk = int(chunk_steps)
+        self.device = device
+        self._lock = threading.Condition()
+        self._queue = []
+        self._in_use = 0
+        self._running = 0
+        self._results = {}
+        self._errors = []
+        self._t0 = None
+        self.peak_slots = 0
+        self.peak_bytes = 0
+
+    def _footprint(self, req):
+        dom = req.domain
+        return estimate_bytes(dom.n_cells, req.gammas.shape[0], dom.horizon_depth.shape[0],
+                              calendar.n_forcing_days_needed(req.n_steps), dom.dmax,
+                              self.chunk, req.n_steps)
+
+    def submit(self, req):
+        fp = self.footprint(req)
+        if fp > self.budget:
+            raise ValueError(f"request {req.rid} needs {fp} B of device memory, over the "
+                             f"budget of {self.budget} B")
+        with self._lock:
+            self._queue.append((req, fp, time.perf_counter()))
+            self._lock.notify_all()
+
+    def _worker(self, req, fp, t_arr, t_adm):
+        try:
+            g = self.runner(req)
+            t_fin = time.perf_counter()
+            with self._lock:
+                self._results[req.rid] = Result(req.rid, g, t_arr - self._t0, t_adm - self._t0,
+                                                t_fin - self._t0)
+        except Exception as e:  # pragma: no cover - surfaced by serve()
+            with self._lock:
+                self._errors.append((req.rid, repr(e)))
+        finally:
+            with self._lock:
+                self._in_use -= fp
+                self._running -= 1
+                self._lock.notify_all()
+
+    def _admit_loop(self, n_total, stop):
+        threads = []
+        finished = False
+        while not finished:
+            with self._lock:
+                while True:
+                    if len(self._results) + len(self._errors) >= n_total or stop.is_set():
+                        finished = True
+                        break
+                    pick = None
+                    if self._running < self.max_slots:
+                        for i, (req, fp, t_arr) in enumerate(self._queue):
+                            if self._in_use + fp <= self.budget:
+                                pick = i
+                                break
+                    if pick is not None:
+                        break
+                    self._lock.wait(timeout=0.05)
+                if finished:
+                    break
+                req, fp, t_arr = self._queue.pop(pick)
+                self._in_use += fp
+                self._running += 1
+                self.peak_slots = max(self.peak_slots, self._running)
+                self.peak_bytes = max(self.peak_bytes, self._in_use)
+            t = threading.Thread(target=self._worker, args=(req, fp, t_arr, time.perf_counter()),
+                                 daemon=True)
+            t.start()
+            threads.append(t)
+        for t in threads:
+            t.join()
+
+    def serve(self, requests):
+        """Feed requests at their arrival times, run them, return results by id."""
+        self._t0 = time.perf_counter()
+        stop = threading.Event()
+        adm = threading.Thread(target=self._admit_loop, args=(len(requests), stop), daemon=True)
+        adm.start()
+        for req in sorted(requests, key=lambda r: r.arrival):
+            wait = self._t0 + req.arrival - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            self.submit(req)
+        adm.join()
+        stop.set()
+        if self._errors:
+            raise RuntimeError(f"requests failed: {self._errors}")
+        self.wall = time.perf_counter() - self._t0
+        return dict(self._results)
+
+
+def poisson_requests(n, qps, seed=0, cells=(1000, 100000), horizon=(30, 365), sets=(1, 64),
+                     timestep=24, start=None):
+    """Requests with Poisson arrivals: basin size log-uniform, horizon and
+    parameter-set count uniform within the given bounds."""
+    import datetime as dt
+    rng = np.random.default_rng(seed)
+    start = start or dt.datetime(1990, 1, 1)
+    t = 0.0
+    reqs = []
+    for i in range(n):
+        t += rng.exponential(1.0 / qps)
+        nc = int(math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1]))))
+        nr = max(2, int(round(math.sqrt(nc))))
+        dom = synth.make_domain(nr, max(2, nc // nr), seed=seed * 10007 + i)
+        h = int(rng.integers(horizon[0], horizon[1] + 1))
+        synth.make_forcing(dom, calendar.n_forcing_days_needed(h), seed * 10007 + 5000 + i)
+        k = int(rng.integers(sets[0], sets[1] + 1))
+        gam = PM.draw(rng, k)
+        reqs.append(Request(i, dom, gam, h, timestep, start, t))
+    return reqs
+
+
+def summarize(results, requests, wall):
+    lat = np.array([results[r.rid].latency for r in requests])
+    cs = sum(r.cell_steps for r in requests)
+    return {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+            "cell_steps_per_s": cs / wall, "wall_s": wall}

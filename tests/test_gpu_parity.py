@@ -96,6 +96,25 @@ def test_graph_replays_match_eager_and_oracle():
     m2.close()
 
 
+def test_profiling_inside_graph_replays():
+    """Kernel timing recorded as graph nodes: every replay adds its launches."""
+    _log("case: graph profiling")
+    dom = synth.make_domain(40, 40, seed=13)
+    m1, _, steps, _ = make_pair([dom], n_sets=1, n_steps=48, seed=6)
+    m2, _, _, _ = make_pair([dom], n_sets=1, n_steps=48, seed=6)
+    m2.set_profiling(True, 3)
+    for a in (0, 24):
+        g1 = m1.run(steps[a:a + 24], mode=1, steps_per_launch=8, route_mode=1)
+        g2 = m2.run(steps[a:a + 24], mode=1, steps_per_launch=8, route_mode=1, use_graph=True)
+        assert np.array_equal(g1, g2)
+    p = m2.profile()
+    assert p["cell_fused"][1] == 6 and p["cell_fused"][0] > 0.0
+    assert p["cell_fused"][2] == pytest.approx(6 * 8 * dom.n_cells)
+    assert p["route_diag"][1] > 0 and p["route_diag"][0] > 0.0
+    m1.close()
+    m2.close()
+
+
 def test_packed_domains_match_oracle():
     """Uneven basins with their own parameters packed in one context."""
     _log("case: packed domains")
