@@ -1,29 +1,42 @@
 """Benchmark of the mHM + mRM hot path on one MI355X.
 
-Workload: one synthetic basin of 1024 x 1024 L1 cells (L0 = L1 = L11,
-1 km, D8 network on an fBm DEM, ~1400 routing levels), 3 soil horizons,
-hourly time step for 90 days (2160 steps), Hargreaves PET on daily forcing,
-Muskingum routing.  One bench step = the full 90-day time loop (cells and
-routing) over that basin with all inputs resident in HBM.
+Headline workload: one synthetic basin of 1024 x 1024 L1 cells (L0 = L1 =
+L11, 1 km, D8 network on an fBm DEM, ~1400 routing levels), 3 soil
+horizons, hourly time step for 90 days (2160 steps), Hargreaves PET on daily
+forcing, Muskingum routing.  One bench step = the full 90-day time loop
+(cells and routing) over that basin with all inputs resident in HBM.
 
-Prints one JSON line: cell-steps/s of the time loop (plus plain-path and
-parameter-ensemble figures), the roofline of the dominant kernel measured
-with HIP events inside the timed region, and the oracle's OpenMP CPU rate
-on a bounded sample of the same workload.
+Besides the headline (fused time loop) the same JSON line carries
+  plain       the one-step-per-launch path on the same basin (every flux
+              written every step), with its HBM roofline
+  ensemble    256 parameter sets x 4096 cells, daily x 3650, settings
+              chosen by the autotuner
+  multi_basin 64 basins of 500..200k cells packed in one context, hourly x 720
+  serving     Poisson request stream (1k..100k cells, 30..365 daily steps,
+              1..64 sets), continuous batching, p50/p99 latency
+  cpu_baseline  the oracle (C restatement, OpenMP) on a bounded sample of
+              the headline workload, on this host's cores
+
+`--summarize-pmc FETCH_DIR WRITE_DIR` turns two rocprofv3 --pmc runs
+(FETCH_SIZE, then WRITE_SIZE, each over `bench.py --pmc-workload`) into
+profiles/traffic.json: HBM bytes per launch of each kernel, scaled by what
+the same counters read for a plain copy of known size.
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
 import time
-
-import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+ROUTE_STRIDE = 16          # one routing launch in 16 is bracketed by events
+COPY_CALIB_BYTES = 1 << 30  # 1 GiB read + 1 GiB written by the calibration copy
 
 
 def log(*a):
@@ -39,18 +52,20 @@ def bytes_per_unit(kernel, steps_per_launch, steps_per_day):
         # forcing once per forcing day, total runoff out every step, params + states
         # in/out once per launch: 32/F + 8 + (128 + 256)/S
         return 32.0 / steps_per_day + 8.0 + 384.0 / steps_per_launch
-    # routing node-step: qOUT source, C1, C2, own qTIN/qTR, upstream qTR, node
-    # metadata read; qTIN, qTR written
-    return 80.0
+    # routing node-step: qOUT from the ring 8, C1 + C2 16, own qTIN read + write 16,
+    # own qTR(IT1) read 8 + qTR(IT2) write 8, upstream qTR read 8 (one per edge),
+    # node metadata (child index + packed depth/flags) 8
+    return 72.0
 
 
-def roofline(prof, steps_per_launch, steps_per_day, traffic):
-    best = max(prof.items(), key=lambda kv: kv[1][0])
-    name, (ms, launches, units) = best
-    if launches == 0 or ms <= 0:
-        return None, {}
-    bpu = bytes_per_unit(name, steps_per_launch, steps_per_day)
-    achieved = units * bpu / (ms * 1e-3) / 1e9
+def load_json(path):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def kernel_table(prof, steps_per_launch, steps_per_day):
     detail = {}
     for k, (kms, kn, ku) in prof.items():
         if kn:
@@ -59,11 +74,35 @@ def roofline(prof, steps_per_launch, steps_per_day, traffic):
                          "avg_us": round(kms * 1e3 / kn, 3),
                          "bytes_per_unit": round(b, 3),
                          "GBs": round(ku * b / (kms * 1e-3) / 1e9, 1)}
+    return detail
+
+
+def hbm_roofline(name, stats, steps_per_launch, steps_per_day, traffic):
+    ms, launches, units = stats
+    if launches == 0 or ms <= 0:
+        return None
+    bpu = bytes_per_unit(name, steps_per_launch, steps_per_day)
+    achieved = units * bpu / (ms * 1e-3) / 1e9
     t = traffic.get(name)
     return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": t, "avg_launch_us": round(ms * 1e3 / launches, 3),
-            "algorithmic_bytes_per_launch": round(units * bpu / launches, 1)}, detail
+            "traffic": None if t is None else t.get("bytes_per_launch"),
+            "avg_launch_us": round(ms * 1e3 / launches, 3),
+            "algorithmic_bytes_per_launch": round(units * bpu / launches, 1)}
+
+
+def fp64_roofline(name, stats, flops_per_unit):
+    """FP64 VALU roofline: flops per cell-step (PMC count of the FP64 VALU
+    instructions of this kernel on this workload, 64 lanes each, FMA = 2)
+    x cell-steps per launch / measured launch time."""
+    ms, launches, units = stats
+    if launches == 0 or ms <= 0 or not flops_per_unit:
+        return None
+    achieved = units * flops_per_unit / (ms * 1e-3) / 1e12
+    return {"bound": "fp64-valu", "kernel": name, "achieved": round(achieved, 2),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+            "flops_per_cell_step": round(flops_per_unit, 1),
+            "avg_launch_us": round(ms * 1e3 / launches, 3)}
 
 
 def cpu_baseline(dom, gam, n_days, sample_steps, threads):
@@ -85,9 +124,179 @@ def cpu_baseline(dom, gam, n_days, sample_steps, threads):
     dt = time.perf_counter() - t0
     n = dom.n_cells * (sample_steps - 1)
     return {"value": n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle OpenMP ({threads} threads): steps 2..{sample_steps} of the same "
-                      f"{dom.nrows}x{dom.ncols} basin, meteo + mHM + serial mRM routing, "
+            "sample": f"oracle, OpenMP {threads} threads: hourly steps 2..{sample_steps} of the "
+                      f"same {dom.nrows}x{dom.ncols} basin, meteo + mHM + serial mRM routing, "
                       f"{dt:.1f} s"}
+
+
+def timed_runs(model, steps, k, **kw):
+    model.sync()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        model.run(steps, **kw)
+    model.sync()
+    return time.perf_counter() - t0
+
+
+def bench_ensemble(N, workloads, autotune, traffic):
+    t0 = time.perf_counter()
+    em, esteps, edom, _ = workloads.plain_domain(64, 64, 3650, timestep=24, seed=3, n_sets=256)
+    setup = time.perf_counter() - t0
+    best, timings = autotune.autotune(em, esteps, trial_steps=365,
+                                      cands=list(autotune.candidates(em, S_values=(73, 365, 730))),
+                                      repeats=2)
+    snap = em.snapshot()
+    em.set_profiling(True, ROUTE_STRIDE)
+    em.run(esteps[:730], want_gauges=False, **best)  # graph capture, if chosen
+    em.restore(snap)
+    em.set_profiling(True, ROUTE_STRIDE)
+    te = timed_runs(em, esteps, 1, **best)
+    eprof = em.profile()
+    em.close()
+    units = 256 * edom.n_cells * 3650
+    S = best["steps_per_launch"]
+    return {"value": units / te, "unit": "cell-steps/s",
+            "workload": "256 parameter sets x 4096 cells (64x64), daily x 3650, shared forcing",
+            "settings": best, "autotune_trials": len(timings), "setup_s": round(setup, 1),
+            "kernels": kernel_table(eprof, S, 1)}
+
+
+def bench_multi_basin(N, workloads, S):
+    t0 = time.perf_counter()
+    mm, msteps, mdoms, _ = workloads.multi_basin(64, 500, 200000, 720, seed=7)
+    setup = time.perf_counter() - t0
+    cells = sum(d.n_cells for d in mdoms)
+    kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=True)
+    mm.run(msteps, **kw)
+    mm.set_profiling(True, ROUTE_STRIDE)
+    mm.run(msteps, **kw)
+    mm.set_profiling(True, ROUTE_STRIDE)
+    tm = timed_runs(mm, msteps, 1, **kw)
+    mprof = mm.profile()
+    mm.close()
+    return {"value": cells * len(msteps) / tm, "unit": "cell-steps/s",
+            "workload": f"64 basins, log-uniform 500..200k cells ({cells} in all, largest "
+                        f"{max(d.n_cells for d in mdoms)}, max depth {max(d.dmax for d in mdoms)}),"
+                        f" own DEM/network/forcing/parameters each, hourly x 720, fused S={S}",
+            "setup_s": round(setup, 1), "kernels": kernel_table(mprof, S, 24)}
+
+
+def bench_serving(serve, n_req, qps, seed=11):
+    reqs = serve.poisson_requests(n_req, qps=qps, seed=seed)
+    # warm the code paths (module loads, first context) outside the stream
+    serve.run_alone(serve.poisson_requests(1, qps=1.0, seed=99, cells=(1000, 1000),
+                                           horizon=(30, 30), sets=(1, 1))[0])
+    srv = serve.Server(hbm_budget_bytes=64 << 30, max_slots=8, chunk_steps=73)
+    res = srv.serve(reqs)
+    s = serve.summarize(res, reqs, srv.wall)
+    s.update({"qps": qps, "peak_slots": srv.peak_slots,
+              "peak_admitted_GiB": round(srv.peak_bytes / 2**30, 2),
+              "workload": f"{n_req} Poisson arrivals at {qps} req/s: log-uniform 1k..100k cells, "
+                          "30..365 daily steps, 1..64 parameter sets; budget 64 GiB, 8 slots"})
+    return s
+
+
+# -- PMC traffic ------------------------------------------------------------------
+
+def pmc_workload(args):
+    """Short run for counter collection: the headline kernels over two days of
+    the same basin, then a plain device copy of COPY_CALIB_BYTES."""
+    import torch
+    from mhm_amd import _native as N, workloads
+    model, steps, dom, _ = workloads.plain_domain(args.grid, args.grid, 48, seed=1)
+    model.run(steps, mode=N.RUN_FUSED, steps_per_launch=args.steps_per_launch)
+    model.run(steps[:2], mode=N.RUN_PLAIN)
+    model.sync()
+    model.close()
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    with open(os.path.join(ROOT, "gpurun_out", "pmc_meta.json"), "w") as f:
+        json.dump({"cells": dom.n_cells, "steps_per_launch": args.steps_per_launch,
+                   "fused_launches": len(steps) // args.steps_per_launch, "plain_launches": 2}, f)
+    a = torch.empty(COPY_CALIB_BYTES // 8, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"pmc workload done: cells={dom.n_cells}")
+
+
+def _pmc_rows(d):
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _per_kernel(rows, counter):
+    """{kernel short name: [value per dispatch]} for one counter."""
+    out = {}
+    for r in rows:
+        if r.get("Counter_Name") != counter:
+            continue
+        name = r["Kernel_Name"]
+        short = ("copy" if ("copy" in name.lower() or "elementwise" in name.lower())
+                 and "mhm" not in name.lower() else None)
+        for k in ("cell_fused", "cell_step", "route_diag", "route_wg"):
+            if k in name:
+                short = k
+        if short:
+            out.setdefault(short, []).append(float(r["Counter_Value"]))
+    return out
+
+
+def summarize_pmc(fetch_dir, write_dir, out_path):
+    fetch = _per_kernel(_pmc_rows(fetch_dir), "FETCH_SIZE")
+    write = _per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
+    if "copy" not in fetch or "copy" not in write:
+        raise SystemExit("calibration copy not found in the counter runs")
+    kib = COPY_CALIB_BYTES / 1024.0
+    f_scale = kib / max(fetch["copy"])
+    w_scale = kib / max(write["copy"])
+    res = {"_calibration": {"copy_bytes": COPY_CALIB_BYTES, "fetch_kib": max(fetch["copy"]),
+                            "write_kib": max(write["copy"]), "fetch_scale": f_scale,
+                            "write_scale": w_scale}}
+    for k in fetch:
+        if k == "copy" or k not in write:
+            continue
+        fv = sum(fetch[k]) / len(fetch[k]) * f_scale * 1024.0
+        wv = sum(write[k]) / len(write[k]) * w_scale * 1024.0
+        res[k] = {"bytes_per_launch": round(fv + wv), "fetch_bytes": round(fv),
+                  "write_bytes": round(wv), "dispatches": len(fetch[k])}
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+
+
+FP64_COUNTERS = {"SQ_INSTS_VALU_ADD_F64": 64, "SQ_INSTS_VALU_MUL_F64": 64,
+                 "SQ_INSTS_VALU_FMA_F64": 128, "SQ_INSTS_VALU_TRANS_F64": 64}
+
+
+def summarize_flops(pmc_dir, meta_path, out_path):
+    """FP64 lane-ops per cell-step of the cell kernels from one --pmc run of
+    the FP64 VALU instruction counters (per wave instruction: 64 lanes, FMA 2)."""
+    rows = _pmc_rows(pmc_dir)
+    with open(meta_path) as f:
+        meta = json.load(f)
+    units = {"cell_fused": meta["cells"] * meta["steps_per_launch"], "cell_step": meta["cells"]}
+    res = {}
+    for k, u in units.items():
+        tot, parts = 0.0, {}
+        for c, w in FP64_COUNTERS.items():
+            v = _per_kernel(rows, c).get(k)
+            if not v:
+                continue
+            parts[c] = sum(v) / len(v)
+            tot += w * parts[c]
+        if parts:
+            res[k] = {"flops_per_unit": tot / u, "unit": "cell-step",
+                      "wave_insts_per_dispatch": parts}
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
 
 
 def main():
@@ -98,81 +307,104 @@ def main():
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--days", type=int, default=90)
     ap.add_argument("--steps-per-launch", type=int, default=24)
+    ap.add_argument("--no-graph", action="store_true", help="headline loop without hipGraph")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=25)
+    ap.add_argument("--cpu-steps", type=int, default=241)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-qps", type=float, default=8.0)
+    ap.add_argument("--pmc-workload", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--summarize-flops", metavar="PMC_DIR")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (every basin, "
               "parameter set and request here fits one MI355X; see the sharding design)",
               file=sys.stderr)
         sys.exit(2)
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, os.path.join(ROOT, "profiles", "traffic.json"))
+        return
+    if args.summarize_flops:
+        summarize_flops(args.summarize_flops, os.path.join(ROOT, "gpurun_out", "pmc_meta.json"),
+                        os.path.join(ROOT, "profiles", "fp64_flops.json"))
+        return
+    if args.pmc_workload:
+        pmc_workload(args)
+        return
 
-    from mhm_amd import _native as N, workloads
+    from mhm_amd import _native as N, autotune, serve, workloads
 
+    traffic = load_json(os.path.join(ROOT, "profiles", "traffic.json"))
+    flops = load_json(os.path.join(ROOT, "profiles", "fp64_flops.json"))
     n_steps = args.days * 24
     t0 = time.perf_counter()
     model, steps, dom, gam = workloads.plain_domain(args.grid, args.grid, n_steps, seed=1)
     log(f"setup {time.perf_counter() - t0:.1f}s: cells={dom.n_cells} dmax={dom.dmax} "
         f"steps={n_steps} device_bytes={model.device_bytes / 2**30:.2f} GiB")
     S = args.steps_per_launch
+    kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=not args.no_graph)
+    # profiling on before the warm-up so a captured graph holds its event nodes
+    model.set_profiling(True, ROUTE_STRIDE)
     for _ in range(args.warmup):
-        model.run(steps, mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True)
-    model.set_profiling(True)
+        model.run(steps, **kw)
+    model.set_profiling(True, ROUTE_STRIDE)  # counters from zero for the timed region
     model.sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        model.run(steps, mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True)
+        model.run(steps, **kw)
         log(f"bench step {k + 1}/{args.steps} done")
     model.sync()
     elapsed = time.perf_counter() - t0
     prof = model.profile()
     model.set_profiling(False)
-    cell_steps = float(dom.n_cells) * n_steps * args.steps
-    value = cell_steps / elapsed
-    traffic = {}
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath))
-    roof, kernels = roofline(prof, S, 24, traffic)
+    value = float(dom.n_cells) * n_steps * args.steps / elapsed
+    kernels = kernel_table(prof, S, 24)
+    dominant = max(prof.items(), key=lambda kv: kv[1][0])[0]
+    hbm = hbm_roofline(dominant, prof[dominant], S, 24, traffic)
+    fpu = flops.get(dominant, {}).get("flops_per_unit")
+    fp64 = fp64_roofline(dominant, prof[dominant], fpu)
+    roof = fp64 if (fp64 and dominant == "cell_fused") else hbm
     log(f"fused: {value:.4g} cell-steps/s  kernels={json.dumps(kernels)}")
 
-    extra = {}
+    extra = {"roofline_hbm": hbm}
     if not args.no_plain:
-        model.set_profiling(True)
-        t0 = time.perf_counter()
-        model.run(steps, mode=N.RUN_PLAIN, want_gauges=True)
-        model.sync()
-        tp = time.perf_counter() - t0
+        pkw = dict(mode=N.RUN_PLAIN, want_gauges=True)
+        model.run(steps[:48], **pkw)
+        model.set_profiling(True, ROUTE_STRIDE)
+        tp = timed_runs(model, steps, 1, **pkw)
         pprof = model.profile()
         model.set_profiling(False)
-        proof, pk = roofline({"cell_step": pprof["cell_step"]}, 1, 24, traffic)
         extra["plain"] = {"value": dom.n_cells * n_steps / tp, "unit": "cell-steps/s",
-                          "cell_step_roofline": proof, "kernels": pk}
-        log(f"plain: {extra['plain']['value']:.4g} cell-steps/s  {json.dumps(pk)}")
+                          "workload": "same basin and steps, one launch per step, every flux "
+                                      "written every step",
+                          "cell_step_roofline": hbm_roofline("cell_step", pprof["cell_step"], 1,
+                                                             24, traffic),
+                          "kernels": kernel_table(pprof, 1, 24)}
+        log(f"plain: {extra['plain']['value']:.4g} cell-steps/s")
     model.close()
 
     if not args.no_ensemble:
-        em, esteps, edom, _ = workloads.plain_domain(64, 64, 3650, timestep=24, seed=3, n_sets=256)
-        em.run(esteps[:48], mode=N.RUN_FUSED, steps_per_launch=48)
-        em.set_profiling(True)
-        t0 = time.perf_counter()
-        em.run(esteps, mode=N.RUN_FUSED, steps_per_launch=365)
-        te = time.perf_counter() - t0
-        eprof = em.profile()
-        em.close()
-        _, ek = roofline(eprof, 365, 1, traffic)
-        extra["ensemble"] = {"value": 256 * edom.n_cells * 3650 / te, "unit": "cell-steps/s",
-                             "workload": "256 parameter sets x 4096 cells, daily x 3650",
-                             "kernels": ek}
-        log(f"ensemble: {extra['ensemble']['value']:.4g} (cell,set)-steps/s {json.dumps(ek)}")
+        extra["ensemble"] = bench_ensemble(N, workloads, autotune, traffic)
+        log(f"ensemble: {extra['ensemble']['value']:.4g} {extra['ensemble']['settings']}")
+    if not args.no_multi:
+        extra["multi_basin"] = bench_multi_basin(N, workloads, S)
+        log(f"multi_basin: {extra['multi_basin']['value']:.4g}")
+    if not args.no_serve:
+        extra["serving"] = bench_serving(serve, args.serve_requests, args.serve_qps)
+        log(f"serving: {json.dumps(extra['serving'])}")
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
         cpu = cpu_baseline(dom, gam, dom.n_days, args.cpu_steps, threads)
         log(f"cpu: {cpu['value']:.4g} cell-steps/s")
+        one = cpu_baseline(dom, gam, dom.n_days, 4, 1)
+        extra["cpu_baseline_1thread"] = one
+        log(f"cpu 1 thread: {one['value']:.4g} cell-steps/s")
 
     out = {
         "metric": "cell-steps/s FP64 plain vs batched; p50/p99 request latency at fixed QPS",
@@ -181,8 +413,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"single basin {args.grid}x{args.grid} L1 cells, 3 horizons, "
-                               f"hourly x {n_steps} steps, D8 routing (dmax {dom.dmax}), "
-                               f"fused S={S}",
+                               f"hourly x {n_steps} steps, D8 routing (max depth {dom.dmax}), "
+                               f"fused S={S}, {'eager' if args.no_graph else 'hipGraph'} time loop",
                    "cells": dom.n_cells, "time_steps": n_steps, "routing_levels": dom.dmax + 1,
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -673,9 +673,12 @@ hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
   } else if ((e = ev_get(x, &p.a)) != hipSuccess || (e = ev_get(x, &p.b)) != hipSuccess) {
     return e;
   }
-  if ((e = hipEventRecord(p.a, x->stream)) != hipSuccess) return e;
+  // under capture a plain record only marks a dependency; the external flag
+  // makes it an event-record node that timestamps every replay
+  const unsigned flags = x->capturing ? hipEventRecordExternal : 0u;
+  if ((e = hipEventRecordWithFlags(p.a, x->stream, flags)) != hipSuccess) return e;
   if ((e = f()) != hipSuccess) return e;
-  if ((e = hipEventRecord(p.b, x->stream)) != hipSuccess) return e;
+  if ((e = hipEventRecordWithFlags(p.b, x->stream, flags)) != hipSuccess) return e;
   if (!x->capturing) x->pending.push_back(p);
   return hipSuccess;
 }
