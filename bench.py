@@ -49,9 +49,10 @@ def bytes_per_unit(kernel, steps_per_launch, steps_per_day):
         # forcing 32 + PET params 24 + mHM params 232 + states 64 r/64 w + 23 fluxes 184 w
         return 600.0
     if kernel == "cell_fused":
-        # forcing once per forcing day, total runoff out every step, params + states
-        # in/out once per launch: 32/F + 8 + (128 + 256)/S
-        return 32.0 / steps_per_day + 8.0 + 384.0 / steps_per_launch
+        # forcing once per forcing day (32/F), qOUT to the ring every step (8);
+        # once per launch: params 256 + states 64 read, states 64 + 23 fluxes 184
+        # written (568/S)
+        return 32.0 / steps_per_day + 8.0 + 568.0 / steps_per_launch
     # routing node-step: qOUT from the ring 8, C1 + C2 16, own qTIN read + write 16,
     # own qTR(IT1) read 8 + qTR(IT2) write 8, upstream qTR read 8 (one per edge),
     # node metadata (child index + packed depth/flags) 8

@@ -166,7 +166,8 @@ typedef struct {
   int32_t steps_per_launch; /* fused path: S (0 = autotuned)                   */
   int32_t block;         /* cell-kernel block size (0 = autotuned)               */
   int32_t use_graph;     /* capture the tile sequence in a hipGraph              */
-  int32_t route_mode;    /* 0 auto, 1 grid diagonals, 2 workgroup-resident       */
+  int32_t route_mode;    /* 0 auto (= 3), 1 grid diagonals, 2 workgroup-resident
+                            trees, 3 tiles (windows of up to 24 steps per launch) */
   int32_t reserved[3];
 } mhmhip_run_opts;
 
@@ -217,6 +218,27 @@ int mhmhip_mhm_step(mhmhip_ctx *ctx, const mhmhip_step *st);
 int mhmhip_routing_order(int64_t n_nodes, int64_t n_links, const int32_t *fromN,
                          const int32_t *toN, int32_t *rOrder, int32_t *netPerm);
 
+/* Device layout of one domain's network as mhmhip_bind_network builds it
+ * (no counterpart in the reference; for footprint estimates before a
+ * context exists and for inspection).  Host-only.  netPerm/fromN/toN as in
+ * mhmhip_network for one domain (n_nodes - n_outlets links).  Optional
+ * outputs: pos[n_nodes] = device index of each node; slots[4 * n_slots] =
+ * per bin slot: device index (-1: proxy or padding), boundary row (-1: none),
+ * lag | nchild << 12 | flags (bit 16 node, 17 proxy, 18 tile root),
+ * first upstream slot within the bin | tile depth << 16.  MHMHIP_EINVAL when
+ * slots_cap < n_slots (plan still filled). */
+typedef struct {
+  int32_t dmax;     /* deepest node (links to the sink)                      */
+  int32_t tau_max;  /* deepest tile                                          */
+  int64_t n_tiles;
+  int64_t n_bins;   /* workgroups per parameter set (256 slots each)         */
+  int64_t n_bnd;    /* tile roots draining into another tile                 */
+  int64_t n_slots;  /* n_bins * 256                                          */
+} mhmhip_plan;
+int mhmhip_plan_network(int64_t n_nodes, int32_t n_outlets, const int32_t *netPerm,
+                        const int32_t *fromN, const int32_t *toN, mhmhip_plan *plan,
+                        int64_t *pos, int32_t *slots, int64_t slots_cap);
+
 /* Network of every packed domain (input of mRM_routing,
  * src/mRM/mo_mrm_routing.f90:104-110); resets routing states to 0
  * (src/mRM/mo_mrm_init.f90:453-494). */
@@ -256,7 +278,7 @@ int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
  * context's stream; the device time, launch count and work units (cell-steps
  * for the cell kernels, node-steps for the routing) accumulate per kernel. */
 enum mhmhip_kernel { MHMHIP_K_CELL_STEP = 0, MHMHIP_K_CELL_FUSED, MHMHIP_K_ROUTE_DIAG,
-                     MHMHIP_K_ROUTE_WG, MHMHIP_K_COUNT };
+                     MHMHIP_K_ROUTE_WG, MHMHIP_K_ROUTE_TILE, MHMHIP_K_COUNT };
 /* on > 0 enables timing and resets the totals; routing-diagonal launches are
  * then timed one in `on` (1: all), the totals scaled to the timed ones. */
 int mhmhip_set_profiling(mhmhip_ctx *ctx, int32_t on);

@@ -83,6 +83,8 @@ SIGNATURES = {
     "mhmhip_set_forcing": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int64]),
     "mhmhip_mhm_step": (C.c_int, [C.c_void_p, C.POINTER(Step)]),
     "mhmhip_routing_order": (C.c_int, [C.c_int64, C.c_int64, _ip, _ip, _ip, _ip]),
+    "mhmhip_plan_network": (C.c_int, [C.c_int64, C.c_int32, _ip, _ip, _ip, C.c_void_p, _lp, _ip,
+                                      C.c_int64]),
     "mhmhip_bind_network": (C.c_int, [C.c_void_p, C.POINTER(Network)]),
     "mhmhip_set_routing_params": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
     "mhmhip_set_inflow": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
@@ -93,7 +95,12 @@ SIGNATURES = {
     "mhmhip_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "mhmhip_get_profile": (C.c_int, [C.c_void_p, C.c_int32, _dp, _lp, _dp]),
 }
-K_NAMES = ["cell_step", "cell_fused", "route_diag", "route_wg"]
+K_NAMES = ["cell_step", "cell_fused", "route_diag", "route_wg", "route_tile"]
+
+
+class Plan(C.Structure):
+    _fields_ = [("dmax", C.c_int32), ("tau_max", C.c_int32), ("n_tiles", C.c_int64),
+                ("n_bins", C.c_int64), ("n_bnd", C.c_int64), ("n_slots", C.c_int64)]
 
 
 class MhmHipError(RuntimeError):
@@ -181,3 +188,28 @@ def routing_order(from_n, to_n, n_nodes):
     if rc != OK:
         raise MhmHipError(rc, "routing_order: invalid network")
     return r_order, net_perm
+
+
+def plan_network(domain_arrays, slots=False):
+    """Device layout of one domain (mhmhip_plan_network): (plan, pos, slots)
+    with slots an (n_slots, 4) int32 array when asked for.  domain_arrays:
+    object with n_cells, n_outlets, netPerm, fromN, toN (1-based)."""
+    d = domain_arrays
+    n = int(d.n_cells)
+    perm = np.ascontiguousarray(d.netPerm, np.int32)
+    fr = np.ascontiguousarray(d.fromN, np.int32)
+    to = np.ascontiguousarray(d.toN, np.int32)
+    plan = Plan()
+    pos = np.zeros(n, np.int64)
+    rc = lib().mhmhip_plan_network(n, int(d.n_outlets), iptr(perm), iptr(fr), iptr(to),
+                                   C.byref(plan), lptr(pos), None, 0)
+    if rc != OK:
+        raise MhmHipError(rc, "plan_network: invalid network")
+    table = None
+    if slots:
+        table = np.zeros((plan.n_slots, 4), np.int32)
+        rc = lib().mhmhip_plan_network(n, int(d.n_outlets), iptr(perm), iptr(fr), iptr(to),
+                                       C.byref(plan), None, iptr(table), plan.n_slots)
+        if rc != OK:
+            raise MhmHipError(rc, "plan_network: slot table")
+    return plan, pos, table

@@ -1,5 +1,6 @@
 """Autotuner for the time loop: picks steps per launch (S), the cell-kernel
-block size, the routing schedule (grid diagonals or workgroup-resident) and
+block size, the routing schedule (tiles, grid diagonals or workgroup-resident
+trees) and
 hipGraph capture by timing short trials on the model itself.  The device
 states are snapshotted before and restored after every trial, so tuning
 leaves the simulation where it was.
@@ -12,7 +13,7 @@ from . import _native as N
 
 def candidates(model, S_values=(8, 24, 72, 168), blocks=(64, 128, 256), graphs=(False, True)):
     """Fused-path settings worth trying for this model."""
-    route_modes = [1, 2] if _small_trees(model) else [1]
+    route_modes = [3, 1, 2] if _small_trees(model) else [3, 1]
     for S, b, rm, g in itertools.product(S_values, blocks, route_modes, graphs):
         yield {"mode": N.RUN_FUSED, "steps_per_launch": S, "block": b, "route_mode": rm,
                "use_graph": g}

@@ -2,6 +2,7 @@
 
   mhm_amd/libmhmhip.so  HIP kernels + C-ABI (include/mhmhip.h), gfx950
   mhm_amd/libmhmsyn.so  host-only synthetic-domain tools (priority-flood D8)
+  mhm_amd/libmhmcheck.so  host + device evaluation of the chain's log/exp/pow (tests)
   mhm_amd/fortran/mhmhip_driver  Fortran host on the iso_c_binding module
 
 Run ``python -m mhm_amd.build`` (or ``__graft_entry__.build()``).
@@ -73,6 +74,21 @@ def build_hip(verbose=False, out=None, defines=(), obj_dir=None):
     return target
 
 
+CHECK_LIB = os.path.join(PKG, "libmhmcheck.so")
+
+
+def build_check():
+    """Host + device evaluation of the chain's log/exp/pow (accuracy tests)."""
+    obj = os.path.join(OBJ, "mathcheck.hip.o")
+    os.makedirs(OBJ, exist_ok=True)
+    hipcc = _hipcc()
+    _run([hipcc] + hip_flags() + ["-c", "-x", "hip", os.path.join(CSRC, "mathcheck.hip"), "-o", obj])
+    tmp = CHECK_LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj])
+    os.replace(tmp, CHECK_LIB)
+    return CHECK_LIB
+
+
 def build_syn():
     tmp = SYN_LIB + ".tmp"
     _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-o", tmp,
@@ -107,6 +123,7 @@ def build_fortran():
 def build_all(verbose=False):
     build_syn()
     build_hip(verbose)
+    build_check()
     build_fortran()
 
 

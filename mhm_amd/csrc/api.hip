@@ -55,6 +55,13 @@ struct mhmhip_ctx {
          *d_pub = nullptr, *d_qOUT = nullptr;
   int64_t *d_dom_start = nullptr;
   int32_t *d_dom_dmax = nullptr;
+  int32_t *d_child_list = nullptr;
+  int64_t n_child_dev = 0;
+  // tiled routing
+  SlotMeta *d_slots = nullptr;
+  BinInfo *d_bins = nullptr;
+  double *d_bnd = nullptr;
+  int64_t n_slots_dev = 0, n_bins_dev = 0, n_bnd_dev = 0;
   std::vector<double> h_length, h_slope, h_fFPimp, h_area;
   std::vector<int64_t> dom_link_start;
   std::vector<int32_t> h_nout;
@@ -75,8 +82,9 @@ struct mhmhip_ctx {
   // Kernels index steps relative to the start of each call; the parity slot
   // of pub holding qTR(:,IT1) is normalised to 1 before a call (step -1).
   int qtr_slot = 1;
-  // positions [lev_start[k], lev_start[k+1]) hold the nodes at depth k
-  // (single-domain networks; lets a diagonal launch only its active band)
+  // positions [lev_start[k], lev_start[k+1]) hold the nodes at depth k when
+  // the layout keeps depths contiguous (lets a diagonal launch only its
+  // active band); empty for the tile-major layout
   std::vector<int64_t> lev_start;
 
   // time loop captured as a hipGraph, replayed while its key matches
@@ -105,16 +113,16 @@ struct mhmhip_ctx {
   // every replay.
   bool capturing = false;
   std::vector<Pending> graph_events;
-  int64_t graph_all[MHMHIP_K_COUNT] = {0, 0, 0, 0};
+  int64_t graph_all[MHMHIP_K_COUNT] = {};
   bool prof = false;
   int prof_stride = 1;         // time every prof_stride-th routing launch
   int64_t route_launches = 0;  // routing launches seen while profiling
   std::vector<Pending> pending;
   std::vector<hipEvent_t> ev_pool;
-  double prof_ms[MHMHIP_K_COUNT] = {0, 0, 0, 0};
-  int64_t prof_n[MHMHIP_K_COUNT] = {0, 0, 0, 0};      // timed launches
-  int64_t prof_all[MHMHIP_K_COUNT] = {0, 0, 0, 0};    // all launches
-  double prof_units[MHMHIP_K_COUNT] = {0, 0, 0, 0};
+  double prof_ms[MHMHIP_K_COUNT] = {};
+  int64_t prof_n[MHMHIP_K_COUNT] = {};      // timed launches
+  int64_t prof_all[MHMHIP_K_COUNT] = {};    // all launches
+  double prof_units[MHMHIP_K_COUNT] = {};
 
   int fail(int code, const std::string &m) {
     err = m;
@@ -292,6 +300,10 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_qOUT);
   (void)hipFree(x->d_dom_start);
   (void)hipFree(x->d_dom_dmax);
+  (void)hipFree(x->d_child_list);
+  (void)hipFree(x->d_slots);
+  (void)hipFree(x->d_bins);
+  (void)hipFree(x->d_bnd);
   (void)hipFree(x->d_gauge_node);
   (void)hipFree(x->d_inflow_node);
   (void)hipFree(x->d_inflow_hw);
@@ -606,6 +618,7 @@ int ensure_gauge_buf(mhmhip_ctx *x, int64_t rows) {
 RouteArgs route_args(mhmhip_ctx *x, int64_t t_lo, int64_t t_hi, int64_t t_base) {
   RouteArgs r{};
   r.meta = x->d_meta;
+  r.child_list = x->d_child_list;
   r.C1 = x->d_C1;
   r.C2 = x->d_C2;
   r.qTIN = x->d_qTIN;
@@ -639,10 +652,35 @@ int64_t max_domain_nodes(const mhmhip_ctx *x) {
   return m;
 }
 
-bool use_wg_routing(const mhmhip_ctx *x, int route_mode) {
-  if (route_mode == 1) return false;
-  if (route_mode == 2) return true;
-  return max_domain_nodes(x) <= 16384;
+// Routing kernel for a route_mode: 1 grid diagonals, 2 workgroup-resident
+// trees, 3 and 0 (auto) tiles.
+int route_kind(int route_mode) {
+  if (route_mode == 1) return MHMHIP_K_ROUTE_DIAG;
+  if (route_mode == 2) return MHMHIP_K_ROUTE_WG;
+  return MHMHIP_K_ROUTE_TILE;
+}
+
+// Steps per routing window of the tiled kernel for S steps per cell launch.
+int32_t tile_window(int S) { return std::max(1, std::min(S, kTileWindowMax)); }
+
+// Ring rows the routing of a call needs: the pipelined diagonals read a
+// step up to dmax diagonals after its cell launch, the tiles up to tau_max
+// windows after.
+int64_t ring_need(const mhmhip_ctx *x, int kind, int S, int32_t B) {
+  if (kind == MHMHIP_K_ROUTE_TILE) return ((int64_t)x->lay.tau_max + 1) * B + S + 1;
+  return (int64_t)S + x->lay.dmax + 2;
+}
+
+TileArgs tile_args(const mhmhip_ctx *x, int32_t B, int64_t nsteps) {
+  TileArgs g{};
+  g.slot = x->d_slots;
+  g.bin = x->d_bins;
+  g.bnd = x->d_bnd;
+  g.n_bnd = std::max<int64_t>(1, x->lay.n_bnd);
+  g.tau_max = x->lay.tau_max;
+  g.B = B;
+  g.nwin = (int32_t)((nsteps + B - 1) / B);
+  return g;
 }
 
 hipError_t ev_get(mhmhip_ctx *x, hipEvent_t *e) {
@@ -660,6 +698,7 @@ template <typename Fn>
 hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
   if (!x->prof) return f();
   (x->capturing ? x->graph_all : x->prof_all)[kind] += 1;
+  // diagonal launches come by the thousand: time one in prof_stride
   if (kind == MHMHIP_K_ROUTE_DIAG && (x->route_launches++ % x->prof_stride) != 0) return f();
   mhmhip_ctx::Pending p{kind, nullptr, nullptr, units};
   hipError_t e;
@@ -752,11 +791,19 @@ int route_diagonals(mhmhip_ctx *x, const RouteArgs &r, int64_t d_lo, int64_t d_h
   return MHMHIP_OK;
 }
 
+// Tiled routing launches [L_lo, L_hi) of a call.
+int route_tiles(mhmhip_ctx *x, const RouteArgs &r, const TileArgs &g, int32_t L_lo, int32_t L_hi) {
+  for (int32_t L = L_lo; L < L_hi; ++L)
+    CHECK_HIP(x, timed(x, MHMHIP_K_ROUTE_TILE, 0.0, [&] {
+                return launch_route_tile(r, g, (int64_t)x->lay.bins.size(), L, x->stream);
+              }),
+              "route_tile");
+  return MHMHIP_OK;
+}
+
 // node-steps routed, credited to the routing kernel that did them
-void credit_route(mhmhip_ctx *x, bool wg, int64_t nsteps) {
-  if (x->prof)
-    x->prof_units[wg ? MHMHIP_K_ROUTE_WG : MHMHIP_K_ROUTE_DIAG] +=
-        (double)x->n_cells * x->n_sets * (double)nsteps;
+void credit_route(mhmhip_ctx *x, int kind, int64_t nsteps) {
+  if (x->prof) x->prof_units[kind] += (double)x->n_cells * x->n_sets * (double)nsteps;
 }
 
 // Moves qTR(:,IT1) into parity slot 1, where a call's step -1 reads it.
@@ -865,7 +912,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
       // links draining into this node have their routed outflow cut (mo_mrm_routing.f90:452-457)
       const int64_t c0 = lay.child[p];
       const int nc = nm_nchild(lay.packed[p]);
-      for (int c = 0; c < nc; ++c) lay.packed[c0 + c] |= NF_CUT;
+      for (int c = 0; c < nc; ++c) lay.packed[lay.child_list[c0 + c]] |= NF_CUT;
     }
   }
 
@@ -915,6 +962,11 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->dfree(x->d_qOUT, x->ld);
   x->dfree(x->d_dom_start, x->n_domains + 1);
   x->dfree(x->d_dom_dmax, x->n_domains);
+  x->dfree(x->d_child_list, x->n_child_dev);
+  x->dfree(x->d_slots, x->n_slots_dev);
+  x->dfree(x->d_bins, x->n_bins_dev);
+  x->dfree(x->d_bnd, x->n_bnd_dev);
+  x->n_child_dev = x->n_slots_dev = x->n_bins_dev = x->n_bnd_dev = 0;
   x->dfree(x->d_gauge_node, std::max(1, x->n_gauges));
   x->dfree(x->d_inflow_node, std::max(1, x->n_inflow));
   x->dfree(x->d_inflow_hw, std::max(1, x->n_inflow));
@@ -927,6 +979,29 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   CHECK_RC(x->dalloc(&x->d_qOUT, x->ld, "alloc qOUT"));
   CHECK_RC(x->dalloc(&x->d_dom_start, x->n_domains + 1, "alloc domains"));
   CHECK_RC(x->dalloc(&x->d_dom_dmax, x->n_domains, "alloc domains"));
+  // the slot flags carry the node flags (gauges, inflow, cut links set above)
+  for (SlotMeta &m : lay.slots)
+    if (m.a & SF_REAL) m.a |= lay.packed[m.pos] & NF_MASK;
+  const int64_t n_child = std::max<int64_t>(1, (int64_t)lay.child_list.size());
+  const int64_t n_bnd = 2 * (int64_t)x->n_sets * std::max<int64_t>(1, lay.n_bnd) * kTileWindowMax;
+  CHECK_RC(x->dalloc(&x->d_child_list, n_child, "alloc child list"));
+  x->n_child_dev = n_child;
+  CHECK_RC(x->dalloc(&x->d_slots, (int64_t)lay.slots.size(), "alloc tiles"));
+  x->n_slots_dev = (int64_t)lay.slots.size();
+  CHECK_RC(x->dalloc(&x->d_bins, (int64_t)lay.bins.size(), "alloc tiles"));
+  x->n_bins_dev = (int64_t)lay.bins.size();
+  CHECK_RC(x->dalloc(&x->d_bnd, n_bnd, "alloc tile boundaries"));
+  x->n_bnd_dev = n_bnd;
+  if (!lay.child_list.empty())
+    CHECK_HIP(x, sync_copy(x, x->d_child_list, lay.child_list.data(),
+                           sizeof(int32_t) * lay.child_list.size(), hipMemcpyHostToDevice),
+              "upload child list");
+  CHECK_HIP(x, sync_copy(x, x->d_slots, lay.slots.data(), sizeof(SlotMeta) * lay.slots.size(),
+                         hipMemcpyHostToDevice),
+            "upload tiles");
+  CHECK_HIP(x, sync_copy(x, x->d_bins, lay.bins.data(), sizeof(BinInfo) * lay.bins.size(),
+                         hipMemcpyHostToDevice),
+            "upload tiles");
   x->n_gauges = net->n_gauges;
   x->n_inflow = net->n_inflow;
   CHECK_RC(x->dalloc(&x->d_gauge_node, std::max(1, x->n_gauges), "alloc gauges"));
@@ -951,12 +1026,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   CHECK_HIP(x, sync_copy(x, x->d_inflow_hw, ihw.data(), sizeof(int32_t) * ihw.size(), hipMemcpyHostToDevice), "upload inflow");
   x->h_inflow_node_dev = inode;
   x->h_inflow_hw = ihw;
-  x->lev_start.clear();
-  if (x->n_domains == 1) {
-    x->lev_start.assign(lay.dmax + 2, 0);
-    for (int64_t q = 0; q < x->n_cells; ++q) x->lev_start[nm_depth(lay.packed[q]) + 1]++;
-    for (int k = 0; k <= lay.dmax; ++k) x->lev_start[k + 1] += x->lev_start[k];
-  }
+  x->lev_start.clear();  // the tile-major layout does not keep depths contiguous
   x->lay = std::move(lay);
   x->net_bound = true;
   x->qtr_slot = 1;
@@ -1093,10 +1163,16 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   CHECK_RC(normalize_qtr(x));
   // this step is step 0 of the call: mhm_step wrote its qOUT to ring slot 0
   RouteArgs r = route_args(x, 0, 1, 0);
-  const bool wg = use_wg_routing(x, 0);
-  CHECK_RC(route_diagonals(x, r, 0, 1, false, wg));
-  CHECK_RC(route_diagonals(x, r, 1, 0, true, wg));
-  credit_route(x, wg, 1);
+  const int kind = route_kind(0);
+  if (kind == MHMHIP_K_ROUTE_TILE) {
+    const TileArgs g = tile_args(x, 1, 1);
+    CHECK_RC(route_tiles(x, r, g, 0, g.nwin + g.tau_max));
+  } else {
+    const bool wg = kind == MHMHIP_K_ROUTE_WG;
+    CHECK_RC(route_diagonals(x, r, 0, 1, false, wg));
+    CHECK_RC(route_diagonals(x, r, 1, 0, true, wg));
+  }
+  credit_route(x, kind, 1);
   x->qtr_slot = 0;
   x->t_route++;
   if (gauge_out && x->n_gauges > 0) {
@@ -1119,7 +1195,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     return x->fail(MHMHIP_EINVAL, "run: unknown mode");
   if (o.block != 0 && o.block != 64 && o.block != 128 && o.block != 256)
     return x->fail(MHMHIP_EINVAL, "run: block must be 64, 128 or 256");
-  if (o.route_mode < 0 || o.route_mode > 2) return x->fail(MHMHIP_EINVAL, "run: bad route_mode");
+  if (o.route_mode < 0 || o.route_mode > 3) return x->fail(MHMHIP_EINVAL, "run: bad route_mode");
   if (o.steps_per_launch < 0) return x->fail(MHMHIP_EINVAL, "run: bad steps_per_launch");
   const bool route = routing_on(x);
   if (route && x->t_route != x->t_cell)
@@ -1134,20 +1210,26 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   for (int32_t k = 0; k < nsteps; ++k) CHECK_RC(make_step(x, steps[k], hs[k]));
   const int S = o.mode == MHMHIP_RUN_PLAIN ? 1 : (o.steps_per_launch > 0 ? o.steps_per_launch : 24);
   const int block = o.block > 0 ? o.block : 256;
-  const bool wg = route && use_wg_routing(x, o.route_mode);
+  const int kind = route_kind(o.route_mode);
+  const bool wg = kind == MHMHIP_K_ROUTE_WG;
+  // tiled routing: windows of B steps (the plain path routes in windows too)
+  const int32_t B = tile_window(o.mode == MHMHIP_RUN_PLAIN ? kTileWindowMax : S);
   CHECK_RC(ensure_steps(x, nsteps));
   CHECK_HIP(x, hipMemcpyAsync(x->d_steps, hs.data(), sizeof(StepDev) * nsteps,
                               hipMemcpyHostToDevice, x->stream),
             "upload steps");
   if (route) {
-    CHECK_RC(ensure_ring(x, (int64_t)S + x->lay.dmax + 2));
+    CHECK_RC(ensure_ring(x, ring_need(x, kind, S, B)));
     CHECK_RC(ensure_gauge_buf(x, nsteps));
   }
   if (route) CHECK_RC(normalize_qtr(x));
   CellArgs a = cell_args(x, route);
   RouteArgs r = route_args(x, 0, nsteps, 0);
+  const TileArgs g = tile_args(x, B, nsteps);
+  const int32_t L_end = g.nwin + g.tau_max;
   // the whole loop, in steps relative to this call (replayable as a graph)
   auto enqueue = [&]() -> int {
+    int32_t L = 0;  // next tiled routing launch
     for (int32_t k = 0; k < nsteps; k += S) {
       const int n = std::min<int32_t>(S, nsteps - k);
       if (o.mode == MHMHIP_RUN_PLAIN) {
@@ -1161,9 +1243,19 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
                   }),
                   "cell_fused");
       }
-      if (route) CHECK_RC(route_diagonals(x, r, k, k + n, false, wg));
+      if (!route) continue;
+      if (kind == MHMHIP_K_ROUTE_TILE) {
+        // launch L needs the steps of window L (its deepest tiles' window)
+        int32_t L1 = L;
+        while (L1 < L_end && std::min<int64_t>((int64_t)(L1 + 1) * B, nsteps) <= k + n) ++L1;
+        CHECK_RC(route_tiles(x, r, g, L, L1));
+        L = L1;
+      } else {
+        CHECK_RC(route_diagonals(x, r, k, k + n, false, wg));
+      }
     }
-    if (route) CHECK_RC(route_diagonals(x, r, nsteps, 0, true, wg));
+    if (route && kind == MHMHIP_K_ROUTE_TILE) CHECK_RC(route_tiles(x, r, g, L, L_end));
+    if (route && kind != MHMHIP_K_ROUTE_TILE) CHECK_RC(route_diagonals(x, r, nsteps, 0, true, wg));
     return MHMHIP_OK;
   };
   const bool graph = o.use_graph && o.mode == MHMHIP_RUN_FUSED;
@@ -1173,7 +1265,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     key.mode = o.mode;
     key.S = S;
     key.block = block;
-    key.wg = wg ? 1 : 0;
+    key.wg = kind;
     key.prof = x->prof ? x->prof_stride : 0;
     key.ring = route ? x->d_ring : nullptr;
     key.gauge = route ? x->d_gauge_buf : nullptr;
@@ -1202,7 +1294,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     CHECK_RC(enqueue());
   }
   if (route) {
-    credit_route(x, wg, nsteps);
+    credit_route(x, kind, nsteps);
     x->qtr_slot = (int)((nsteps - 1) & 1);
   }
   x->t_cell += nsteps;

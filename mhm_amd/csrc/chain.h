@@ -3,7 +3,8 @@
 // Each function restates one reference routine with its operation order,
 // branch structure and clamps unchanged, so the result of every +, -, *, /
 // and sqrt is the reference's (the library is built with -ffp-contract=off).
-// Only exp/log/pow/acos/sin come from the device math library.
+// pow is evaluated by exp_ylogx below (own log and exp); acos and sin of
+// the daily PET come from the device math library.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,13 +27,9 @@ constexpr double kDuffieDelta2 = 1.3900;
 constexpr double kPi = 3.141592653589793238462643383279502884197;
 constexpr double kRadConst = 86400.0 / kPi / 2.45e06 * 1367.0;
 
-// x**y for y > 0 as exp(y*log(x)).  For the chain's arguments (|y*log(x)| of
-// order 10 at most) this is within a few 1e-16 relative of the correctly
-// rounded power, at a fraction of pow()'s cost; x <= 0 gives pow()'s result
-// for non-integer y (0 for x == 0, NaN below).
-// exp(y * log(x)), kept out of line: inlined at every call site inside the
-// time loop, the polynomial constants of exp and log are hoisted into
-// registers for the whole loop and cut the kernel's occupancy.
+// Out-of-line device functions: inlined at every call site inside the time
+// loop, their polynomial constants are hoisted into registers for the whole
+// loop and cut the kernel's occupancy.
 #ifndef MHMHIP_OUTLINE_TRANSC
 #define MHMHIP_OUTLINE_TRANSC 1
 #endif
@@ -41,13 +38,143 @@ constexpr double kRadConst = 86400.0 / kPi / 2.45e06 * 1367.0;
 #else
 #define MHMHIP_OUTLINE __forceinline__
 #endif
-__device__ MHMHIP_OUTLINE double exp_ylogx(double y, double x) {
-  return exp(y * log(x));
+
+// fma(a, b, c) for a constant addend c.  On the device c is passed in a
+// scalar register pair, so each Horner step is one v_fma_f64 (the compiler
+// otherwise copies every 64-bit constant into two VGPRs with v_mov_b32 and
+// accumulates with v_fmac, three vector instructions per step).  The operands
+// a and b never come straight from a transcendental instruction.  Same IEEE
+// fused multiply-add on host and device.
+__host__ __device__ inline double fma_k(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+#else
+  return fma(a, b, c);
+#endif
+}
+
+// ln 2 split so that k * kLn2Hi is exact for |k| < 2^21 (32 significant bits)
+constexpr double kLn2Hi = 0x1.62e42fee00000p-1;
+constexpr double kLn2Lo = 0x1.a39ef35793c76p-33;
+constexpr double kInvLn2 = 1.4426950408889634;    // RN(1/ln 2)
+constexpr double kSqrtHalf = 0x1.6a09e667f3bcdp-1;  // RN(sqrt(1/2))
+
+// N doubles passed and returned by value (registers across an out-of-line call)
+template <int N>
+struct DV {
+  double v[N];
+};
+
+// Natural log of finite x > 0 (normal or subnormal), N values in lockstep.
+// Reduction x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1,
+// s = f / (2 + f), and
+//   log(1 + f) = 2 atanh(s) = 2s + s R(s^2),  R(z) = sum_k 2 z^k / (2k + 1),
+// arranged as f - (f^2/2 - s (f^2/2 + R)) so that the rounding of s only
+// enters the small correction term (the scheme of fdlibm's e_log.c).
+// |s| <= 0.1716 -> z <= 0.0295; the Taylor terms through z^9 leave a
+// truncation error below 3e-17 relative.  Every operation is an IEEE
+// +, -, *, /, fma or an exact frexp, so host and device builds agree bit for
+// bit.  Each step runs over all N values before the next one, which gives
+// the scheduler N independent dependency chains to interleave.
+template <int N>
+__host__ __device__ inline DV<N> log_pos_v(const DV<N> &x) {
+  DV<N> f, s, z, r, out;
+  int e[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double m = frexp(x.v[i], &e[i]);  // [0.5, 1)
+    if (m < kSqrtHalf) {
+      m = m + m;
+      e[i] = e[i] - 1;
+    }
+    f.v[i] = m - 1.0;  // exact
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) s.v[i] = f.v[i] / (2.0 + f.v[i]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    z.v[i] = s.v[i] * s.v[i];
+    r.v[i] = 2.0 / 19.0;
+  }
+  constexpr double kR[8] = {2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0, 2.0 / 11.0,
+                            2.0 / 9.0,  2.0 / 7.0,  2.0 / 5.0,  2.0 / 3.0};
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.v[i] = fma_k(r.v[i], z.v[i], kR[k]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double rz = r.v[i] * z.v[i];
+    const double hfsq = 0.5 * (f.v[i] * f.v[i]);
+    const double de = (double)e[i];
+    out.v[i] = fma(de, kLn2Hi, f.v[i] - (hfsq - fma(s.v[i], hfsq + rz, de * kLn2Lo)));
+  }
+  return out;
+}
+
+// exp(z) for finite z, N values in lockstep: z = k ln2 + r with
+// |r| <= ln2/2, exp(r) by its Taylor series through r^13 (truncation below
+// 5e-18 relative), scaled by 2^k.  z is clamped to [-1100, 710] first, which
+// gives 0 and +inf beyond the range.
+template <int N>
+__host__ __device__ inline DV<N> exp_fin_v(const DV<N> &zin) {
+  DV<N> k, r, p, out;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double z = zin.v[i];
+    if (z > 710.0) z = 710.0;
+    if (z < -1100.0) z = -1100.0;
+    k.v[i] = rint(z * kInvLn2);
+    r.v[i] = fma(-k.v[i], kLn2Lo, fma(-k.v[i], kLn2Hi, z));
+    p.v[i] = 1.0 / 6227020800.0;  // 1/13!
+  }
+  constexpr double kE[10] = {1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0,
+                             1.0 / 362880.0,    1.0 / 40320.0,    1.0 / 5040.0,
+                             1.0 / 720.0,       1.0 / 120.0,      1.0 / 24.0,
+                             1.0 / 6.0};
+#pragma unroll
+  for (int j = 0; j < 10; ++j)
+#pragma unroll
+    for (int i = 0; i < N; ++i) p.v[i] = fma_k(p.v[i], r.v[i], kE[j]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double q = fma(p.v[i], r.v[i], 0.5);
+    q = fma(q, r.v[i], 1.0);
+    q = fma(q, r.v[i], 1.0);
+    out.v[i] = ldexp(q, (int)k.v[i]);
+  }
+  return out;
+}
+
+// x**y for x > 0 as exp(y log x), N pairs at once: within (2 + 3 |y log x|)
+// ulp of the exact power (log's error and the rounding of y log x, amplified
+// by exp); for the chain's arguments |y log x| stays of order 10, a few
+// 1e-15 relative at most.
+template <int N>
+__host__ __device__ MHMHIP_OUTLINE DV<N> exp_ylogx_v(DV<N> y, DV<N> x) {
+  DV<N> l = log_pos_v<N>(x);
+#pragma unroll
+  for (int i = 0; i < N; ++i) l.v[i] = y.v[i] * l.v[i];
+  return exp_fin_v<N>(l);
+}
+
+__host__ __device__ inline double log_pos(double x) { return log_pos_v<1>(DV<1>{{x}}).v[0]; }
+__host__ __device__ inline double exp_fin(double z) { return exp_fin_v<1>(DV<1>{{z}}).v[0]; }
+__host__ __device__ inline double exp_ylogx(double y, double x) {
+  return exp_ylogx_v<1>(DV<1>{{y}}, DV<1>{{x}}).v[0];
+}
+
+// pow(x, y) of the reference for y > 0 and the x > 0 power already in hand:
+// pow's value for non-integer y when x <= 0 (0 for x == 0, NaN below)
+__device__ __forceinline__ double pow_pos_from(double x, double px) {
+  if (x > 0.0) return px;
+  return x == 0.0 ? 0.0 : __builtin_nan("");
 }
 
 __device__ __forceinline__ double pow_pos(double x, double y) {
-  if (x > 0.0) return exp_ylogx(y, x);
-  return x == 0.0 ? 0.0 : __builtin_nan("");
+  return pow_pos_from(x, x > 0.0 ? exp_ylogx(y, x) : 0.0);
 }
 
 // FORCES mo_utils LE: a <= b within eps*|b| (pre-proc/mtCLIM/mo_utils.f90:922-934)
@@ -55,10 +182,12 @@ __device__ __forceinline__ bool le_eps(double a, double b) {
   return !(((DBL_EPSILON * fabs(b) - fabs(a - b)) < 0.0) && (a > b));
 }
 
-// src/mHM/mo_canopy_interc.f90:75-133
-__device__ __forceinline__ void canopy_interc(double pet, double interc_max, double precip,
-                                              double &interc, double &throughfall,
-                                              double &evap_canopy) {
+// src/mHM/mo_canopy_interc.f90:75-133, in two parts around the power
+// (interc / interc_max)**(2/3): canopy_fill updates the storage and the
+// throughfall and returns the ratio (1.0 when interc_max <= eps, where the
+// reference takes no power); canopy_evap finishes with the ratio's power.
+__device__ __forceinline__ double canopy_fill(double interc_max, double precip, double &interc,
+                                              double &throughfall) {
   const double aux = interc + precip;
   if (aux >= interc_max) {
     throughfall = aux - interc_max;
@@ -67,8 +196,14 @@ __device__ __forceinline__ void canopy_interc(double pet, double interc_max, dou
     throughfall = 0.0;
     interc = aux;
   }
+  return interc_max > kEps ? interc / interc_max : 1.0;
+}
+
+// ratio: canopy_fill's result; pw: ratio**(2/3) when ratio > 0
+__device__ __forceinline__ void canopy_evap(double pet, double interc_max, double ratio, double pw,
+                                            double &interc, double &evap_canopy) {
   if (interc_max > kEps)
-    evap_canopy = pet * pow_pos(interc / interc_max, kTwoThird);
+    evap_canopy = pet * pow_pos_from(ratio, pw);
   else
     evap_canopy = 0.0;
   if (evap_canopy < 0.0) evap_canopy = 0.0;
@@ -141,15 +276,17 @@ __device__ __forceinline__ double jarvis(double sm, double sat, double wp, doubl
   return r;
 }
 
-// src/mHM/mo_soil_moisture.f90:94-290
+// src/mHM/mo_soil_moisture.f90:94-290.  frac_pw[h] = (sm[h] / sat[h])**beta[h]
+// of the start-of-step moisture (the reference's infiltration fraction,
+// line 219), evaluated for all horizons before the horizon loop.
 template <int NH>
 __device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double thrS, double pet,
                                               double evap, const double *sat, const double *fr,
-                                              const double *fc, const double *wp,
-                                              const double *bexp, double c1, double ecan,
-                                              double peff, double &runoff_sealed,
-                                              double &storage_sealed, double *infil, double *sm,
-                                              double *aet, double &aet_sealed) {
+                                              const double *fc, const double *wp, double c1,
+                                              double ecan, double peff, const double *frac_pw,
+                                              double &runoff_sealed, double &storage_sealed,
+                                              double *infil, double *sm, double *aet,
+                                              double &aet_sealed) {
   runoff_sealed = 0.0;
   aet_sealed = 0.0;
   if (fS > 0.0) {
@@ -183,7 +320,7 @@ __device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double
     } else {
       double frac;
       if (sm[h] > kEps)
-        frac = exp_ylogx(bexp[h], sm[h] / sat[h]);
+        frac = frac_pw[h];  // (sm[h] / sat[h])**bexp[h], start-of-step sm[h]
       else
         frac = 0.0;
       const double tmp = pin * (1.0 - frac);

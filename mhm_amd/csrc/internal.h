@@ -92,9 +92,9 @@ struct CellArgs {
 };
 
 // Routing node metadata, packed in 8 bytes per node.
-//   x: index of the first upstream node (children are contiguous, in the
-//      reference's routing rank)
-//   y: depth (bits 0..19) | nchild (20..23) | flags (24..28)
+//   child: offset of the node's upstream nodes in the child list (device
+//          indices, in the reference's routing rank)
+//   packed: depth (bits 0..19) | nchild (20..23) | flags (24..28)
 enum NodeFlag {
   NF_LINK = 1 << 24,    // node has an outgoing link (computes qTR)
   NF_QOUT = 1 << 25,    // node adds its own qOUT to qTIN
@@ -108,9 +108,44 @@ struct NodeMeta {
 };
 __host__ __device__ inline int nm_depth(int32_t y) { return y & 0xFFFFF; }
 __host__ __device__ inline int nm_nchild(int32_t y) { return (y >> 20) & 0xF; }
+constexpr int32_t NF_MASK = NF_LINK | NF_QOUT | NF_CUT | NF_GAUGE | NF_INFLOW;
+
+// Tiled routing.  Each domain's tree is cut into connected tiles of at most
+// kTileSlots slots and kTileHeight levels; a tile holds its nodes plus one
+// proxy slot per upstream node that roots another tile.  Tiles are packed
+// into bins of kTileSlots slots, one workgroup per (bin, parameter set).
+constexpr int kTileSlots = 256;
+constexpr int kTileHeight = 32;
+constexpr int kTileWindowMax = 24;  // steps per window (LDS: (B + 2) * slots * 8 B)
+
+// One slot of a bin (16 bytes).
+//   pos: device node index of a real slot, -1 for proxies and padding
+//   bnd: boundary row read by a proxy / written by a tile root, else -1
+//   a:   lag (bits 0..11) | nchild (12..15) | SF_* | NF_* flags of the node
+//   b:   bin-local slot of the first upstream slot (bits 0..15) | tile depth (16..31)
+// lag = tile height - 1 - local depth: the slot handles window step j on the
+// bin's internal diagonal j + lag.
+enum SlotFlag {
+  SF_REAL = 1 << 16,   // a node of this tile
+  SF_PROXY = 1 << 17,  // stands for the root of an upstream tile
+  SF_TROOT = 1 << 18,  // roots a tile that drains into another tile
+};
+struct SlotMeta {
+  int32_t pos;
+  int32_t bnd;
+  int32_t a;
+  int32_t b;
+};
+struct BinInfo {
+  int32_t height;  // tallest tile in the bin
+  int32_t tau_lo;  // smallest / largest tile depth in the bin
+  int32_t tau_hi;
+  int32_t pad;
+};
 
 struct RouteArgs {
   const NodeMeta *meta;    // [n_nodes]
+  const int32_t *child_list;  // upstream device indices, NodeMeta.child offsets
   const double *C1;        // [ld]
   const double *C2;        // [ld]
   double *qTIN;            // [ld]   qTIN(:,IT1)
@@ -132,6 +167,18 @@ struct RouteArgs {
   int32_t n_inflow;
   int32_t dmax;            // maximum depth over the nodes this launch covers
   double tst;              // HourSecs * timestep [s]
+};
+
+// Extra arguments of the tiled routing kernel.
+struct TileArgs {
+  const SlotMeta *slot;    // [n_bins * kTileSlots]
+  const BinInfo *bin;      // [n_bins]
+  double *bnd;             // [2][n_sets][n_bnd][kTileWindowMax] tile-root outflow by window parity
+  int64_t n_bnd;
+  int32_t tau_max;         // deepest tile
+  int32_t B;               // steps per window
+  int32_t nwin;            // windows in [t_lo, t_hi)
+  int32_t pad;
 };
 
 }  // namespace mhmhip

@@ -163,11 +163,26 @@ __device__ __forceinline__ double runoff_acc(double q, double area, double tst) 
 template <int NH>
 __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, FluxRegs<NH> &x,
                                            double maxinter, double evap) {
-  canopy_interc(x.pet, maxinter, x.prec, c.inter, x.thr, x.ecan);
+  const double ratio = canopy_fill(maxinter, x.prec, c.inter, x.thr);
+  // The canopy's (interc / interc_max)**(2/3) and every horizon's
+  // (sm / sat)**beta depend only on values known here: one call evaluates
+  // these NH + 1 powers side by side.  Arguments the reference never raises
+  // to a power are replaced by 1 and their results left unused.
+  DV<NH + 1> py, px;
+  py.v[0] = kTwoThird;
+  px.v[0] = ratio > 0.0 ? ratio : 1.0;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    const double q = c.sm[h] / c.sat[h];
+    py.v[h + 1] = c.bexp[h];
+    px.v[h + 1] = q > 0.0 ? q : 1.0;
+  }
+  const DV<NH + 1> pw = exp_ylogx_v<NH + 1>(py, px);
+  canopy_evap(x.pet, maxinter, ratio, pw.v[0], c.inter, x.ecan);
   snow_accum_melt(c.incr, c.dmaxs, c.dnps, c.ddthr, x.prec, x.temp, c.tth, x.thr, c.pack, x.ddf,
                   x.melt, x.peff, x.rain, x.snow);
-  soil_moisture<NH>(a.jarvis != 0, c.fS, c.thrS, x.pet, evap, c.sat, c.fr, c.fc, c.wp, c.bexp,
-                    c.c1, x.ecan, x.peff, x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
+  soil_moisture<NH>(a.jarvis != 0, c.fS, c.thrS, x.pet, evap, c.sat, c.fr, c.fc, c.wp, c.c1,
+                    x.ecan, x.peff, &pw.v[1], x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
   runoff_unsat_zone(c.k1s, c.kps, c.k0s, c.alpha, c.karst, x.infil[NH - 1], c.thrU, c.satS,
                     c.unsat, x.slow, x.fast, x.perc);
   runoff_sat_zone(c.k2s, c.satS, x.bflow);
@@ -214,7 +229,11 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
 // and the last step's fluxes are written at the end.  Per-step fluxes are
 // scoped to the loop body so they do not stay live across steps.
 template <int NH>
-__global__ __launch_bounds__(256) void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
+#ifndef MHMHIP_FUSED_WAVES
+#define MHMHIP_FUSED_WAVES 2
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHMHIP_FUSED_WAVES)))
+void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
                                                          int32_t step0) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
@@ -279,10 +298,11 @@ __device__ __forceinline__ void route_node(const RouteArgs &r, uint32_t p, int32
   }
   if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qout);
   // qTIN(iNode,IT): upstream qTR in routing rank, then own qOUT
-  const double *pub_t = r.pub + (t & 1) * ld + (int64_t)set * r.sstride + m.child;
+  const double *pub_t = r.pub + (t & 1) * ld + (int64_t)set * r.sstride;
+  const int32_t *up = r.child_list + m.child;
   double s = 0.0;
   const int nc = nm_nchild(y);
-  for (int c = 0; c < nc; ++c) s = s + pub_t[c];
+  for (int c = 0; c < nc; ++c) s = s + pub_t[up[c]];
   if (y & NF_QOUT) s = s + qout;
   if (y & NF_LINK) {
     const double qtin1 = ldr(r.qTIN, bv);
@@ -329,6 +349,102 @@ __global__ __launch_bounds__(1024) void route_wg_kernel(RouteArgs r, const int64
       if (t >= r.t_lo && t < r.t_hi) route_node(r, p, set, t, m);
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled routing: launch L handles, for every tile of depth tau, the window
+// k = L - (tau_max - tau) of B steps.  Tile-root outflow of window k is
+// written to the boundary buffer (parity k & 1) in launch L and read by the
+// downstream tile's proxy slot in launch L + 1.  Inside the window the
+// workgroup walks its own (depth, step) diagonals: slot with lag g handles
+// window step j on diagonal j + g, upstream slots (lag g - 1) produced step j
+// on the diagonal before, and a two-entry LDS parity buffer keeps it
+// readable while they write step j + 1.  qOUT (or a proxy's boundary
+// series) of the whole window is staged in LDS first.
+// Same per-node arithmetic as route_node (mo_mrm_routing.f90:425-472).
+__global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, TileArgs g,
+                                                                int32_t L) {
+  extern __shared__ double lds[];
+  const int W = kTileSlots;
+  const int bin = blockIdx.x, set = blockIdx.y, s = threadIdx.x;
+  const BinInfo bi = g.bin[bin];
+  // whole bin idle in this launch: every thread leaves before any barrier
+  if (L - (g.tau_max - bi.tau_hi) < 0 || L - (g.tau_max - bi.tau_lo) >= g.nwin) return;
+  const SlotMeta m = g.slot[(int64_t)bin * W + s];
+  const int32_t k = L - (g.tau_max - (m.b >> 16));
+  const bool real = (m.a & SF_REAL) != 0, proxy = (m.a & SF_PROXY) != 0;
+  const bool act = (real || proxy) && k >= 0 && k < g.nwin;
+  const int64_t t0 = r.t_lo + (int64_t)k * g.B;
+  const int nb = act ? (int)min((int64_t)g.B, r.t_hi - t0) : 0;
+  const int64_t ld = (int64_t)r.n_sets * r.sstride;
+  const uint32_t bv = real ? (uint32_t)(set * r.sstride + m.pos) * 8u : 0u;
+  double *qin = lds;             // [B][W]
+  double *qtr = lds + g.B * W;   // [2][W]
+  double qtin1 = 0.0, qtr1 = 0.0, c1 = 0.0, c2 = 0.0;
+  if (act && real) {
+    for (int j = 0; j < nb; ++j) {
+      const int64_t t = t0 + j;
+      double q = ldr(r.ring + (t % r.ring_len) * ld, bv);
+      if (m.a & NF_INFLOW) {
+        for (int32_t u = 0; u < r.n_inflow; ++u) {
+          if (r.inflow_node[u] == m.pos) {
+            const double qi = r.inflow_q[(t - r.t_base) * r.n_inflow + u];
+            q = r.inflow_hw[u] ? q + qi : qi;
+          }
+        }
+      }
+      qin[j * W + s] = q;
+    }
+    qtin1 = ldr(r.qTIN, bv);
+    if (m.a & NF_LINK) {
+      qtr1 = ldr(r.pub + ((t0 - 1) & 1) * ld, bv);
+      c1 = ldr(r.C1, bv);
+      c2 = ldr(r.C2, bv);
+    }
+  } else if (act) {
+    const double *src = g.bnd + (((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd + m.bnd) * kTileWindowMax;
+    for (int j = 0; j < nb; ++j) qin[j * W + s] = src[j];
+  }
+  __syncthreads();
+  const int lag = m.a & 0xFFF, nc = (m.a >> 12) & 0xF, c0 = m.b & 0xFFFF;
+  double *bout = (m.a & SF_TROOT)
+                     ? g.bnd + (((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd + m.bnd) * kTileWindowMax
+                     : nullptr;
+  const int nd = bi.height + g.B - 1;
+  for (int d = 0; d < nd; ++d) {
+    const int j = d - lag;
+    if (act && j >= 0 && j < nb) {
+      double *qt = qtr + (j & 1) * W;
+      if (proxy) {
+        qt[s] = qin[j * W + s];
+      } else {
+        const double qout = qin[j * W + s];
+        double sum = 0.0;
+        for (int c = 0; c < nc; ++c) sum = sum + qt[c0 + c];
+        if (m.a & NF_QOUT) sum = sum + qout;
+        if (m.a & NF_LINK) {
+          double q = qtr1 + c1 * (qtin1 - qtr1) + c2 * (sum - qtin1);
+          if (m.a & NF_CUT) q = 0.0;
+          qt[s] = q;
+          qtr1 = q;
+          if (bout) bout[j] = q;
+        }
+        qtin1 = sum;
+        const int64_t t = t0 + j;
+        if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qout);
+        if (m.a & NF_GAUGE) {
+          for (int32_t u = 0; u < r.n_gauges; ++u)
+            if (r.gauge_node[u] == m.pos)
+              r.gauge_out[((t - r.t_base) * r.n_sets + set) * r.n_gauges + u] = sum;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (act && real) {
+    str(r.qTIN, bv, qtin1);
+    if (m.a & NF_LINK) str(r.pub + ((t0 + nb - 1) & 1) * ld, bv, qtr1);
   }
 }
 
@@ -382,6 +498,15 @@ hipError_t launch_route_diag(const RouteArgs &r, int64_t d, int64_t p_lo, int64_
   dim3 grid((unsigned)((p_hi - p_lo + 255) / 256), (unsigned)r.n_sets);
   hipLaunchKernelGGL(route_diag_kernel, grid, dim3(256), 0, s, r, d, (uint32_t)p_lo,
                      (uint32_t)p_hi);
+  return hipGetLastError();
+}
+
+hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t n_bins, int32_t L,
+                             hipStream_t s) {
+  if (n_bins < 1 || g.B < 1 || g.B > kTileWindowMax || g.nwin < 1) return hipErrorInvalidValue;
+  dim3 grid((unsigned)n_bins, (unsigned)r.n_sets);
+  const size_t shm = sizeof(double) * (size_t)(g.B + 2) * kTileSlots;
+  hipLaunchKernelGGL(route_tile_kernel, grid, dim3(kTileSlots), shm, s, r, g, L);
   return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@
 #include "../../include/mhmhip.h"
 #include "network.h"
 
+
 extern "C" int mhmhip_routing_order(int64_t n_nodes, int64_t n_links, const int32_t *fromN,
                                     const int32_t *toN, int32_t *rOrder, int32_t *netPerm) {
   if (n_nodes < 1 || n_links < 0 || n_links > n_nodes || (n_links > 0 && (!fromN || !toN)) ||
@@ -80,10 +81,19 @@ extern "C" int mhmhip_routing_order(int64_t n_nodes, int64_t n_links, const int3
 
 namespace mhmhip {
 
-// Breadth-first layout of every domain's forest: roots are the domain's sink
-// nodes in node order, the upstream nodes of a node are appended in the
-// reference's routing rank, so they are contiguous and their qTR sum runs in
-// the order of L11_routing's accumulation.
+// Device layout of every domain's forest.  Depth = links to the sink.  The
+// upstream nodes of a node are kept in the reference's routing rank, so
+// their qTR sum runs in the order of L11_routing's accumulation.
+//
+// Tile partition, bottom-up (reverse breadth-first order): a node's open
+// tile takes its upstream nodes' open tiles while it stays within
+// kTileSlots slots and kTileHeight levels; otherwise the largest upstream
+// open tile (by slots, or by levels when the height is exceeded) is closed
+// -- it becomes a tile of its own and a proxy slot in the node's tile.
+// Tiles are numbered breadth-first over the tile tree (tile depth tau
+// nondecreasing), laid out breadth-first inside (upstream slots of a slot
+// contiguous, in routing rank) and packed next-fit into bins of kTileSlots
+// slots; device indices follow the real slots of the bins in order.
 int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
   const int64_t nN = in.n_nodes;
   out.pos.assign(nN, -1);
@@ -93,10 +103,20 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
   out.link_of_node.assign(nN, -1);
   out.dom_dmax.assign(in.n_domains, 0);
   out.dmax = 0;
+  out.child_list.clear();
+  out.slots.clear();
+  out.bins.clear();
+  out.tau_max = 0;
+  out.n_bnd = 0;
+  out.n_tiles = 0;
   int64_t link_off = 0;
   int64_t next_pos = 0;
   for (int d = 0; d < in.n_domains; ++d) {
     const int64_t n0 = in.dom_node_start[d], n1 = in.dom_node_start[d + 1], nd = n1 - n0;
+    if (n0 != next_pos) {
+      err = "bind_network: domain node ranges must be contiguous from 0";
+      return MHMHIP_EINVAL;
+    }
     const int64_t nOut = in.n_outlets[d];
     const int64_t nL = nd - nOut;
     if (nOut < 1 || nL < 0) {
@@ -155,11 +175,158 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
       return MHMHIP_EINVAL;
     }
     int32_t dm = 0;
-    const int64_t base = next_pos;
-    for (int64_t h = 0; h < nd; ++h) out.pos[n0 + q[h]] = base + h;
-    for (int64_t h = 0; h < nd; ++h) {
+    for (int64_t n = 0; n < nd; ++n) dm = std::max(dm, depth[n]);
+    if (dm >= (1 << 20)) {
+      err = "bind_network: network deeper than 2^20 levels";
+      return MHMHIP_EINVAL;
+    }
+    auto child_node = [&](int64_t i) { return (int64_t)fr[i] - 1; };
+    // -- tile partition (open tile size and height per node) --
+    std::vector<int32_t> osize(nd), oh(nd), ocut(nd);
+    std::vector<char> cut(nd, 0);
+    for (int64_t h = nd - 1; h >= 0; --h) {
       const int64_t n = q[h];
-      const int64_t p = base + h;
+      int32_t sz = 1, ht = 1;
+      for (int64_t i : up[n]) {
+        sz += osize[child_node(i)];
+        ht = std::max(ht, 1 + oh[child_node(i)]);
+      }
+      // over the height limit: close the tallest upstream tiles
+      while (ht > kTileHeight) {
+        int64_t best = -1;
+        for (int64_t i : up[n]) {
+          const int64_t c = child_node(i);
+          if (!cut[c] && (best < 0 || oh[c] > oh[best])) best = c;
+        }
+        cut[best] = 1;
+        sz += 1 - osize[best];
+        ht = 1;
+        for (int64_t i : up[n]) {
+          const int64_t c = child_node(i);
+          ht = std::max(ht, 1 + (cut[c] ? 1 : oh[c]));
+        }
+      }
+      // over the slot limit: close the upstream tiles with the fewest cuts
+      // below them first (closing adds one to their paths), the largest
+      // among equals
+      while (sz > kTileSlots) {
+        int64_t best = -1;
+        for (int64_t i : up[n]) {
+          const int64_t c = child_node(i);
+          if (cut[c]) continue;
+          if (best < 0 || ocut[c] < ocut[best] || (ocut[c] == ocut[best] && osize[c] > osize[best]))
+            best = c;
+        }
+        if (best < 0) break;  // cannot happen: 1 + nchild <= kTileSlots
+        cut[best] = 1;
+        sz += 1 - osize[best];
+      }
+      int32_t oc = 0;
+      for (int64_t i : up[n]) {
+        const int64_t c = child_node(i);
+        oc = std::max(oc, ocut[c] + (cut[c] ? 1 : 0));
+      }
+      ocut[n] = oc;
+      osize[n] = sz;
+      oh[n] = ht;
+    }
+    // -- tiles, breadth-first over the tile tree --
+    struct TSlot {
+      int64_t node;  // domain-local node
+      bool proxy;
+      int32_t ldepth, first, nc;
+    };
+    struct Tile {
+      std::vector<TSlot> s;
+      int32_t tau, height;
+      int64_t root;
+    };
+    std::vector<Tile> tiles;
+    std::vector<int64_t> troots;
+    std::vector<int32_t> ttau;
+    for (int64_t n = 0; n < nd; ++n)
+      if (out_link[n] < 0) {
+        troots.push_back(n);
+        ttau.push_back(0);
+      }
+    for (size_t ti = 0; ti < troots.size(); ++ti) {
+      Tile T;
+      T.root = troots[ti];
+      T.tau = ttau[ti];
+      T.s.push_back({T.root, false, 0, 0, 0});
+      for (size_t k = 0; k < T.s.size(); ++k) {
+        if (T.s[k].proxy) continue;
+        const int64_t n = T.s[k].node;
+        const int32_t first = (int32_t)T.s.size(), ld1 = T.s[k].ldepth + 1;
+        T.s[k].first = first;
+        T.s[k].nc = (int32_t)up[n].size();
+        for (int64_t i : up[n]) {
+          const int64_t c = child_node(i);
+          T.s.push_back({c, cut[c] != 0, ld1, 0, 0});
+          if (cut[c]) {
+            troots.push_back(c);
+            ttau.push_back(T.tau + 1);
+          }
+        }
+      }
+      T.height = 0;
+      for (const TSlot &u : T.s) T.height = std::max(T.height, u.ldepth + 1);
+      if ((int)T.s.size() > kTileSlots || T.height > kTileHeight) {
+        err = "bind_network: tile partition failed";
+        return MHMHIP_EINVAL;
+      }
+      tiles.push_back(std::move(T));
+    }
+    // -- bins (next-fit) and device indices --
+    std::vector<int64_t> bnd_row(nd, -1);
+    for (int64_t n = 0; n < nd; ++n)
+      if (cut[n]) bnd_row[n] = out.n_bnd++;
+    size_t t0 = 0;
+    while (t0 < tiles.size()) {
+      size_t t1 = t0;
+      int32_t used = 0;
+      BinInfo bi{0, tiles[t0].tau, tiles[t0].tau, 0};
+      while (t1 < tiles.size() && used + (int32_t)tiles[t1].s.size() <= kTileSlots) {
+        used += (int32_t)tiles[t1].s.size();
+        bi.height = std::max(bi.height, tiles[t1].height);
+        bi.tau_lo = std::min(bi.tau_lo, tiles[t1].tau);
+        bi.tau_hi = std::max(bi.tau_hi, tiles[t1].tau);
+        ++t1;
+      }
+      const size_t sbase = out.slots.size();
+      out.slots.resize(sbase + kTileSlots, SlotMeta{-1, -1, 0, 0});
+      int32_t off = 0;
+      for (size_t t = t0; t < t1; ++t) {
+        const Tile &T = tiles[t];
+        for (size_t k = 0; k < T.s.size(); ++k) {
+          const TSlot &u = T.s[k];
+          SlotMeta &m = out.slots[sbase + off + k];
+          const int32_t lag = T.height - 1 - u.ldepth;
+          if (u.proxy) {
+            m.bnd = (int32_t)bnd_row[u.node];
+            m.a = lag | SF_PROXY;
+          } else {
+            const int64_t p = next_pos++;
+            out.pos[n0 + u.node] = p;
+            m.pos = (int32_t)p;
+            m.a = lag | (u.nc << 12) | SF_REAL;
+            if (k == 0 && cut[u.node]) {
+              m.bnd = (int32_t)bnd_row[u.node];
+              m.a |= SF_TROOT;
+            }
+          }
+          m.b = (u.proxy ? 0 : (off + u.first)) | (T.tau << 16);
+          out.tau_max = std::max(out.tau_max, T.tau);
+        }
+        off += (int32_t)T.s.size();
+      }
+      out.bins.push_back(bi);
+      out.n_tiles += (int64_t)(t1 - t0);
+      t0 = t1;
+    }
+    // -- node metadata and child list --
+    for (int64_t n = 0; n < nd; ++n) {
+      const int64_t p = out.pos[n0 + n];
       out.node_at[p] = n0 + n;
       int32_t flags = 0;
       if (out_link[n] >= 0) {
@@ -168,17 +335,15 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
       }
       if (n == last_t) flags |= NF_QOUT;
       const int32_t nc = (int32_t)up[n].size();
-      out.child[p] = nc ? (int32_t)out.pos[n0 + fr[up[n][0]] - 1] : 0;
-      if (depth[n] >= (1 << 20)) {
-        err = "bind_network: network deeper than 2^20 levels";
-        return MHMHIP_EINVAL;
-      }
       out.packed[p] = depth[n] | (nc << 20) | flags;
-      dm = std::max(dm, depth[n]);
+    }
+    for (int64_t p = n0; p < n1; ++p) {
+      const int64_t n = out.node_at[p] - n0;
+      out.child[p] = (int32_t)out.child_list.size();
+      for (int64_t i : up[n]) out.child_list.push_back((int32_t)out.pos[n0 + child_node(i)]);
     }
     out.dom_dmax[d] = dm;
     out.dmax = std::max(out.dmax, dm);
-    next_pos += nd;
     link_off += nL;
   }
   out.n_links_total = link_off;
@@ -186,3 +351,35 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
 }
 
 }  // namespace mhmhip
+
+extern "C" int mhmhip_plan_network(int64_t n_nodes, int32_t n_outlets, const int32_t *netPerm,
+                                   const int32_t *fromN, const int32_t *toN, mhmhip_plan *plan,
+                                   int64_t *pos, int32_t *slots, int64_t slots_cap) {
+  if (n_nodes < 1 || n_outlets < 1 || n_outlets > n_nodes || !plan ||
+      (n_nodes > n_outlets && (!netPerm || !fromN || !toN)))
+    return MHMHIP_EINVAL;
+  const int64_t start[2] = {0, n_nodes};
+  mhmhip::LayoutIn in{n_nodes, 1, start, &n_outlets, netPerm, fromN, toN};
+  mhmhip::LayoutOut lay;
+  std::string err;
+  const int rc = mhmhip::build_layout(in, lay, err);
+  if (rc != MHMHIP_OK) return rc;
+  plan->dmax = lay.dmax;
+  plan->tau_max = lay.tau_max;
+  plan->n_tiles = lay.n_tiles;
+  plan->n_bins = (int64_t)lay.bins.size();
+  plan->n_bnd = lay.n_bnd;
+  plan->n_slots = (int64_t)lay.slots.size();
+  if (pos)
+    for (int64_t i = 0; i < n_nodes; ++i) pos[i] = lay.pos[i];
+  if (slots) {
+    if (slots_cap < plan->n_slots) return MHMHIP_EINVAL;
+    for (size_t k = 0; k < lay.slots.size(); ++k) {
+      slots[4 * k + 0] = lay.slots[k].pos;
+      slots[4 * k + 1] = lay.slots[k].bnd;
+      slots[4 * k + 2] = lay.slots[k].a;
+      slots[4 * k + 3] = lay.slots[k].b;
+    }
+  }
+  return MHMHIP_OK;
+}

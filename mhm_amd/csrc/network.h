@@ -1,4 +1,4 @@
-// network.h -- breadth-first device layout of the packed L11 networks.
+// network.h -- tiled device layout of the packed L11 networks.
 #pragma once
 
 #include <cstdint>
@@ -22,14 +22,24 @@ struct LayoutIn {
 struct LayoutOut {
   std::vector<int64_t> pos;           // reference node (global 0-based) -> device index
   std::vector<int64_t> node_at;       // device index -> reference node
-  std::vector<int32_t> child;         // NodeMeta.child
+  std::vector<int32_t> child;         // NodeMeta.child (offset into child_list)
+  std::vector<int32_t> child_list;    // upstream device indices in routing rank
   std::vector<int32_t> packed;        // NodeMeta.packed
   std::vector<int64_t> link_of_node;  // device index -> packed link index (or -1)
   std::vector<int32_t> dom_dmax;
   int32_t dmax;
   int64_t n_links_total;
+  // tiles (kTileSlots slots per bin)
+  std::vector<SlotMeta> slots;        // [n_bins * kTileSlots]; a without NF_* flags
+  std::vector<BinInfo> bins;
+  int32_t tau_max;
+  int64_t n_bnd;
+  int64_t n_tiles;
 };
 
+// Validates the packed networks and builds the device layout: the tile
+// partition of every domain's trees, tile-major device indices (domains stay
+// contiguous), node metadata and the child list in routing rank.
 int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err);
 
 }  // namespace mhmhip

@@ -46,7 +46,8 @@ def test_step_by_step_matches_oracle():
     model.close()
 
 
-@pytest.mark.parametrize("mode,S,route_mode", [(0, 0, 1), (1, 24, 1), (1, 5, 2), (1, 48, 2)])
+@pytest.mark.parametrize("mode,S,route_mode", [(0, 0, 1), (1, 24, 1), (1, 5, 2), (1, 48, 2),
+                                               (0, 0, 3), (1, 24, 3), (1, 5, 3), (1, 48, 3)])
 def test_run_matches_oracle(mode, S, route_mode):
     _log(f"case: run mode={mode} S={S} route_mode={route_mode}")
     dom = synth.make_domain(20, 28, seed=5)
@@ -104,24 +105,29 @@ def test_profiling_inside_graph_replays():
     m2, _, _, _ = make_pair([dom], n_sets=1, n_steps=48, seed=6)
     m2.set_profiling(True, 3)
     for a in (0, 24):
-        g1 = m1.run(steps[a:a + 24], mode=1, steps_per_launch=8, route_mode=1)
-        g2 = m2.run(steps[a:a + 24], mode=1, steps_per_launch=8, route_mode=1, use_graph=True)
-        assert np.array_equal(g1, g2)
+        for rm in (1, 3):
+            g1 = m1.run(steps[a:a + 12], mode=1, steps_per_launch=6, route_mode=rm)
+            g2 = m2.run(steps[a:a + 12], mode=1, steps_per_launch=6, route_mode=rm, use_graph=True)
+            assert np.array_equal(g1, g2)
+            a += 12
     p = m2.profile()
-    assert p["cell_fused"][1] == 6 and p["cell_fused"][0] > 0.0
-    assert p["cell_fused"][2] == pytest.approx(6 * 8 * dom.n_cells)
+    assert p["cell_fused"][1] == 8 and p["cell_fused"][0] > 0.0
+    assert p["cell_fused"][2] == pytest.approx(8 * 6 * dom.n_cells)
     assert p["route_diag"][1] > 0 and p["route_diag"][0] > 0.0
+    assert p["route_tile"][1] > 0 and p["route_tile"][0] > 0.0
+    assert p["route_tile"][2] == pytest.approx(24 * dom.n_cells)
     m1.close()
     m2.close()
 
 
-def test_packed_domains_match_oracle():
+@pytest.mark.parametrize("route_mode", [1, 3])
+def test_packed_domains_match_oracle(route_mode):
     """Uneven basins with their own parameters packed in one context."""
-    _log("case: packed domains")
+    _log(f"case: packed domains route_mode={route_mode}")
     doms = [synth.make_domain(12, 20, seed=21), synth.make_domain(30, 30, seed=22),
             synth.make_domain(5, 7, seed=23)]
     model, sim, steps, recs = make_pair(doms, n_sets=1, n_steps=40, seed=9)
-    g = model.run(steps, mode=1, steps_per_launch=16, route_mode=1)
+    g = model.run(steps, mode=1, steps_per_launch=16, route_mode=route_mode)
     go = _oracle_series(sim, recs)
     assert rel_close(g, go, RTOL, ATOL)[0]
     assert compare_cells(model, sim, RTOL, ATOL) == []

@@ -1,0 +1,179 @@
+"""Accuracy of the chain's own log / exp / pow (mhm_amd/csrc/chain.h).
+
+The cell kernels evaluate the reference's x**y (mo_canopy_interc.f90:121,
+mo_soil_moisture.f90:219, mo_runoff.f90:134) as exp(y log x) with a log and
+an exp written from their series (no library calls), so the same source
+compiles for the host and for gfx950.  Pinned here on the host build:
+  * against mpmath at 200 bits: log and exp within 1 ulp, pow within
+    (2 + 3 |y log x|) ulp (log's 1 ulp and the rounding of y log x, amplified
+    by exp), over the ranges the chain feeds them and beyond;
+  * closed forms: log(2^k) = k ln 2, log(1) = 0, exp(0) = 1, exp(1) = e,
+    exp(z) exp(-z) = 1, x**1 = x, 4**0.5 = 2, 2**10 = 1024, overflow and
+    underflow, subnormal arguments;
+  * planted mistakes (a dropped series term in exact-rounding Python copies
+    of log_pos and exp_fin) fail the 1-ulp checks.
+The GPU test checks that the device build gives the host's bits.
+"""
+import ctypes as C
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+mp = pytest.importorskip("mpmath")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "mhm_amd", "libmhmcheck.so")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        from mhm_amd import build
+        build.build_check()
+    h = C.CDLL(LIB)
+    for f in (h.mhmhip_check_eval_host, h.mhmhip_check_eval_device):
+        f.restype = C.c_int
+        f.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 3 + [C.c_int64]
+    return h
+
+
+def _eval(lib, which, x, y=None, device=False):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
+    out = np.empty_like(x)
+    p = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    f = lib.mhmhip_check_eval_device if device else lib.mhmhip_check_eval_host
+    assert f(which, p(x), p(y), p(out), x.shape[0]) == 0
+    return out
+
+
+def _ulp_err(got, exact):
+    """|got - exact| in units of the last place of the double nearest exact."""
+    e = float(exact)
+    return float(abs(mp.mpf(got) - exact) / mp.mpf(math.ulp(e)))
+
+
+mp.mp.prec = 200
+
+
+def test_log_within_one_ulp(lib):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([np.exp(rng.uniform(-700, 700, 3000)),
+                        1.0 + rng.uniform(-0.3, 0.45, 3000),          # the reduction's edges
+                        rng.uniform(1e-12, 2.0, 2000),
+                        np.array([0.7071067811865475, 0.7071067811865476, 1.4142135623730951,
+                                  1.0 - 2**-53, 1.0 + 2**-52, 5e-324, 2.2250738585072014e-308])])
+    got = _eval(lib, 0, x)
+    worst = max(_ulp_err(g, mp.log(mp.mpf(v))) for g, v in zip(got, x) if v != 1.0)
+    assert worst <= 1.0, worst
+
+
+def test_exp_within_one_ulp(lib):
+    rng = np.random.default_rng(2)
+    z = np.concatenate([rng.uniform(-700, 700, 3000), rng.uniform(-1, 1, 3000),
+                        rng.uniform(-1e-8, 1e-8, 500), np.array([0.34657359, -0.34657359])])
+    got = _eval(lib, 1, z)
+    worst = max(_ulp_err(g, mp.exp(mp.mpf(v))) for g, v in zip(got, z))
+    assert worst <= 1.0, worst
+
+
+def test_pow_error_bound(lib):
+    """x in the chain's ranges: ratios in (0, 1] (canopy, soil), storages up
+    to 1e4 mm (unsaturated zone); exponents 2/3, 0.1..10."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(1e-9, 1.0, 4000), np.exp(rng.uniform(-20, 9.2, 4000))])
+    y = np.concatenate([np.full(1000, 2.0 / 3.0), rng.uniform(0.1, 10.0, 7000)])
+    got = _eval(lib, 2, x, y)
+    for g, a, b in zip(got, x, y):
+        exact = mp.power(mp.mpf(a), mp.mpf(b))
+        bound = 2.0 + 3.0 * abs(b * math.log(a))
+        assert _ulp_err(g, exact) <= bound, (a, b, g)
+
+
+def test_batched_equals_scalar(lib):
+    """exp_ylogx_v<4> (four powers in lockstep) gives each element the bits
+    of the scalar evaluation."""
+    rng = np.random.default_rng(6)
+    x = np.exp(rng.uniform(-30, 9, 4001))
+    y = rng.uniform(0.1, 10.0, 4001)
+    assert np.array_equal(_eval(lib, 3, x, y), _eval(lib, 2, x, y))
+
+
+def test_closed_forms(lib):
+    k = np.arange(-1074, 1024, dtype=np.float64)
+    lg = _eval(lib, 0, np.ldexp(1.0, k.astype(int)))
+    exact = [mp.mpf(int(v)) * mp.log(2) for v in k]
+    assert max(_ulp_err(g, e) for g, e in zip(lg, exact) if e != 0) <= 0.5
+    assert _eval(lib, 0, [1.0])[0] == 0.0
+    assert _eval(lib, 1, [0.0])[0] == 1.0
+    assert _ulp_err(_eval(lib, 1, [1.0])[0], mp.e) <= 0.5
+    z = np.random.default_rng(5).uniform(-300, 300, 2000)
+    prod = _eval(lib, 1, z) * _eval(lib, 1, -z)
+    assert np.max(np.abs(prod - 1.0)) <= 4.5e-16
+    x = np.array([3.7, 0.123, 1e-5, 42.0])
+    assert np.all(np.abs(_eval(lib, 2, x, np.ones(4)) - x) <= (2 + 3 * np.abs(np.log(x))) * np.spacing(x))
+    assert _eval(lib, 2, [4.0], [0.5])[0] == 2.0
+    assert _eval(lib, 2, [2.0], [10.0])[0] == 1024.0
+    assert _eval(lib, 1, [800.0])[0] == np.inf
+    assert _eval(lib, 1, [-800.0])[0] == 0.0
+
+
+LN2_HI = float.fromhex("0x1.62e42fee00000p-1")
+LN2_LO = float.fromhex("0x1.a39ef35793c76p-33")
+
+
+def _fma(a, b, c):
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _log_copy(x, terms=9):
+    """log_pos restated in Python (every operation correctly rounded)."""
+    m, e = math.frexp(x)
+    if m < 0.7071067811865476:
+        m, e = m + m, e - 1
+    f = m - 1.0
+    s = f / (2.0 + f)
+    z = s * s
+    r = 2.0 / (2 * terms + 1)
+    for k in range(terms - 1, 0, -1):
+        r = _fma(r, z, 2.0 / (2 * k + 1))
+    r = r * z
+    hfsq = 0.5 * (f * f)
+    return _fma(float(e), LN2_HI,
+                f - (hfsq - _fma(s, hfsq + r, float(e) * LN2_LO)))
+
+
+def _exp_copy(z, top=13):
+    k = float(round(z * 1.4426950408889634))
+    r = _fma(-k, LN2_HI, z)
+    r = _fma(-k, LN2_LO, r)
+    p = 1.0 / math.factorial(top)
+    for n in range(top - 1, -1, -1):
+        p = _fma(p, r, 1.0 / math.factorial(n))
+    return math.ldexp(p, int(k))
+
+
+def test_copies_match_the_library_and_planted_mistakes_fail(lib):
+    xs = [0.7071067811865476, 1.4142135623730949, 1.41, 0.7072, 3.3e-7, 812.5]
+    zs = [0.3465, -0.3465, 5.0 + 0.3465, -7.0 - 0.3465]
+    assert [_log_copy(x) for x in xs] == list(_eval(lib, 0, xs))
+    assert [_exp_copy(z) for z in zs] == list(_eval(lib, 1, zs))
+    # a dropped term: R(z) through z^8, exp through r^11
+    assert max(_ulp_err(_log_copy(x, terms=8), mp.log(mp.mpf(x))) for x in xs) > 1.0
+    assert max(_ulp_err(_exp_copy(z, top=11), mp.exp(mp.mpf(z))) for z in zs) > 1.0
+
+
+@pytest.mark.gpu
+def test_device_gives_host_bits(lib):
+    print("case: pow device vs host bits", flush=True)
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(1e-9, 1.0, 50000), np.exp(rng.uniform(-700, 700, 50000))])
+    y = rng.uniform(0.1, 10.0, x.shape[0])
+    z = rng.uniform(-745, 709, x.shape[0])
+    assert np.array_equal(_eval(lib, 0, x, device=True), _eval(lib, 0, x))
+    assert np.array_equal(_eval(lib, 1, z, device=True), _eval(lib, 1, z))
+    assert np.array_equal(_eval(lib, 2, x, y, device=True), _eval(lib, 2, x, y))
+    assert np.array_equal(_eval(lib, 3, x, y, device=True), _eval(lib, 2, x, y))
