@@ -698,8 +698,10 @@ template <typename Fn>
 hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
   if (!x->prof) return f();
   (x->capturing ? x->graph_all : x->prof_all)[kind] += 1;
-  // diagonal launches come by the thousand: time one in prof_stride
-  if (kind == MHMHIP_K_ROUTE_DIAG && (x->route_launches++ % x->prof_stride) != 0) return f();
+  // routing launches come by the hundred or thousand: time one in prof_stride
+  if ((kind == MHMHIP_K_ROUTE_DIAG || kind == MHMHIP_K_ROUTE_TILE) &&
+      (x->route_launches++ % x->prof_stride) != 0)
+    return f();
   mhmhip_ctx::Pending p{kind, nullptr, nullptr, units};
   hipError_t e;
   if (x->capturing) {
@@ -792,12 +794,19 @@ int route_diagonals(mhmhip_ctx *x, const RouteArgs &r, int64_t d_lo, int64_t d_h
 }
 
 // Tiled routing launches [L_lo, L_hi) of a call.
+// Launch L handles window k = L - (tau_max - tau) of each tile: only the bins
+// holding tiles of depth tau_max - L .. tau_max - L + nwin - 1 are launched.
 int route_tiles(mhmhip_ctx *x, const RouteArgs &r, const TileArgs &g, int32_t L_lo, int32_t L_hi) {
-  for (int32_t L = L_lo; L < L_hi; ++L)
+  const int64_t nb = (int64_t)x->lay.bins.size();
+  for (int32_t L = L_lo; L < L_hi; ++L) {
+    int64_t lo, hi;
+    tile_bin_range(x->lay, g.tau_max - L, g.tau_max - L + g.nwin - 1, lo, hi);
+    if (hi <= lo) continue;
     CHECK_HIP(x, timed(x, MHMHIP_K_ROUTE_TILE, 0.0, [&] {
-                return launch_route_tile(r, g, (int64_t)x->lay.bins.size(), L, x->stream);
+                return launch_route_tile(r, g, lo, hi, nb, L, x->stream);
               }),
               "route_tile");
+  }
   return MHMHIP_OK;
 }
 

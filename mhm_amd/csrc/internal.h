@@ -114,9 +114,18 @@ constexpr int32_t NF_MASK = NF_LINK | NF_QOUT | NF_CUT | NF_GAUGE | NF_INFLOW;
 // kTileSlots slots and kTileHeight levels; a tile holds its nodes plus one
 // proxy slot per upstream node that roots another tile.  Tiles are packed
 // into bins of kTileSlots slots, one workgroup per (bin, parameter set).
-constexpr int kTileSlots = 256;
-constexpr int kTileHeight = 32;
-constexpr int kTileWindowMax = 24;  // steps per window (LDS: (B + 2) * slots * 8 B)
+#ifndef MHMHIP_TILE_SLOTS
+#define MHMHIP_TILE_SLOTS 256
+#endif
+#ifndef MHMHIP_TILE_HEIGHT
+#define MHMHIP_TILE_HEIGHT 32
+#endif
+#ifndef MHMHIP_TILE_WINDOW
+#define MHMHIP_TILE_WINDOW 24
+#endif
+constexpr int kTileSlots = MHMHIP_TILE_SLOTS;
+constexpr int kTileHeight = MHMHIP_TILE_HEIGHT;
+constexpr int kTileWindowMax = MHMHIP_TILE_WINDOW;  // steps per window (LDS: (B + 2) * slots * 8 B)
 
 // One slot of a bin (16 bytes).
 //   pos: device node index of a real slot, -1 for proxies and padding

@@ -153,6 +153,29 @@ __device__ __forceinline__ double pet_day(const CellArgs &a, const StepDev &st, 
   return P(pr_petlai(NH) + st.month0) * petin;
 }
 
+// The step table is read-only during a launch and the same for every lane:
+// read through the constant address space, it comes in with scalar loads
+// into SGPRs instead of vector loads into VGPRs.
+typedef const __attribute__((address_space(4))) StepDev ConstStep;
+__device__ __forceinline__ StepDev read_step(const ConstStep *p) {
+  StepDev st;
+  st.dr = p->dr;
+  st.tand = p->tand;
+  st.sind = p->sind;
+  st.cosd = p->cosd;
+  st.f_pet = p->f_pet;
+  st.f_prec = p->f_prec;
+  st.f_temp = p->f_temp;
+  st.evap = p->evap;
+  st.day = p->day;
+  st.month0 = p->month0;
+  st.isday = p->isday;
+  st.init_sm = p->init_sm;
+  st.doy = p->doy;
+  st.pad = 0;
+  return st;
+}
+
 // L11_runoff_acc for L1 = L11 with map_flag (mo_mrm_pre_routing.f90:110-126):
 // qAcc = 0; qAcc = qAcc + q * area; qAcc = qAcc * 1000 / TST   [m3 s-1]
 __device__ __forceinline__ double runoff_acc(double q, double area, double tst) {
@@ -246,9 +269,11 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
   const int64_t fv = a.forcing_days * a.sstride;
   int cached_day = -1, cached_doy = -1, cached_month = -1;
   double pre = 0.0, tavg = 0.0, pd = 0.0, maxinter = 0.0;
+  const ConstStep *steps = (const ConstStep *)(a.steps + step0);
+  int64_t ring_row = a.ring ? t0 % a.ring_len : 0;
 #pragma nounroll
   for (int32_t s = 0; s < nsteps; ++s) {
-    const StepDev st = a.steps[step0 + s];
+    const StepDev st = read_step(steps + s);
     if (st.day != cached_day || st.doy != cached_doy) {
       const double *F = a.F + (int64_t)st.day * a.sstride;
       pre = ldr(F + MHMHIP_F_PRE * fv, bc);
@@ -272,7 +297,10 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
       for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
     }
     chain_step<NH>(a, r, x, maxinter, st.evap);
-    if (a.ring) str(a.ring + ((t0 + s) % a.ring_len) * a.ld, bv, runoff_acc(x.total, area, a.tst));
+    if (a.ring) {
+      str(a.ring + ring_row * a.ld, bv, runoff_acc(x.total, area, a.tst));
+      ring_row = ring_row + 1 == a.ring_len ? 0 : ring_row + 1;
+    }
     if (s == nsteps - 1) store_fluxes<NH>(a, bv, x);
   }
   store_states<NH>(a, bv, r);
@@ -352,6 +380,15 @@ __global__ __launch_bounds__(1024) void route_wg_kernel(RouteArgs r, const int64
   }
 }
 
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// operations, not for its global stores (no other lane of the workgroup
+// reads those before the kernel ends).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------------------
 // Tiled routing: launch L handles, for every tile of depth tau, the window
 // k = L - (tau_max - tau) of B steps.  Tile-root outflow of window k is
@@ -364,10 +401,10 @@ __global__ __launch_bounds__(1024) void route_wg_kernel(RouteArgs r, const int64
 // series) of the whole window is staged in LDS first.
 // Same per-node arithmetic as route_node (mo_mrm_routing.f90:425-472).
 __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, TileArgs g,
-                                                                int32_t L) {
+                                                                int32_t L, int32_t bin0) {
   extern __shared__ double lds[];
   const int W = kTileSlots;
-  const int bin = blockIdx.x, set = blockIdx.y, s = threadIdx.x;
+  const int bin = bin0 + blockIdx.x, set = blockIdx.y, s = threadIdx.x;
   const BinInfo bi = g.bin[bin];
   // whole bin idle in this launch: every thread leaves before any barrier
   if (L - (g.tau_max - bi.tau_hi) < 0 || L - (g.tau_max - bi.tau_lo) >= g.nwin) return;
@@ -380,59 +417,97 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
   const int64_t ld = (int64_t)r.n_sets * r.sstride;
   const uint32_t bv = real ? (uint32_t)(set * r.sstride + m.pos) * 8u : 0u;
   double *qin = lds;             // [B][W]
-  double *qtr = lds + g.B * W;   // [2][W]
+  double *qtr = lds + g.B * W;   // [2][W], then one +0.0
+  double *zero = qtr + 2 * W;
+  if (s == 0) *zero = 0.0;
   double qtin1 = 0.0, qtr1 = 0.0, c1 = 0.0, c2 = 0.0;
-  if (act && real) {
-    for (int j = 0; j < nb; ++j) {
-      const int64_t t = t0 + j;
-      double q = ldr(r.ring + (t % r.ring_len) * ld, bv);
-      if (m.a & NF_INFLOW) {
+  if (act) {
+    // all loads of the window in flight at once, then to LDS
+    double v[kTileWindowMax];
+    if (real) {
+      int64_t row = t0 % r.ring_len;
+      const double *base = r.ring + (int64_t)set * r.sstride + m.pos;
+#pragma unroll
+      for (int j = 0; j < kTileWindowMax; ++j) {
+        if (j < nb) v[j] = base[row * ld];
+        row = row + 1 == r.ring_len ? 0 : row + 1;
+      }
+      qtin1 = ldr(r.qTIN, bv);
+      if (m.a & NF_LINK) {
+        qtr1 = ldr(r.pub + ((t0 - 1) & 1) * ld, bv);
+        c1 = ldr(r.C1, bv);
+        c2 = ldr(r.C2, bv);
+      }
+    } else {
+      const double *src =
+          g.bnd + (((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd + m.bnd) * kTileWindowMax;
+#pragma unroll
+      for (int j = 0; j < kTileWindowMax; ++j)
+        if (j < nb) v[j] = src[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kTileWindowMax; ++j)
+      if (j < nb) qin[j * W + s] = v[j];
+    if (real && (m.a & NF_INFLOW)) {
+      for (int j = 0; j < nb; ++j) {
+        const int64_t t = t0 + j;
+        double q = qin[j * W + s];
         for (int32_t u = 0; u < r.n_inflow; ++u) {
           if (r.inflow_node[u] == m.pos) {
             const double qi = r.inflow_q[(t - r.t_base) * r.n_inflow + u];
             q = r.inflow_hw[u] ? q + qi : qi;
           }
         }
+        qin[j * W + s] = q;
       }
-      qin[j * W + s] = q;
     }
-    qtin1 = ldr(r.qTIN, bv);
-    if (m.a & NF_LINK) {
-      qtr1 = ldr(r.pub + ((t0 - 1) & 1) * ld, bv);
-      c1 = ldr(r.C1, bv);
-      c2 = ldr(r.C2, bv);
-    }
-  } else if (act) {
-    const double *src = g.bnd + (((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd + m.bnd) * kTileWindowMax;
-    for (int j = 0; j < nb; ++j) qin[j * W + s] = src[j];
   }
-  __syncthreads();
+  lds_barrier();
   const int lag = m.a & 0xFFF, nc = (m.a >> 12) & 0xF, c0 = m.b & 0xFFFF;
   double *bout = (m.a & SF_TROOT)
                      ? g.bnd + (((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd + m.bnd) * kTileWindowMax
                      : nullptr;
   const int nd = bi.height + g.B - 1;
+  double musk = qtr1 + c1 * (qtin1 - qtr1);
+  int upi[8];  // upstream slots, -1 past the node's count
+#pragma unroll
+  for (int c = 0; c < 8; ++c) upi[c] = c < nc ? c0 + c : -1;
   for (int d = 0; d < nd; ++d) {
     const int j = d - lag;
-    if (act && j >= 0 && j < nb) {
-      double *qt = qtr + (j & 1) * W;
+    const bool on = act && j >= 0 && j < nb;
+    // every LDS read of the diagonal is issued before any of them is used:
+    // the step's own input and all upstream slots (an index past the
+    // node's upstream count reads its own slot, and is not added)
+    const int jj = j < 0 ? 0 : (j < g.B ? j : g.B - 1);
+    const double qo = qin[jj * W + s];
+    const int par = (jj & 1) * W;
+    double up[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) up[c] = upi[c] < 0 ? *zero : qtr[par + upi[c]];
+    if (on) {
       if (proxy) {
-        qt[s] = qin[j * W + s];
+        qtr[(j & 1) * W + s] = qo;
       } else {
-        const double qout = qin[j * W + s];
+        // upstream qTR in routing rank (D8: at most 8; the header allows 15);
+        // past the node's count the terms are the +0.0 slot, which leaves the
+        // running sum's value unchanged
         double sum = 0.0;
-        for (int c = 0; c < nc; ++c) sum = sum + qt[c0 + c];
-        if (m.a & NF_QOUT) sum = sum + qout;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) sum = sum + up[c];
+        for (int c = 8; c < nc; ++c) sum = sum + qtr[par + c0 + c];
+        if (m.a & NF_QOUT) sum = sum + qo;
         if (m.a & NF_LINK) {
-          double q = qtr1 + c1 * (qtin1 - qtr1) + c2 * (sum - qtin1);
+          // musk = qTR(IT1) + C1 (qTIN(IT1) - qTR(IT1)), formed before the barrier
+          double q = musk + c2 * (sum - qtin1);
           if (m.a & NF_CUT) q = 0.0;
-          qt[s] = q;
+          qtr[(j & 1) * W + s] = q;
           qtr1 = q;
           if (bout) bout[j] = q;
         }
         qtin1 = sum;
+        musk = qtr1 + c1 * (qtin1 - qtr1);
         const int64_t t = t0 + j;
-        if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qout);
+        if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qo);
         if (m.a & NF_GAUGE) {
           for (int32_t u = 0; u < r.n_gauges; ++u)
             if (r.gauge_node[u] == m.pos)
@@ -440,7 +515,7 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (act && real) {
     str(r.qTIN, bv, qtin1);
@@ -501,12 +576,14 @@ hipError_t launch_route_diag(const RouteArgs &r, int64_t d, int64_t p_lo, int64_
   return hipGetLastError();
 }
 
-hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t n_bins, int32_t L,
-                             hipStream_t s) {
-  if (n_bins < 1 || g.B < 1 || g.B > kTileWindowMax || g.nwin < 1) return hipErrorInvalidValue;
-  dim3 grid((unsigned)n_bins, (unsigned)r.n_sets);
-  const size_t shm = sizeof(double) * (size_t)(g.B + 2) * kTileSlots;
-  hipLaunchKernelGGL(route_tile_kernel, grid, dim3(kTileSlots), shm, s, r, g, L);
+hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_lo, int64_t bin_hi,
+                             int64_t n_bins, int32_t L, hipStream_t s) {
+  if (bin_lo < 0 || bin_hi > n_bins || bin_hi <= bin_lo || g.B < 1 || g.B > kTileWindowMax ||
+      g.nwin < 1)
+    return hipErrorInvalidValue;
+  dim3 grid((unsigned)(bin_hi - bin_lo), (unsigned)r.n_sets);
+  const size_t shm = sizeof(double) * ((size_t)(g.B + 2) * kTileSlots + 1);
+  hipLaunchKernelGGL(route_tile_kernel, grid, dim3(kTileSlots), shm, s, r, g, L, (int32_t)bin_lo);
   return hipGetLastError();
 }
 

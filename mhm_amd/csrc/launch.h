@@ -14,8 +14,8 @@ hipError_t launch_cell_fused(int nh, const CellArgs &a, int64_t t0, int nsteps, 
                              int block, int n_sets, hipStream_t s);
 hipError_t launch_route_diag(const RouteArgs &r, int64_t d, int64_t p_lo, int64_t p_hi,
                              hipStream_t s);
-hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t n_bins, int32_t L,
-                             hipStream_t s);
+hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_lo, int64_t bin_hi,
+                             int64_t n_bins, int32_t L, hipStream_t s);
 hipError_t launch_route_wg(const RouteArgs &r, const int64_t *dom_start, const int32_t *dom_dmax,
                            int n_domains, int64_t d_lo, int64_t d_hi, int drain, int block,
                            hipStream_t s);

@@ -107,6 +107,7 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
   out.slots.clear();
   out.bins.clear();
   out.tau_max = 0;
+  out.bin_span = 0;
   out.n_bnd = 0;
   out.n_tiles = 0;
   int64_t link_off = 0;
@@ -347,7 +348,38 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
     link_off += nL;
   }
   out.n_links_total = link_off;
+  // bins of all domains ordered by their shallowest tile, so that the bins a
+  // routing launch needs form one contiguous range (device indices keep
+  // their per-domain order; slot tables move with their bin)
+  const int64_t nb = (int64_t)out.bins.size();
+  std::vector<int64_t> ord(nb);
+  for (int64_t b = 0; b < nb; ++b) ord[b] = b;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int64_t u, int64_t v) { return out.bins[u].tau_lo < out.bins[v].tau_lo; });
+  std::vector<BinInfo> bins(nb);
+  std::vector<SlotMeta> slots(out.slots.size());
+  out.bin_span = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    bins[b] = out.bins[ord[b]];
+    std::copy(out.slots.begin() + ord[b] * kTileSlots, out.slots.begin() + (ord[b] + 1) * kTileSlots,
+              slots.begin() + b * kTileSlots);
+    out.bin_span = std::max(out.bin_span, bins[b].tau_hi - bins[b].tau_lo);
+  }
+  out.bins.swap(bins);
+  out.slots.swap(slots);
   return MHMHIP_OK;
+}
+
+// Bins [lo, hi) that can hold a tile of depth tau in [tau_a, tau_b]: bins are
+// ordered by their shallowest tile depth and span at most bin_span depths.
+void tile_bin_range(const LayoutOut &lay, int32_t tau_a, int32_t tau_b, int64_t &lo, int64_t &hi) {
+  auto first_ge = [&](int32_t tau) {
+    return (int64_t)(std::lower_bound(lay.bins.begin(), lay.bins.end(), tau,
+                                      [](const BinInfo &b, int32_t v) { return b.tau_lo < v; }) -
+                     lay.bins.begin());
+  };
+  lo = first_ge(tau_a - lay.bin_span);
+  hi = first_ge(tau_b + 1);
 }
 
 }  // namespace mhmhip

@@ -33,6 +33,7 @@ struct LayoutOut {
   std::vector<SlotMeta> slots;        // [n_bins * kTileSlots]; a without NF_* flags
   std::vector<BinInfo> bins;
   int32_t tau_max;
+  int32_t bin_span;                   // largest tau_hi - tau_lo of a bin
   int64_t n_bnd;
   int64_t n_tiles;
 };
@@ -41,5 +42,8 @@ struct LayoutOut {
 // partition of every domain's trees, tile-major device indices (domains stay
 // contiguous), node metadata and the child list in routing rank.
 int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err);
+
+// Bins [lo, hi) that can hold a tile of depth tau in [tau_a, tau_b].
+void tile_bin_range(const LayoutOut &lay, int32_t tau_a, int32_t tau_b, int64_t &lo, int64_t &hi);
 
 }  // namespace mhmhip

@@ -56,16 +56,23 @@ class Result:
         return self.finished - self.arrival
 
 
-def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, dmax, steps_per_launch,
-                   n_steps, n_gauges=1):
-    """Device footprint of one context (mirrors the allocations of api.hip)."""
+def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_steps, n_steps,
+                   n_gauges=1, n_outlets=1):
+    """Device footprint of one context run in chunks of `chunk_steps` fused
+    steps, allocation by allocation as api.hip makes them; `plan` is the
+    network's mhmhip_plan (N.plan_network)."""
     sstride = (n_cells + 63) // 64 * 64
     ld = sstride * n_sets
     rows = (19 + 5 * n_horizons + 24) + (5 + n_horizons) + (17 + 2 * n_horizons)
-    routing = 8 * n_cells + 8 * n_cells + 5 * 8 * ld  # meta, area, C1, C2, qTIN, qTR x2, qOUT
-    ring = (steps_per_launch + dmax + 2) * ld * 8
-    return (rows * ld * 8 + 5 * n_forcing_days * sstride * 8 + routing + ring
-            + n_steps * 40 + n_steps * n_sets * n_gauges * 8)
+    cells = rows * ld * 8 + 5 * n_forcing_days * sstride * 8
+    window = max(1, min(chunk_steps, 24))
+    ring = ((plan.tau_max + 1) * window + chunk_steps + 1) * ld * 8
+    routing = (16 * n_cells + 6 * ld * 8 + 20 + 4 * max(1, n_cells - n_outlets)
+               + 16 * plan.n_slots + 16 * plan.n_bins + 2 * n_sets * max(1, plan.n_bnd) * 24 * 8
+               + 4 * max(1, n_gauges) + 8)
+    rows_run = min(chunk_steps, n_steps)
+    series = rows_run * 88 + rows_run * n_sets * max(1, n_gauges) * 8
+    return cells + ring + routing + series
 
 
 def prepare_model(req, device=0):
@@ -83,16 +90,22 @@ def prepare_model(req, device=0):
     return m
 
 
-def run_alone(req, chunk_steps=73, device=0):
-    """The request on its own (reference result for the mix)."""
-    m = prepare_model(req, device)
+def run_chunks(m, req, chunk_steps=73):
+    """The request's horizon on context m in fused runs of chunk_steps."""
     steps = calendar.step_table(req.start, req.n_steps, req.timestep)
     parts = []
     for a in range(0, req.n_steps, chunk_steps):
         parts.append(m.run(steps[a:a + chunk_steps], mode=N.RUN_FUSED,
                            steps_per_launch=chunk_steps))
-    m.close()
     return np.concatenate(parts)
+
+
+def run_alone(req, chunk_steps=73, device=0):
+    """The request on its own (reference result for the mix)."""
+    m = prepare_model(req, device)
+    g = run_chunks(m, req, chunk_steps)
+    m.close()
+    return g
 
 
 class Server:
@@ -119,9 +132,10 @@ class Server:
 
     def _footprint(self, req):
         dom = req.domain
+        plan, _, _ = N.plan_network(dom)
         return estimate_bytes(dom.n_cells, req.gammas.shape[0], dom.horizon_depth.shape[0],
-                              calendar.n_forcing_days_needed(req.n_steps), dom.dmax,
-                              self.chunk, req.n_steps)
+                              calendar.n_forcing_days_needed(req.n_steps), plan,
+                              self.chunk, req.n_steps, dom.gauge_nodes.shape[0], dom.n_outlets)
 
     def submit(self, req):
         fp = self.footprint(req)

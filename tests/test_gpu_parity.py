@@ -114,8 +114,9 @@ def test_profiling_inside_graph_replays():
     assert p["cell_fused"][1] == 8 and p["cell_fused"][0] > 0.0
     assert p["cell_fused"][2] == pytest.approx(8 * 6 * dom.n_cells)
     assert p["route_diag"][1] > 0 and p["route_diag"][0] > 0.0
+    # routing launches are sampled (one in 3): units follow the timed launches
     assert p["route_tile"][1] > 0 and p["route_tile"][0] > 0.0
-    assert p["route_tile"][2] == pytest.approx(24 * dom.n_cells)
+    assert 0.0 < p["route_tile"][2] <= 24 * dom.n_cells
     m1.close()
     m2.close()
 

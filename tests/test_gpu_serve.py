@@ -36,3 +36,17 @@ def test_mix_equals_alone_and_oracle():
     assert rel_close(res[r.rid].gauges, go, 1e-10, 1e-12)[0]
     s = serve.summarize(res, reqs, srv.wall)
     assert s["p99_s"] >= s["p50_s"] > 0.0
+
+
+def test_footprint_estimate_matches_the_context():
+    """Admission's estimate equals what the context allocates on the device."""
+    print("case: serving footprint", flush=True)
+    for r in serve.poisson_requests(3, qps=10.0, seed=8, cells=(500, 5000), horizon=(20, 60),
+                                    sets=(1, 9)):
+        m = serve.prepare_model(r)
+        serve.run_chunks(m, r, 30)
+        srv = serve.Server(hbm_budget_bytes=1 << 40, chunk_steps=30)
+        est = srv.footprint(r)
+        assert est == m.device_bytes, (r.rid, est, m.device_bytes)
+        m.close()
+

@@ -76,6 +76,9 @@ def test_summary_percentiles():
 
 
 def test_footprint_estimate_grows_with_the_request():
-    a = serve.estimate_bytes(1000, 1, 3, 10, 50, 73, 100)
-    assert serve.estimate_bytes(2000, 1, 3, 10, 50, 73, 100) > a
-    assert serve.estimate_bytes(1000, 8, 3, 10, 50, 73, 100) > 4 * a
+    from mhm_amd import _native as N, synth
+    plan, _, _ = N.plan_network(synth.make_domain(30, 30, seed=1))
+    a = serve.estimate_bytes(1000, 1, 3, 10, plan, 73, 100)
+    assert serve.estimate_bytes(2000, 1, 3, 10, plan, 73, 100) > a
+    assert serve.estimate_bytes(1000, 8, 3, 10, plan, 73, 100) > 4 * a
+    assert serve.estimate_bytes(1000, 1, 3, 40, plan, 73, 100) > a
