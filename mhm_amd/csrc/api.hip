@@ -554,6 +554,7 @@ int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
   d.f_prec = d.isday ? x->mt.fday_prec[m] : x->mt.fnight_prec[m];
   d.f_temp = d.isday ? x->mt.fday_temp[m] : x->mt.fnight_temp[m];
   d.evap = x->mt.evap[m];
+  d.revap = d.evap != 0.0 ? 1.0 / d.evap : 0.0;
   d.init_sm = (s.tt == 1) && !s.read_states;  // mo_mhm.f90:448
   d.doy = s.doy;
   d.pad = 0;
@@ -569,6 +570,7 @@ CellArgs cell_args(mhmhip_ctx *x, bool ring) {
   a.ring = ring ? x->d_ring : nullptr;
   a.area = x->d_area;
   a.tst = kHourSecs * (double)x->cfg.timestep;  // mo_mrm_pre_routing.f90:110
+  a.rtst = 1.0 / a.tst;
   a.steps = x->d_steps;
   a.n_cells = x->n_cells;
   a.sstride = x->sstride;
@@ -579,6 +581,7 @@ CellArgs cell_args(mhmhip_ctx *x, bool ring) {
   a.jarvis = (x->cfg.process_soil == 2 || x->cfg.process_soil == 3);
   a.hourly = x->cfg.hourly_forcing;
   a.nT = x->nT;
+  a.rnT = 1.0 / x->nT;
   return a;
 }
 

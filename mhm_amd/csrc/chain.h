@@ -166,6 +166,16 @@ __host__ __device__ inline double exp_ylogx(double y, double x) {
   return exp_ylogx_v<1>(DV<1>{{y}}, DV<1>{{x}}).v[0];
 }
 
+// x / d for a divisor whose correctly rounded reciprocal r = RN(1/d) is known:
+// q = x r, then one correction with the exact residual e = x - q d (fma):
+// RN(q + e r) is the correctly rounded quotient (Markstein, IBM J. Res. Dev.
+// 34 (1990) 111-119; finite, normal operands and quotient), i.e. the same
+// bits as x / d in three operations instead of a full division.
+__host__ __device__ inline double div_rcp(double x, double d, double r) {
+  const double q = x * r;
+  return fma(fma(-q, d, x), r, q);
+}
+
 // pow(x, y) of the reference for y > 0 and the x > 0 power already in hand:
 // pow's value for non-integer y when x <= 0 (0 for x == 0, NaN below)
 __device__ __forceinline__ double pow_pos_from(double x, double px) {
@@ -281,7 +291,8 @@ __device__ __forceinline__ double jarvis(double sm, double sat, double wp, doubl
 // line 219), evaluated for all horizons before the horizon loop.
 template <int NH>
 __device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double thrS, double pet,
-                                              double evap, const double *sat, const double *fr,
+                                              double evap, double revap, const double *sat,
+                                              const double *fr,
                                               const double *fc, const double *wp, double c1,
                                               double ecan, double peff, const double *frac_pw,
                                               double &runoff_sealed, double &storage_sealed,
@@ -299,7 +310,9 @@ __device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double
       storage_sealed = tmp;
     }
     if (thrS > kEps) {
-      aet_sealed = (pet / evap - ecan) * (storage_sealed / thrS);
+      // revap = RN(1/evap): pet / evap as div_rcp, same bits
+      const double pe = revap != 0.0 ? div_rcp(pet, evap, revap) : pet / evap;
+      aet_sealed = (pe - ecan) * (storage_sealed / thrS);
       if (aet_sealed < 0.0) aet_sealed = 0.0;
     } else {
       aet_sealed = DBL_MAX;  // huge(1.0_dp)
@@ -420,8 +433,9 @@ __device__ MHMHIP_OUTLINE double pet_hargreaves(double coeff, double tavg, doubl
 
 // src/meteo/mo_meteo_temporal_tools.f90:54-79 with the day or night factor f
 // already chosen: 2 * v * f / ntimesteps_day
-__device__ __forceinline__ double disagg_flux(double nT, double v, double f) {
-  if (nT > 1.0) return 2.0 * v * f / nT;
+// (rnT = RN(1/nT): the division by nT as div_rcp, same bits)
+__device__ __forceinline__ double disagg_flux(double nT, double rnT, double v, double f) {
+  if (nT > 1.0) return div_rcp(2.0 * v * f, nT, rnT);
   return v;
 }
 

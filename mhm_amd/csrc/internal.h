@@ -55,6 +55,7 @@ struct StepDev {
   // monthly namelist factors of this step, day or night already selected
   // (mo_meteo_handler.f90:1111-1117, 1192-1198, 1273-1279; mo_mhm.f90:473)
   double f_pet, f_prec, f_temp, evap;
+  double revap;                 // RN(1 / evap), 0 when evap is 0
   int32_t day;                  // 0-based forcing column
   int32_t month0;               // 0..11
   int32_t isday;                // 6 < hour <= 18
@@ -78,6 +79,7 @@ struct CellArgs {
   double *ring;            // nullptr: no routing input wanted
   const double *area;      // [n_cells] L1 cell area [km2] (routing input)
   double tst;              // HourSecs * timestep [s]
+  double rtst;             // RN(1 / tst)
   const StepDev *steps;    // device step table (fused kernel)
   int64_t n_cells;
   int64_t sstride;
@@ -89,6 +91,7 @@ struct CellArgs {
   int32_t hourly;
   int32_t pad;
   double nT;               // steps per day as real (nTstepDay_dp)
+  double rnT;              // RN(1 / nT)
 };
 
 // Routing node metadata, packed in 8 bytes per node.

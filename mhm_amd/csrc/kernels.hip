@@ -167,6 +167,7 @@ __device__ __forceinline__ StepDev read_step(const ConstStep *p) {
   st.f_prec = p->f_prec;
   st.f_temp = p->f_temp;
   st.evap = p->evap;
+  st.revap = p->revap;
   st.day = p->day;
   st.month0 = p->month0;
   st.isday = p->isday;
@@ -178,14 +179,15 @@ __device__ __forceinline__ StepDev read_step(const ConstStep *p) {
 
 // L11_runoff_acc for L1 = L11 with map_flag (mo_mrm_pre_routing.f90:110-126):
 // qAcc = 0; qAcc = qAcc + q * area; qAcc = qAcc * 1000 / TST   [m3 s-1]
-__device__ __forceinline__ double runoff_acc(double q, double area, double tst) {
-  return ((0.0 + q * area) * 1000.0) / tst;
+// (rtst = RN(1/TST): the division as div_rcp, same bits)
+__device__ __forceinline__ double runoff_acc(double q, double area, double tst, double rtst) {
+  return div_rcp((0.0 + q * area) * 1000.0, tst, rtst);
 }
 
 // One step of mHM() for one cell after the meteo getters (mo_mhm.f90:455-519)
 template <int NH>
 __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, FluxRegs<NH> &x,
-                                           double maxinter, double evap) {
+                                           double maxinter, double evap, double revap) {
   const double ratio = canopy_fill(maxinter, x.prec, c.inter, x.thr);
   // The canopy's (interc / interc_max)**(2/3) and every horizon's
   // (sm / sat)**beta depend only on values known here: one call evaluates
@@ -204,7 +206,7 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
   canopy_evap(x.pet, maxinter, ratio, pw.v[0], c.inter, x.ecan);
   snow_accum_melt(c.incr, c.dmaxs, c.dnps, c.ddthr, x.prec, x.temp, c.tth, x.thr, c.pack, x.ddf,
                   x.melt, x.peff, x.rain, x.snow);
-  soil_moisture<NH>(a.jarvis != 0, c.fS, c.thrS, x.pet, evap, c.sat, c.fr, c.fc, c.wp, c.c1,
+  soil_moisture<NH>(a.jarvis != 0, c.fS, c.thrS, x.pet, evap, revap, c.sat, c.fr, c.fc, c.wp, c.c1,
                     x.ecan, x.peff, &pw.v[1], x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
   runoff_unsat_zone(c.k1s, c.kps, c.k0s, c.alpha, c.karst, x.infil[NH - 1], c.thrU, c.satS,
                     c.unsat, x.slow, x.fast, x.perc);
@@ -230,18 +232,18 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
   const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, bc) : 0.0;
   const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, bc) : 0.0;
   const double pd = pet_day<NH>(a, st, bv, bc, tavg, tmax, tmin);
-  x.pet = a.hourly ? pd : disagg_flux(a.nT, pd, st.f_pet);
+  x.pet = a.hourly ? pd : disagg_flux(a.nT, a.rnT, pd, st.f_pet);
   x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
-  x.prec = a.hourly ? pre : disagg_flux(a.nT, pre, st.f_prec);
+  x.prec = a.hourly ? pre : disagg_flux(a.nT, a.rnT, pre, st.f_prec);
   if (st.init_sm) {
 #pragma unroll
     for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
   }
   const double maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld, bv);
-  chain_step<NH>(a, r, x, maxinter, st.evap);
+  chain_step<NH>(a, r, x, maxinter, st.evap, st.revap);
   store_states<NH>(a, bv, r);
   store_fluxes<NH>(a, bv, x);
-  if (a.ring) str(a.ring + (t % a.ring_len) * a.ld, bv, runoff_acc(x.total, ldr(a.area, bc), a.tst));
+  if (a.ring) str(a.ring + (t % a.ring_len) * a.ld, bv, runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst));
 }
 
 // ---------------------------------------------------------------------------
@@ -289,16 +291,16 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
       cached_month = st.month0;
     }
     FluxRegs<NH> x;
-    x.pet = a.hourly ? pd : disagg_flux(a.nT, pd, st.f_pet);
+    x.pet = a.hourly ? pd : disagg_flux(a.nT, a.rnT, pd, st.f_pet);
     x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
-    x.prec = a.hourly ? pre : disagg_flux(a.nT, pre, st.f_prec);
+    x.prec = a.hourly ? pre : disagg_flux(a.nT, a.rnT, pre, st.f_prec);
     if (st.init_sm) {
 #pragma unroll
       for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
     }
-    chain_step<NH>(a, r, x, maxinter, st.evap);
+    chain_step<NH>(a, r, x, maxinter, st.evap, st.revap);
     if (a.ring) {
-      str(a.ring + ring_row * a.ld, bv, runoff_acc(x.total, area, a.tst));
+      str(a.ring + ring_row * a.ld, bv, runoff_acc(x.total, area, a.tst, a.rtst));
       ring_row = ring_row + 1 == a.ring_len ? 0 : ring_row + 1;
     }
     if (s == nsteps - 1) store_fluxes<NH>(a, bv, x);

@@ -71,7 +71,7 @@ def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_step
                + 16 * plan.n_slots + 16 * plan.n_bins + 2 * n_sets * max(1, plan.n_bnd) * 24 * 8
                + 4 * max(1, n_gauges) + 8)
     rows_run = min(chunk_steps, n_steps)
-    series = rows_run * 88 + rows_run * n_sets * max(1, n_gauges) * 8
+    series = rows_run * 96 + rows_run * n_sets * max(1, n_gauges) * 8
     return cells + ring + routing + series
 
 

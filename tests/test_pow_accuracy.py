@@ -102,6 +102,21 @@ def test_batched_equals_scalar(lib):
     assert np.array_equal(_eval(lib, 3, x, y), _eval(lib, 2, x, y))
 
 
+def test_division_by_known_reciprocal_is_exact(lib):
+    """div_rcp (x * RN(1/d) plus one fma correction) gives the bits of x / d
+    for the divisors the kernels use it with (steps per day, seconds per
+    step, monthly evaporation coefficients) and random ones, including
+    quotients next to exact multiples."""
+    rng = np.random.default_rng(7)
+    d = np.concatenate([np.full(20000, 24.0), np.full(20000, 3600.0), np.full(20000, 86400.0),
+                        rng.uniform(0.3, 2.0, 40000), np.exp(rng.uniform(-7, 7, 40000))])
+    x = np.ldexp(rng.uniform(0.5, 1.0, d.shape[0]), rng.integers(-60, 60, d.shape[0]))
+    x[:3000] = d[:3000] * rng.integers(1, 500, 3000)
+    x[3000:6000] = np.nextafter(d[3000:6000] * rng.integers(1, 500, 3000), np.inf)
+    assert np.array_equal(_eval(lib, 4, x, d), x / d)
+    assert np.array_equal(_eval(lib, 5, x, d), x / d)
+
+
 def test_closed_forms(lib):
     k = np.arange(-1074, 1024, dtype=np.float64)
     lg = _eval(lib, 0, np.ldexp(1.0, k.astype(int)))
@@ -177,3 +192,5 @@ def test_device_gives_host_bits(lib):
     assert np.array_equal(_eval(lib, 1, z, device=True), _eval(lib, 1, z))
     assert np.array_equal(_eval(lib, 2, x, y, device=True), _eval(lib, 2, x, y))
     assert np.array_equal(_eval(lib, 3, x, y, device=True), _eval(lib, 2, x, y))
+    d = rng.uniform(0.3, 4000.0, x.shape[0])
+    assert np.array_equal(_eval(lib, 4, x, d, device=True), x / d)
