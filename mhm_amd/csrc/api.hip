@@ -193,6 +193,12 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
     g_create_error = "create: configuration out of range";
     return MHMHIP_EINVAL;
   }
+  for (int m = 0; m < 12; ++m)
+    if (!(c.evap_coeff[m] > 0.0) || !std::isfinite(c.evap_coeff[m])) {
+      // the sealed-storage loss divides by it every step (mo_soil_moisture.f90:151)
+      g_create_error = "create: evap_coeff must be positive and finite";
+      return MHMHIP_EINVAL;
+    }
   std::vector<int64_t> ds(c.n_domains + 1);
   if (c.domain_cell_start) {
     for (int d = 0; d <= c.n_domains; ++d) ds[d] = c.domain_cell_start[d];
@@ -554,7 +560,7 @@ int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
   d.f_prec = d.isday ? x->mt.fday_prec[m] : x->mt.fnight_prec[m];
   d.f_temp = d.isday ? x->mt.fday_temp[m] : x->mt.fnight_temp[m];
   d.evap = x->mt.evap[m];
-  d.revap = d.evap != 0.0 ? 1.0 / d.evap : 0.0;
+  d.revap = 1.0 / d.evap;
   d.init_sm = (s.tt == 1) && !s.read_states;  // mo_mhm.f90:448
   d.doy = s.doy;
   d.pad = 0;

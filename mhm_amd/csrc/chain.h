@@ -196,8 +196,9 @@ __device__ __forceinline__ bool le_eps(double a, double b) {
 // (interc / interc_max)**(2/3): canopy_fill updates the storage and the
 // throughfall and returns the ratio (1.0 when interc_max <= eps, where the
 // reference takes no power); canopy_evap finishes with the ratio's power.
-__device__ __forceinline__ double canopy_fill(double interc_max, double precip, double &interc,
-                                              double &throughfall) {
+// (rmax = RN(1/interc_max): the ratio as div_rcp, same bits)
+__device__ __forceinline__ double canopy_fill(double interc_max, double rmax, double precip,
+                                              double &interc, double &throughfall) {
   const double aux = interc + precip;
   if (aux >= interc_max) {
     throughfall = aux - interc_max;
@@ -206,7 +207,7 @@ __device__ __forceinline__ double canopy_fill(double interc_max, double precip, 
     throughfall = 0.0;
     interc = aux;
   }
-  return interc_max > kEps ? interc / interc_max : 1.0;
+  return interc_max > kEps ? div_rcp(interc, interc_max, rmax) : 1.0;
 }
 
 // ratio: canopy_fill's result; pw: ratio**(2/3) when ratio > 0
@@ -267,9 +268,11 @@ __device__ __forceinline__ void snow_accum_melt(double incr, double dmax, double
 }
 
 // src/mHM/mo_soil_moisture.f90:333-363
-__device__ __forceinline__ double feddes(double sm, double fc, double wp, double fr) {
+// (rfw = RN(1/(fc - wp)): the division as div_rcp, same bits; it is only
+// reached with fc > sm > wp)
+__device__ __forceinline__ double feddes(double sm, double fc, double wp, double fr, double rfw) {
   if (sm >= fc) return fr;
-  if (sm > wp) return fr * (sm - wp) / (fc - wp);
+  if (sm > wp) return div_rcp(fr * (sm - wp), fc - wp, rfw);
   return 0.0;
 }
 
@@ -289,11 +292,12 @@ __device__ __forceinline__ double jarvis(double sm, double sat, double wp, doubl
 // src/mHM/mo_soil_moisture.f90:94-290.  frac_pw[h] = (sm[h] / sat[h])**beta[h]
 // of the start-of-step moisture (the reference's infiltration fraction,
 // line 219), evaluated for all horizons before the horizon loop.
-template <int NH>
-__device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double thrS, double pet,
+template <int NH, bool JV>
+__device__ __forceinline__ void soil_moisture(double fS, double thrS, double rthrS, double pet,
                                               double evap, double revap, const double *sat,
                                               const double *fr,
-                                              const double *fc, const double *wp, double c1,
+                                              const double *fc, const double *wp,
+                                              const double *rfw, double c1,
                                               double ecan, double peff, const double *frac_pw,
                                               double &runoff_sealed, double &storage_sealed,
                                               double *infil, double *sm, double *aet,
@@ -310,9 +314,9 @@ __device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double
       storage_sealed = tmp;
     }
     if (thrS > kEps) {
-      // revap = RN(1/evap): pet / evap as div_rcp, same bits
-      const double pe = revap != 0.0 ? div_rcp(pet, evap, revap) : pet / evap;
-      aet_sealed = (pe - ecan) * (storage_sealed / thrS);
+      // revap = RN(1/evap): pet / evap as div_rcp, same bits (evap > 0)
+      const double pe = div_rcp(pet, evap, revap);
+      aet_sealed = (pe - ecan) * div_rcp(storage_sealed, thrS, rthrS);
       if (aet_sealed < 0.0) aet_sealed = 0.0;
     } else {
       aet_sealed = DBL_MAX;  // huge(1.0_dp)
@@ -353,8 +357,11 @@ __device__ __forceinline__ void soil_moisture(bool use_jarvis, double fS, double
         if (aet[j] > 0.0) s = s + aet[j];
       a = a - s;
     }
-    const double f = use_jarvis ? jarvis(sm[h], sat[h], wp[h], fr[h], c1)
-                                : feddes(sm[h], fc[h], wp[h], fr[h]);
+    double f;
+    if constexpr (JV)
+      f = jarvis(sm[h], sat[h], wp[h], fr[h], c1);
+    else
+      f = feddes(sm[h], fc[h], wp[h], fr[h], rfw[h]);
     a = a * f;
     if (a < 0.0) a = 0.0;
     if (sm[h] > a) {

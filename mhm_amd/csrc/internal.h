@@ -55,7 +55,7 @@ struct StepDev {
   // monthly namelist factors of this step, day or night already selected
   // (mo_meteo_handler.f90:1111-1117, 1192-1198, 1273-1279; mo_mhm.f90:473)
   double f_pet, f_prec, f_temp, evap;
-  double revap;                 // RN(1 / evap), 0 when evap is 0
+  double revap;                 // RN(1 / evap)
   int32_t day;                  // 0-based forcing column
   int32_t month0;               // 0..11
   int32_t isday;                // 6 < hour <= 18

@@ -44,6 +44,8 @@ template <int NH>
 struct CellRegs {
   double fS, thrS, c1, k0s, k1s, k2s, kps, alpha, karst, thrU, dmaxs, dnps, incr, tth, ddthr;
   double sat[NH], fr[NH], fc[NH], wp[NH], bexp[NH];
+  // correctly rounded reciprocals of the per-cell divisors (div_rcp)
+  double rthrS, rsat[NH], rfw[NH];
   double inter, pack, seal, unsat, satS;
   double sm[NH];
 };
@@ -55,12 +57,12 @@ struct FluxRegs {
   double infil[NH], aet[NH];
 };
 
-template <int NH>
+template <int NH, bool JV>
 __device__ __forceinline__ void load_params(const CellArgs &a, uint32_t bv, CellRegs<NH> &c) {
   auto P = [&](int r) { return ldr(a.P + (int64_t)r * a.ld, bv); };
   c.fS = P(PR_FS);
   c.thrS = P(PR_SEALTHR);
-  c.c1 = a.jarvis ? P(PR_JARVIS) : 0.0;
+  c.c1 = JV ? P(PR_JARVIS) : 0.0;
   c.k0s = P(PR_K0S);
   c.k1s = P(PR_K1S);
   c.k2s = P(PR_K2S);
@@ -82,6 +84,12 @@ __device__ __forceinline__ void load_params(const CellArgs &a, uint32_t bv, Cell
   }
   // snow_accum_melt's branch threshold (mo_snow_accum_melt.f90:127)
   c.ddthr = (c.dmaxs - c.dnps) / c.incr;
+  c.rthrS = 1.0 / c.thrS;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    c.rsat[h] = 1.0 / c.sat[h];
+    c.rfw[h] = 1.0 / (c.fc[h] - c.wp[h]);
+  }
 }
 
 template <int NH>
@@ -184,11 +192,13 @@ __device__ __forceinline__ double runoff_acc(double q, double area, double tst, 
   return div_rcp((0.0 + q * area) * 1000.0, tst, rtst);
 }
 
-// One step of mHM() for one cell after the meteo getters (mo_mhm.f90:455-519)
-template <int NH>
+// One step of mHM() for one cell after the meteo getters (mo_mhm.f90:455-519);
+// JV: Jarvis soil-moisture stress (processCase(3) = 2, 3) instead of Feddes
+template <int NH, bool JV>
 __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, FluxRegs<NH> &x,
-                                           double maxinter, double evap, double revap) {
-  const double ratio = canopy_fill(maxinter, x.prec, c.inter, x.thr);
+                                           double maxinter, double rmaxinter, double evap,
+                                           double revap) {
+  const double ratio = canopy_fill(maxinter, rmaxinter, x.prec, c.inter, x.thr);
   // The canopy's (interc / interc_max)**(2/3) and every horizon's
   // (sm / sat)**beta depend only on values known here: one call evaluates
   // these NH + 1 powers side by side.  Arguments the reference never raises
@@ -198,7 +208,7 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
   px.v[0] = ratio > 0.0 ? ratio : 1.0;
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
-    const double q = c.sm[h] / c.sat[h];
+    const double q = div_rcp(c.sm[h], c.sat[h], c.rsat[h]);
     py.v[h + 1] = c.bexp[h];
     px.v[h + 1] = q > 0.0 ? q : 1.0;
   }
@@ -206,7 +216,8 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
   canopy_evap(x.pet, maxinter, ratio, pw.v[0], c.inter, x.ecan);
   snow_accum_melt(c.incr, c.dmaxs, c.dnps, c.ddthr, x.prec, x.temp, c.tth, x.thr, c.pack, x.ddf,
                   x.melt, x.peff, x.rain, x.snow);
-  soil_moisture<NH>(a.jarvis != 0, c.fS, c.thrS, x.pet, evap, revap, c.sat, c.fr, c.fc, c.wp, c.c1,
+  soil_moisture<NH, JV>(c.fS, c.thrS, c.rthrS, x.pet, evap, revap, c.sat, c.fr, c.fc, c.wp, c.rfw,
+                        c.c1,
                     x.ecan, x.peff, &pw.v[1], x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
   runoff_unsat_zone(c.k1s, c.kps, c.k0s, c.alpha, c.karst, x.infil[NH - 1], c.thrU, c.satS,
                     c.unsat, x.slow, x.fast, x.perc);
@@ -216,7 +227,7 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
 
 // ---------------------------------------------------------------------------
 // Plain path: one launch per model step; every state and flux is written.
-template <int NH>
+template <int NH, bool JV>
 __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, int64_t t) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
@@ -224,7 +235,7 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
   const uint32_t bv = (uint32_t)(blockIdx.y * a.sstride + c) * 8u;
   CellRegs<NH> r;
   FluxRegs<NH> x;
-  load_params<NH>(a, bv, r);
+  load_params<NH, JV>(a, bv, r);
   load_states<NH>(a, bv, r);
   const double *F = a.F + (int64_t)st.day * a.sstride;
   const int64_t fv = a.forcing_days * a.sstride;
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
     for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
   }
   const double maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld, bv);
-  chain_step<NH>(a, r, x, maxinter, st.evap, st.revap);
+  chain_step<NH, JV>(a, r, x, maxinter, 1.0 / maxinter, st.evap, st.revap);
   store_states<NH>(a, bv, r);
   store_fluxes<NH>(a, bv, x);
   if (a.ring) str(a.ring + (t % a.ring_len) * a.ld, bv, runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst));
@@ -253,7 +264,7 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
 // once per month.  Only the routing input leaves the lane each step; states
 // and the last step's fluxes are written at the end.  Per-step fluxes are
 // scoped to the loop body so they do not stay live across steps.
-template <int NH>
+template <int NH, bool JV>
 #ifndef MHMHIP_FUSED_WAVES
 #define MHMHIP_FUSED_WAVES 2
 #endif
@@ -265,12 +276,12 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
   const uint32_t bc = c * 8u;
   const uint32_t bv = (uint32_t)(blockIdx.y * a.sstride + c) * 8u;
   CellRegs<NH> r;
-  load_params<NH>(a, bv, r);
+  load_params<NH, JV>(a, bv, r);
   load_states<NH>(a, bv, r);
   const double area = a.ring ? ldr(a.area, bc) : 0.0;
   const int64_t fv = a.forcing_days * a.sstride;
   int cached_day = -1, cached_doy = -1, cached_month = -1;
-  double pre = 0.0, tavg = 0.0, pd = 0.0, maxinter = 0.0;
+  double pre = 0.0, tavg = 0.0, pd = 0.0, maxinter = 0.0, rmaxinter = 0.0;
   const ConstStep *steps = (const ConstStep *)(a.steps + step0);
   int64_t ring_row = a.ring ? t0 % a.ring_len : 0;
 #pragma nounroll
@@ -288,6 +299,7 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
     }
     if (st.month0 != cached_month) {
       maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld, bv);
+      rmaxinter = 1.0 / maxinter;
       cached_month = st.month0;
     }
     FluxRegs<NH> x;
@@ -298,7 +310,7 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
 #pragma unroll
       for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
     }
-    chain_step<NH>(a, r, x, maxinter, st.evap, st.revap);
+    chain_step<NH, JV>(a, r, x, maxinter, rmaxinter, st.evap, st.revap);
     if (a.ring) {
       str(a.ring + ring_row * a.ld, bv, runoff_acc(x.total, area, a.tst, a.rtst));
       ring_row = ring_row + 1 == a.ring_len ? 0 : ring_row + 1;
@@ -531,7 +543,10 @@ template <int NH>
 static hipError_t step_nh(const CellArgs &a, const StepDev &st, int64_t t, int block, int n_sets,
                           hipStream_t s) {
   dim3 grid((unsigned)((a.n_cells + block - 1) / block), (unsigned)n_sets);
-  hipLaunchKernelGGL(cell_step_kernel<NH>, grid, dim3(block), 0, s, a, st, t);
+  if (a.jarvis)
+    hipLaunchKernelGGL((cell_step_kernel<NH, true>), grid, dim3(block), 0, s, a, st, t);
+  else
+    hipLaunchKernelGGL((cell_step_kernel<NH, false>), grid, dim3(block), 0, s, a, st, t);
   return hipGetLastError();
 }
 
@@ -539,7 +554,12 @@ template <int NH>
 static hipError_t fused_nh(const CellArgs &a, int64_t t0, int nsteps, int step0, int block,
                            int n_sets, hipStream_t s) {
   dim3 grid((unsigned)((a.n_cells + block - 1) / block), (unsigned)n_sets);
-  hipLaunchKernelGGL(cell_fused_kernel<NH>, grid, dim3(block), 0, s, a, t0, nsteps, step0);
+  if (a.jarvis)
+    hipLaunchKernelGGL((cell_fused_kernel<NH, true>), grid, dim3(block), 0, s, a, t0, nsteps,
+                       step0);
+  else
+    hipLaunchKernelGGL((cell_fused_kernel<NH, false>), grid, dim3(block), 0, s, a, t0, nsteps,
+                       step0);
   return hipGetLastError();
 }
 

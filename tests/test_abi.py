@@ -41,6 +41,12 @@ def test_create_rejects_bad_config_without_touching_a_device():
     cfg.n_cells, cfg.n_horizons, cfg.n_sets, cfg.timestep = 10, 3, 1, 5  # 24 % 5 != 0
     cfg.process_soil, cfg.process_pet, cfg.n_forcing_days, cfg.n_domains = 1, 1, 1, 1
     assert L.mhmhip_create(C.byref(cfg), C.byref(ctx)) == N.EINVAL
+    cfg.timestep = 1
+    for m in range(12):
+        cfg.evap_coeff[m] = 1.0
+    cfg.evap_coeff[4] = 0.0  # a zero evaporation coefficient is rejected before any device call
+    assert L.mhmhip_create(C.byref(cfg), C.byref(ctx)) == N.EINVAL
+    assert b"evap_coeff" in L.mhmhip_last_error(None)
 
 
 def test_routing_order_rejects_cycles_and_double_outlinks():
