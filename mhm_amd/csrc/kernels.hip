@@ -483,52 +483,68 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
                      : nullptr;
   const int nd = bi.height + g.B - 1;
   double musk = qtr1 + c1 * (qtin1 - qtr1);
-  int upi[8];  // upstream slots, -1 past the node's count
+  // LDS byte offsets of the upstream slots in parity buffer 0 and 1 (past the
+  // node's upstream count: the +0.0 slot, which leaves the running sum's
+  // value unchanged)
+  uint32_t up0[8], up1[8];
+  const uint32_t zoff = (uint32_t)(2 * W) * 8u;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) upi[c] = c < nc ? c0 + c : -1;
-  for (int d = 0; d < nd; ++d) {
+  for (int c = 0; c < 8; ++c) {
+    up0[c] = c < nc ? (uint32_t)(c0 + c) * 8u : zoff;
+    up1[c] = c < nc ? (uint32_t)(W + c0 + c) * 8u : zoff;
+  }
+  const bool link = (m.a & NF_LINK) != 0, qflag = (m.a & NF_QOUT) != 0, cut = (m.a & NF_CUT) != 0;
+  const bool rare = (m.a & (NF_GAUGE | SF_TROOT)) != 0 || r.qOUT != nullptr;
+  const char *qtr_b = reinterpret_cast<const char *>(qtr);
+  // Diagonal d reads parity buffer (d + 1) & 1, written on diagonal d - 1 by
+  // the upstream slots (one level deeper, one diagonal earlier, same step),
+  // and writes buffer d & 1.  A slot writes every diagonal: an inactive
+  // slot's value is read only by a parent that is inactive for the same step.
+  auto diagonal = [&](int d, const uint32_t *up, int wb) {
     const int j = d - lag;
     const bool on = act && j >= 0 && j < nb;
-    // every LDS read of the diagonal is issued before any of them is used:
-    // the step's own input and all upstream slots (an index past the
-    // node's upstream count reads its own slot, and is not added)
     const int jj = j < 0 ? 0 : (j < g.B ? j : g.B - 1);
     const double qo = qin[jj * W + s];
-    const int par = (jj & 1) * W;
-    double up[8];
+    double v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) up[c] = upi[c] < 0 ? *zero : qtr[par + upi[c]];
-    if (on) {
-      if (proxy) {
-        qtr[(j & 1) * W + s] = qo;
-      } else {
-        // upstream qTR in routing rank (D8: at most 8; the header allows 15);
-        // past the node's count the terms are the +0.0 slot, which leaves the
-        // running sum's value unchanged
-        double sum = 0.0;
+    for (int c = 0; c < 8; ++c) v[c] = *reinterpret_cast<const double *>(qtr_b + up[c]);
+    // upstream qTR in routing rank (D8: at most 8; the header allows 15)
+    double sum = 0.0;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) sum = sum + up[c];
-        for (int c = 8; c < nc; ++c) sum = sum + qtr[par + c0 + c];
-        if (m.a & NF_QOUT) sum = sum + qo;
-        if (m.a & NF_LINK) {
-          // musk = qTR(IT1) + C1 (qTIN(IT1) - qTR(IT1)), formed before the barrier
-          double q = musk + c2 * (sum - qtin1);
-          if (m.a & NF_CUT) q = 0.0;
-          qtr[(j & 1) * W + s] = q;
-          qtr1 = q;
-          if (bout) bout[j] = q;
-        }
-        qtin1 = sum;
-        musk = qtr1 + c1 * (qtin1 - qtr1);
-        const int64_t t = t0 + j;
-        if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qo);
-        if (m.a & NF_GAUGE) {
-          for (int32_t u = 0; u < r.n_gauges; ++u)
-            if (r.gauge_node[u] == m.pos)
-              r.gauge_out[((t - r.t_base) * r.n_sets + set) * r.n_gauges + u] = sum;
-        }
+    for (int c = 0; c < 8; ++c) sum = sum + v[c];
+    if (nc > 8) {
+      const double *rb = qtr + (wb ^ 1) * W;
+      for (int c = 8; c < nc; ++c) sum = sum + rb[c0 + c];
+    }
+    if (qflag) sum = sum + qo;
+    // musk = qTR(IT1) + C1 (qTIN(IT1) - qTR(IT1)), formed on the diagonal before
+    double q = musk + c2 * (sum - qtin1);
+    if (cut) q = 0.0;
+    qtr[wb * W + s] = proxy ? qo : q;
+    const bool upd = on && real;
+    if (upd && link) qtr1 = q;
+    if (upd) qtin1 = sum;
+    musk = qtr1 + c1 * (qtin1 - qtr1);
+    if (upd && rare) {
+      if (bout) bout[j] = q;
+      const int64_t t = t0 + j;
+      if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qo);
+      if (m.a & NF_GAUGE) {
+        for (int32_t u = 0; u < r.n_gauges; ++u)
+          if (r.gauge_node[u] == m.pos)
+            r.gauge_out[((t - r.t_base) * r.n_sets + set) * r.n_gauges + u] = sum;
       }
     }
+  };
+  int d = 0;
+  for (; d + 1 < nd; d += 2) {
+    diagonal(d, up1, 0);
+    lds_barrier();
+    diagonal(d + 1, up0, 1);
+    lds_barrier();
+  }
+  if (d < nd) {
+    diagonal(d, up1, 0);
     lds_barrier();
   }
   if (act && real) {
