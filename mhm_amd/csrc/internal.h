@@ -121,7 +121,7 @@ constexpr int32_t NF_MASK = NF_LINK | NF_QOUT | NF_CUT | NF_GAUGE | NF_INFLOW;
 #define MHMHIP_TILE_SLOTS 256
 #endif
 #ifndef MHMHIP_TILE_HEIGHT
-#define MHMHIP_TILE_HEIGHT 32
+#define MHMHIP_TILE_HEIGHT 16
 #endif
 #ifndef MHMHIP_TILE_WINDOW
 #define MHMHIP_TILE_WINDOW 24

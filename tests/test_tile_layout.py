@@ -10,7 +10,7 @@ invariant it relies on is checked here:
     itself (same tile, one level deeper: lag - 1) or a proxy for it;
   * every proxy has exactly one tile root with its boundary row, whose tile
     lies one tile-level deeper; domain sinks head depth-0 tiles;
-  * tiles stay within 256 slots and 32 levels (lag 0..31);
+  * tiles stay within 256 slots and 16 levels (lag 0..15);
   * the tile depth grows by one per tile boundary along every path.
 Networks: synthetic D8 basins and hand-made edge cases (single node, a
 chain longer than the height limit, a 15-child star, several outlets).
@@ -20,7 +20,7 @@ import pytest
 
 from mhm_amd import _native as N, synth
 
-SLOTS, HEIGHT = 256, 32
+SLOTS, HEIGHT = 256, 16
 SF_REAL, SF_PROXY, SF_TROOT = 1 << 16, 1 << 17, 1 << 18
 
 
