@@ -308,6 +308,8 @@ def main():
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--days", type=int, default=90)
     ap.add_argument("--steps-per-launch", type=int, default=24)
+    ap.add_argument("--route-window", type=int, default=16,
+                    help="routing window of the headline loop (ring memory grows with it)")
     ap.add_argument("--no-graph", action="store_true", help="headline loop without hipGraph")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
@@ -347,7 +349,8 @@ def main():
     log(f"setup {time.perf_counter() - t0:.1f}s: cells={dom.n_cells} dmax={dom.dmax} "
         f"steps={n_steps} device_bytes={model.device_bytes / 2**30:.2f} GiB")
     S = args.steps_per_launch
-    kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=not args.no_graph)
+    kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=not args.no_graph,
+              route_window=args.route_window)
     # profiling on before the warm-up so a captured graph holds its event nodes
     model.set_profiling(True, ROUTE_STRIDE)
     for _ in range(args.warmup):
@@ -360,6 +363,7 @@ def main():
         log(f"bench step {k + 1}/{args.steps} done")
     model.sync()
     elapsed = time.perf_counter() - t0
+    device_gib = model.device_bytes / 2**30
     prof = model.profile()
     model.set_profiling(False)
     value = float(dom.n_cells) * n_steps * args.steps / elapsed
@@ -415,7 +419,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"single basin {args.grid}x{args.grid} L1 cells, 3 horizons, "
                                f"hourly x {n_steps} steps, D8 routing (max depth {dom.dmax}), "
-                               f"fused S={S}, {'eager' if args.no_graph else 'hipGraph'} time loop",
+                               f"fused S={S}, tiled routing window {args.route_window}, "
+                               f"{'eager' if args.no_graph else 'hipGraph'} time loop",
+                   "device_GiB": round(device_gib, 2),
                    "cells": dom.n_cells, "time_steps": n_steps, "routing_levels": dom.dmax + 1,
                    "parallelism": "single GPU"},
         "roofline": roof,

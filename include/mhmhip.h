@@ -168,7 +168,9 @@ typedef struct {
   int32_t use_graph;     /* capture the tile sequence in a hipGraph              */
   int32_t route_mode;    /* 0 auto (= 3), 1 grid diagonals, 2 workgroup-resident
                             trees, 3 tiles (windows of up to 24 steps per launch) */
-  int32_t reserved[3];
+  int32_t route_window;  /* tiled routing: steps per window, 1..24 (0 = 8); the
+                            runoff ring holds (tile depth + 1) windows + S steps */
+  int32_t reserved[2];
 } mhmhip_run_opts;
 
 #define MHMHIP_RUN_PLAIN 0 /* one launch per step, every flux written each step */

@@ -67,7 +67,7 @@ class Network(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("steps_per_launch", C.c_int32), ("block", C.c_int32),
                 ("use_graph", C.c_int32), ("route_mode", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("route_window", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 # every entry point declared in include/mhmhip.h

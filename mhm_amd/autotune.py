@@ -11,12 +11,15 @@ import time
 from . import _native as N
 
 
-def candidates(model, S_values=(8, 24, 72, 168), blocks=(64, 128, 256), graphs=(False, True)):
-    """Fused-path settings worth trying for this model."""
+def candidates(model, S_values=(8, 24, 72, 168), blocks=(64, 128, 256), graphs=(False, True),
+               windows=(8, 24)):
+    """Fused-path settings worth trying for this model (the routing window
+    only matters for the tiled routing)."""
     route_modes = [3, 1, 2] if _small_trees(model) else [3, 1]
     for S, b, rm, g in itertools.product(S_values, blocks, route_modes, graphs):
-        yield {"mode": N.RUN_FUSED, "steps_per_launch": S, "block": b, "route_mode": rm,
-               "use_graph": g}
+        for w in (windows if rm == 3 else (0,)):
+            yield {"mode": N.RUN_FUSED, "steps_per_launch": S, "block": b, "route_mode": rm,
+                   "use_graph": g, "route_window": w}
 
 
 def _small_trees(model):

@@ -670,7 +670,14 @@ int route_kind(int route_mode) {
 }
 
 // Steps per routing window of the tiled kernel for S steps per cell launch.
-int32_t tile_window(int S) { return std::max(1, std::min(S, kTileWindowMax)); }
+// Steps per routing window: the requested one (0: 8), at most S of the fused
+// path and kTileWindowMax.  The ring spans (tau_max + 1) windows, so smaller
+// windows cost launches and save memory.
+constexpr int32_t kTileWindowDefault = 8;
+int32_t tile_window(int S, int32_t req) {
+  const int32_t b = req > 0 ? req : kTileWindowDefault;
+  return std::max(1, std::min(std::min(b, (int32_t)S), (int32_t)kTileWindowMax));
+}
 
 // Ring rows the routing of a call needs: the pipelined diagonals read a
 // step up to dmax diagonals after its cell launch, the tiles up to tau_max
@@ -1215,6 +1222,8 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     return x->fail(MHMHIP_EINVAL, "run: block must be 64, 128 or 256");
   if (o.route_mode < 0 || o.route_mode > 3) return x->fail(MHMHIP_EINVAL, "run: bad route_mode");
   if (o.steps_per_launch < 0) return x->fail(MHMHIP_EINVAL, "run: bad steps_per_launch");
+  if (o.route_window < 0 || o.route_window > kTileWindowMax)
+    return x->fail(MHMHIP_EINVAL, "run: route_window must be 0..24");
   const bool route = routing_on(x);
   if (route && x->t_route != x->t_cell)
     return x->fail(MHMHIP_ESTATE, "run: previous step not routed (call mrm_step)");
@@ -1231,7 +1240,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   const int kind = route_kind(o.route_mode);
   const bool wg = kind == MHMHIP_K_ROUTE_WG;
   // tiled routing: windows of B steps (the plain path routes in windows too)
-  const int32_t B = tile_window(o.mode == MHMHIP_RUN_PLAIN ? kTileWindowMax : S);
+  const int32_t B = tile_window(o.mode == MHMHIP_RUN_PLAIN ? kTileWindowMax : S, o.route_window);
   CHECK_RC(ensure_steps(x, nsteps));
   CHECK_HIP(x, hipMemcpyAsync(x->d_steps, hs.data(), sizeof(StepDev) * nsteps,
                               hipMemcpyHostToDevice, x->stream),

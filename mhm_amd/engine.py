@@ -197,7 +197,7 @@ class Model:
         return g[:, :self.n_gauges]
 
     def run(self, steps, mode=N.RUN_FUSED, steps_per_launch=0, block=0, route_mode=0,
-            want_gauges=True, use_graph=False):
+            want_gauges=True, use_graph=False, route_window=0):
         """steps: ctypes array of mhmhip_step.  Returns gauges (n_steps, n_sets, n_gauges)."""
         n = len(steps)
         if not isinstance(steps, C.Array):
@@ -208,6 +208,7 @@ class Model:
         opts.block = block
         opts.route_mode = route_mode
         opts.use_graph = 1 if use_graph else 0
+        opts.route_window = route_window
         g = np.zeros((n, self.n_sets, max(self.n_gauges, 1))) if want_gauges else None
         self._chk(N.lib().mhmhip_run(self._ctx, steps, n, C.byref(opts), N.dptr(g)))
         return None if g is None else g[:, :, :self.n_gauges]

@@ -53,7 +53,7 @@ module mo_mhmhip
 
   type, bind(C), public :: mhmhip_run_opts
     integer(c_int32_t) :: mode = 1, steps_per_launch = 0, block = 0, use_graph = 0, &
-                          route_mode = 0, reserved(3) = 0
+                          route_mode = 0, route_window = 0, reserved(2) = 0
   end type
 
   public :: mhmhip_create, mhmhip_destroy, mhmhip_last_error, mhmhip_device_bytes, mhmhip_sync, &

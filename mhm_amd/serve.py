@@ -65,7 +65,7 @@ def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_step
     ld = sstride * n_sets
     rows = (19 + 5 * n_horizons + 24) + (5 + n_horizons) + (17 + 2 * n_horizons)
     cells = rows * ld * 8 + 5 * n_forcing_days * sstride * 8
-    window = max(1, min(chunk_steps, 24))
+    window = max(1, min(chunk_steps, 8))  # default routing window (mhmhip_run_opts)
     ring = ((plan.tau_max + 1) * window + chunk_steps + 1) * ld * 8
     routing = (16 * n_cells + 6 * ld * 8 + 20 + 4 * max(1, n_cells - n_outlets)
                + 16 * plan.n_slots + 16 * plan.n_bins + 2 * n_sets * max(1, plan.n_bnd) * 24 * 8
