@@ -43,7 +43,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_per_unit(kernel, steps_per_launch, steps_per_day):
+def bytes_per_unit(kernel, steps_per_launch, steps_per_day, window=16):
     """Algorithmic HBM bytes per work unit (cell-step or node-step)."""
     if kernel == "cell_step":
         # forcing 32 + PET params 24 + mHM params 232 + states 64 r/64 w + 23 fluxes 184 w
@@ -53,6 +53,10 @@ def bytes_per_unit(kernel, steps_per_launch, steps_per_day):
         # once per launch: params 256 + states 64 read, states 64 + 23 fluxes 184
         # written (568/S)
         return 32.0 / steps_per_day + 8.0 + 568.0 / steps_per_launch
+    if kernel == "route_tile":
+        # node-step: qOUT from the ring 8; once per window of B steps: slot metadata
+        # 16, C1 + C2 16, qTIN read + write 16, qTR read + write 16
+        return 8.0 + 64.0 / window
     # routing node-step: qOUT from the ring 8, C1 + C2 16, own qTIN read + write 16,
     # own qTR(IT1) read 8 + qTR(IT2) write 8, upstream qTR read 8 (one per edge),
     # node metadata (child index + packed depth/flags) 8
@@ -66,11 +70,11 @@ def load_json(path):
     return {}
 
 
-def kernel_table(prof, steps_per_launch, steps_per_day):
+def kernel_table(prof, steps_per_launch, steps_per_day, window=8):
     detail = {}
     for k, (kms, kn, ku) in prof.items():
         if kn:
-            b = bytes_per_unit(k, steps_per_launch, steps_per_day)
+            b = bytes_per_unit(k, steps_per_launch, steps_per_day, window)
             detail[k] = {"ms": round(kms, 3), "launches": kn, "units": ku,
                          "avg_us": round(kms * 1e3 / kn, 3),
                          "bytes_per_unit": round(b, 3),
@@ -367,7 +371,7 @@ def main():
     prof = model.profile()
     model.set_profiling(False)
     value = float(dom.n_cells) * n_steps * args.steps / elapsed
-    kernels = kernel_table(prof, S, 24)
+    kernels = kernel_table(prof, S, 24, min(args.route_window, S))
     dominant = max(prof.items(), key=lambda kv: kv[1][0])[0]
     hbm = hbm_roofline(dominant, prof[dominant], S, 24, traffic)
     fpu = flops.get(dominant, {}).get("flops_per_unit")

@@ -10,7 +10,9 @@ START = dt.datetime(1990, 1, 1)
 
 
 def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, route=True,
-              forcing_seed=100, pet_forcing=False, start=START):
+              forcing_seed=100, pet_forcing=False, start=START, inflow=None):
+    """HIP model and oracle on the same inputs.  inflow: {"domain", "node" (1-based),
+    "headwater", "q" (n_steps, n_inflow)} inflow gauges, given to both."""
     n_days = calendar.n_forcing_days_needed(n_steps)
     doy0 = start.timetuple().tm_yday
     for i, d in enumerate(domains):
@@ -24,7 +26,15 @@ def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, 
     sim = OracleSim(domains, n_sets=n_sets, timestep=timestep, process_soil=soil, process_pet=pet,
                     n_forcing_days=n_days, route=route)
     if route:
-        model.bind_network()
+        model.bind_network(inflow=inflow)
+        if inflow is not None:
+            model.set_inflow(inflow["q"])
+            for i, d in enumerate(domains):
+                sel = np.asarray(inflow["domain"]) == i
+                if sel.any():
+                    d._oracle_inflow = {"node": np.asarray(inflow["node"])[sel],
+                                        "headwater": np.asarray(inflow["headwater"])[sel],
+                                        "q": np.asarray(inflow["q"])[:, sel], "tt0": 0}
     rng = np.random.default_rng(seed)
     gam = PM.draw(rng, n_sets * len(domains)).reshape(n_sets, len(domains), -1)
     for s in range(n_sets):
