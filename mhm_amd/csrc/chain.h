@@ -114,18 +114,17 @@ __host__ __device__ inline DV<N> log_pos_v(const DV<N> &x) {
   return out;
 }
 
-// exp(z) for finite z, N values in lockstep: z = k ln2 + r with
-// |r| <= ln2/2, exp(r) by its Taylor series through r^13 (truncation below
-// 5e-18 relative), scaled by 2^k.  z is clamped to [-1100, 710] first, which
-// gives 0 and +inf beyond the range.
+// exp(z) for |z| < 1.4e6 (k ln2_hi exact for |k| < 2^21), N values in
+// lockstep: z = k ln2 + r with |r| <= ln2/2, exp(r) by its Taylor series
+// through r^13 (truncation below 5e-18 relative), scaled by 2^k; ldexp
+// gives 0 and +inf beyond the double range.  The chain's arguments
+// y log x stay within |z| < 1e4 (x >= 5e-324, y <= 10).
 template <int N>
 __host__ __device__ inline DV<N> exp_fin_v(const DV<N> &zin) {
   DV<N> k, r, p, out;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    double z = zin.v[i];
-    if (z > 710.0) z = 710.0;
-    if (z < -1100.0) z = -1100.0;
+    const double z = zin.v[i];
     k.v[i] = rint(z * kInvLn2);
     r.v[i] = fma(-k.v[i], kLn2Lo, fma(-k.v[i], kLn2Hi, z));
     p.v[i] = 1.0 / 6227020800.0;  // 1/13!
