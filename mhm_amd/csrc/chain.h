@@ -11,6 +11,8 @@
 #include <float.h>
 #include <stdint.h>
 
+#include "logtab.h"
+
 namespace mhmhip {
 
 // src/common/mo_common_constants.f90: eps_dp = epsilon(1.0_dp)
@@ -67,21 +69,45 @@ struct DV {
   double v[N];
 };
 
-// Natural log of finite x > 0 (normal or subnormal), N values in lockstep.
-// Reduction x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1,
-// s = f / (2 + f), and
-//   log(1 + f) = 2 atanh(s) = 2s + s R(s^2),  R(z) = sum_k 2 z^k / (2k + 1),
-// arranged as f - (f^2/2 - s (f^2/2 + R)) so that the rounding of s only
-// enters the small correction term (the scheme of fdlibm's e_log.c).
-// |s| <= 0.1716 -> z <= 0.0295; the Taylor terms through z^9 leave a
-// truncation error below 3e-17 relative.  Every operation is an IEEE
-// +, -, *, /, fma or an exact frexp, so host and device builds agree bit for
-// bit.  Each step runs over all N values before the next one, which gives
-// the scheduler N independent dependency chains to interleave.
+// Natural log of finite x > 0 (normal or subnormal), N values in lockstep,
+// table-driven (the method of Tang, ACM TOMS 16 (1990) 378-400): x = m 2^e
+// with m in [sqrt(1/2), sqrt(2)); F = j/128 the nearest table point, so
+// f = m - F is exact (|f| <= 1/256); u = f RN(1/F) and
+//   log x = e ln2 + log F + log1p(u),
+// log1p(u) = u + u^2 (-1/2 + u/3 - ... + u^5/7) (truncation below 1e-17
+// relative for |u| < 0.0056); within 1.25 ulp (near x = 1 the table term and
+// log1p(u) partly cancel).  log F comes from logtab.h as hi + lo with hi a
+// multiple of 2^-42, so e ln2_hi + hi is exact.  Every operation is an IEEE
+// +, -, *, fma, rint or an exact frexp, and the table holds the same numbers
+// on host and device (LDS, filled by logtab_fill), so the two builds agree
+// bit for bit.  Each step runs over all N values before the next one, which
+// gives the scheduler N independent dependency chains to interleave.
+#if defined(__HIP_DEVICE_COMPILE__)
+__shared__ double s_logtab[kLogTabN * 3];
+#endif
+
+// Every workgroup of a kernel that evaluates a power calls this first (all
+// threads, before any of them returns).
+__device__ inline void logtab_fill() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int i = threadIdx.x; i < kLogTabN * 3; i += blockDim.x) s_logtab[i] = kLogTab[i];
+  __syncthreads();
+#endif
+}
+
+__host__ __device__ inline const double *logtab() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return s_logtab;
+#else
+  return kLogTab;
+#endif
+}
+
 template <int N>
 __host__ __device__ inline DV<N> log_pos_v(const DV<N> &x) {
-  DV<N> f, s, z, r, out;
+  DV<N> u, hi, lo, out;
   int e[N];
+  const double *tab = logtab();
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     double m = frexp(x.v[i], &e[i]);  // [0.5, 1)
@@ -89,27 +115,26 @@ __host__ __device__ inline DV<N> log_pos_v(const DV<N> &x) {
       m = m + m;
       e[i] = e[i] - 1;
     }
-    f.v[i] = m - 1.0;  // exact
+    const double jd = rint(m * 128.0);  // 90..181
+    const double f = m - jd * 0.0078125;  // exact
+    const double *t = tab + ((int)jd - kLogTabJ0) * 3;
+    u.v[i] = f * t[0];
+    hi.v[i] = t[1];
+    lo.v[i] = t[2];
   }
+  DV<N> p;
 #pragma unroll
-  for (int i = 0; i < N; ++i) s.v[i] = f.v[i] / (2.0 + f.v[i]);
+  for (int i = 0; i < N; ++i) p.v[i] = 1.0 / 7.0;
+  constexpr double kP[5] = {-1.0 / 6.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0, -1.0 / 2.0};
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int i = 0; i < N; ++i) p.v[i] = fma_k(p.v[i], u.v[i], kP[k]);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    z.v[i] = s.v[i] * s.v[i];
-    r.v[i] = 2.0 / 19.0;
-  }
-  constexpr double kR[8] = {2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0, 2.0 / 11.0,
-                            2.0 / 9.0,  2.0 / 7.0,  2.0 / 5.0,  2.0 / 3.0};
-#pragma unroll
-  for (int k = 0; k < 8; ++k)
-#pragma unroll
-    for (int i = 0; i < N; ++i) r.v[i] = fma_k(r.v[i], z.v[i], kR[k]);
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const double rz = r.v[i] * z.v[i];
-    const double hfsq = 0.5 * (f.v[i] * f.v[i]);
+    const double poly = fma(u.v[i] * u.v[i], p.v[i], u.v[i]);
     const double de = (double)e[i];
-    out.v[i] = fma(de, kLn2Hi, f.v[i] - (hfsq - fma(s.v[i], hfsq + rz, de * kLn2Lo)));
+    out.v[i] = fma(de, kLn2Hi, hi.v[i]) + (poly + fma(de, kLn2Lo, lo.v[i]));
   }
   return out;
 }

@@ -4,7 +4,8 @@ The cell kernels evaluate the reference's x**y (mo_canopy_interc.f90:121,
 mo_soil_moisture.f90:219, mo_runoff.f90:134) as exp(y log x) with a log and
 an exp written from their series (no library calls), so the same source
 compiles for the host and for gfx950.  Pinned here on the host build:
-  * against mpmath at 200 bits: log and exp within 1 ulp, pow within
+  * against mpmath at 200 bits: log within 1.25 ulp (the table method's
+    near-1 cancellation reaches 1.0002), exp within 1 ulp, pow within
     (2 + 3 |y log x|) ulp (log's 1 ulp and the rounding of y log x, amplified
     by exp), over the ranges the chain feeds them and beyond;
   * closed forms: log(2^k) = k ln 2, log(1) = 0, exp(0) = 1, exp(1) = e,
@@ -59,7 +60,7 @@ def _ulp_err(got, exact):
 mp.mp.prec = 200
 
 
-def test_log_within_one_ulp(lib):
+def test_log_within_a_quarter_more_than_one_ulp(lib):
     rng = np.random.default_rng(1)
     x = np.concatenate([np.exp(rng.uniform(-700, 700, 3000)),
                         1.0 + rng.uniform(-0.3, 0.45, 3000),          # the reduction's edges
@@ -68,7 +69,7 @@ def test_log_within_one_ulp(lib):
                                   1.0 - 2**-53, 1.0 + 2**-52, 5e-324, 2.2250738585072014e-308])])
     got = _eval(lib, 0, x)
     worst = max(_ulp_err(g, mp.log(mp.mpf(v))) for g, v in zip(got, x) if v != 1.0)
-    assert worst <= 1.0, worst
+    assert worst <= 1.25, worst
 
 
 def test_exp_within_one_ulp(lib):
@@ -144,21 +145,28 @@ def _fma(a, b, c):
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
-def _log_copy(x, terms=9):
+def _logtab():
+    import re
+    path = os.path.join(ROOT, "mhm_amd", "csrc", "logtab.h")
+    vals = [float.fromhex(v) for v in re.findall(r"-?0x[0-9a-fp.+-]+", open(path).read())]
+    return [vals[3 * k:3 * k + 3] for k in range(len(vals) // 3)]
+
+
+def _log_copy(x, degree=7):
     """log_pos restated in Python (every operation correctly rounded)."""
+    tab = _logtab()
     m, e = math.frexp(x)
     if m < 0.7071067811865476:
         m, e = m + m, e - 1
-    f = m - 1.0
-    s = f / (2.0 + f)
-    z = s * s
-    r = 2.0 / (2 * terms + 1)
-    for k in range(terms - 1, 0, -1):
-        r = _fma(r, z, 2.0 / (2 * k + 1))
-    r = r * z
-    hfsq = 0.5 * (f * f)
-    return _fma(float(e), LN2_HI,
-                f - (hfsq - _fma(s, hfsq + r, float(e) * LN2_LO)))
+    jd = float(round(m * 128.0))
+    f = m - jd * 0.0078125
+    inv, hi, lo = tab[int(jd) - 90]
+    u = f * inv
+    p = 1.0 / degree * (1 if degree % 2 else -1)
+    for k in range(degree - 1, 1, -1):
+        p = _fma(p, u, (1.0 if k % 2 else -1.0) / k)
+    poly = _fma(u * u, p, u)
+    return _fma(float(e), LN2_HI, hi) + (poly + _fma(float(e), LN2_LO, lo))
 
 
 def _exp_copy(z, top=13):
@@ -172,12 +180,12 @@ def _exp_copy(z, top=13):
 
 
 def test_copies_match_the_library_and_planted_mistakes_fail(lib):
-    xs = [0.7071067811865476, 1.4142135623730949, 1.41, 0.7072, 3.3e-7, 812.5]
+    xs = [0.7071067811865476, 1.4142135623730949, 1.41, 0.7072, 3.3e-7, 812.5, 0.70718, 1.0 + 1.0 / 256]
     zs = [0.3465, -0.3465, 5.0 + 0.3465, -7.0 - 0.3465]
     assert [_log_copy(x) for x in xs] == list(_eval(lib, 0, xs))
     assert [_exp_copy(z) for z in zs] == list(_eval(lib, 1, zs))
-    # a dropped term: R(z) through z^8, exp through r^11
-    assert max(_ulp_err(_log_copy(x, terms=8), mp.log(mp.mpf(x))) for x in xs) > 1.0
+    # a dropped term: log1p through u^6, exp through r^11
+    assert max(_ulp_err(_log_copy(x, degree=6), mp.log(mp.mpf(x))) for x in xs) > 1.25
     assert max(_ulp_err(_exp_copy(z, top=11), mp.exp(mp.mpf(z))) for z in zs) > 1.0
 
 
