@@ -221,16 +221,19 @@ __device__ __forceinline__ bool le_eps(double a, double b) {
 // throughfall and returns the ratio (1.0 when interc_max <= eps, where the
 // reference takes no power); canopy_evap finishes with the ratio's power.
 // (rmax = RN(1/interc_max): the ratio as div_rcp, same bits)
+// Several of the reference's two-way clamps are a minimum or a maximum:
+//   if (a >= b) then d = a - b; s = b else d = 0; s = a   ->  d = max(a - b, 0), s = min(a, b)
+//   if (s > e) then s = s - e else e = s; s = 0          ->  e = min(e, s), s = s - e
+// They give the same value for every non-NaN operand (a zero may come out
+// with the other sign, which no later operation of the chain tells apart).
+__device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
+
 __device__ __forceinline__ double canopy_fill(double interc_max, double rmax, double precip,
                                               double &interc, double &throughfall) {
   const double aux = interc + precip;
-  if (aux >= interc_max) {
-    throughfall = aux - interc_max;
-    interc = interc_max;
-  } else {
-    throughfall = 0.0;
-    interc = aux;
-  }
+  throughfall = vmax(aux - interc_max, 0.0);
+  interc = vmin(aux, interc_max);
   return interc_max > kEps ? div_rcp(interc, interc_max, rmax) : 1.0;
 }
 
@@ -241,13 +244,8 @@ __device__ __forceinline__ void canopy_evap(double pet, double interc_max, doubl
     evap_canopy = pet * pow_pos_from(ratio, pw);
   else
     evap_canopy = 0.0;
-  if (evap_canopy < 0.0) evap_canopy = 0.0;
-  if (interc > evap_canopy) {
-    interc = interc - evap_canopy;
-  } else {
-    evap_canopy = interc;
-    interc = 0.0;
-  }
+  evap_canopy = vmin(vmax(evap_canopy, 0.0), interc);
+  interc = interc - evap_canopy;
 }
 
 // src/mHM/mo_snow_accum_melt.f90:70-158.  ddthr = (deg_day_max - deg_day_noprec)
@@ -259,27 +257,19 @@ __device__ __forceinline__ void snow_accum_melt(double incr, double dmax, double
                                                 double &deg_day, double &melt,
                                                 double &prec_effect, double &rain,
                                                 double &snow) {
-  if (temp > tth) {
-    snow = 0.0;
-    rain = thr;
-  } else {
-    snow = thr;
-    rain = 0.0;
-  }
+  const bool warm = temp > tth;
+  snow = warm ? 0.0 : thr;
+  rain = thr - snow;  // thr (warm) or +0 (cold), as the reference's branches
   if (prec <= ddthr)
     deg_day = dnp + incr * prec;
   else
     deg_day = dmax;
-  if (temp > tth) {
+  // warm with a pack: melt = aux unless aux > pack (then the whole pack)
+  const double m = vmin(deg_day * (temp - tth), pack);
+  if (warm) {
     if (pack > 0.0) {
-      const double aux = deg_day * (temp - tth);
-      if (aux > pack) {
-        melt = pack;
-        pack = 0.0;
-      } else {
-        melt = aux;
-        pack = pack - aux;
-      }
+      melt = m;
+      pack = pack - m;
     } else {
       melt = 0.0;
       pack = 0.0;
@@ -330,27 +320,17 @@ __device__ __forceinline__ void soil_moisture(double fS, double thrS, double rth
   aet_sealed = 0.0;
   if (fS > 0.0) {
     const double tmp = storage_sealed + peff;
-    if (tmp > thrS) {
-      runoff_sealed = tmp - thrS;
-      storage_sealed = thrS;
-    } else {
-      runoff_sealed = 0.0;
-      storage_sealed = tmp;
-    }
+    runoff_sealed = vmax(tmp - thrS, 0.0);
+    storage_sealed = vmin(tmp, thrS);
     if (thrS > kEps) {
       // revap = RN(1/evap): pet / evap as div_rcp, same bits (evap > 0)
       const double pe = div_rcp(pet, evap, revap);
-      aet_sealed = (pe - ecan) * div_rcp(storage_sealed, thrS, rthrS);
-      if (aet_sealed < 0.0) aet_sealed = 0.0;
+      aet_sealed = vmax((pe - ecan) * div_rcp(storage_sealed, thrS, rthrS), 0.0);
     } else {
       aet_sealed = DBL_MAX;  // huge(1.0_dp)
     }
-    if (storage_sealed > aet_sealed) {
-      storage_sealed = storage_sealed - aet_sealed;
-    } else {
-      aet_sealed = storage_sealed;
-      storage_sealed = 0.0;
-    }
+    aet_sealed = vmin(aet_sealed, storage_sealed);
+    storage_sealed = storage_sealed - aet_sealed;
   }
   double pin = peff;
 #pragma unroll
@@ -377,8 +357,7 @@ __device__ __forceinline__ void soil_moisture(double fS, double thrS, double rth
     if (h != 0) {
       double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < h; ++j)
-        if (aet[j] > 0.0) s = s + aet[j];
+      for (int j = 0; j < h; ++j) s = s + vmax(aet[j], 0.0);  // adds only the positive ones
       a = a - s;
     }
     double f;
@@ -386,15 +365,14 @@ __device__ __forceinline__ void soil_moisture(double fS, double thrS, double rth
       f = jarvis(sm[h], sat[h], wp[h], fr[h], c1);
     else
       f = feddes(sm[h], fc[h], wp[h], fr[h], rfw[h]);
-    a = a * f;
-    if (a < 0.0) a = 0.0;
+    a = vmax(a * f, 0.0);
     if (sm[h] > a) {
       sm[h] = sm[h] - a;
     } else {
       a = sm[h] - kEps;
       sm[h] = kEps;
     }
-    if (sm[h] < kEps) sm[h] = kEps;
+    sm[h] = vmax(sm[h], kEps);
     aet[h] = a;
   }
 }
@@ -406,36 +384,22 @@ __device__ __forceinline__ void runoff_unsat_zone(double k1, double kp, double k
                                                   double &fast, double &perc) {
   unsat = unsat + pin;
   fast = 0.0;
-  if (unsat > thr) {
-    const double a = k0 * (unsat - thr), b = unsat - kEps;
-    fast = (b < a) ? b : a;
-  }
+  if (unsat > thr) fast = vmin(unsat - kEps, k0 * (unsat - thr));
   unsat = unsat - fast;
   slow = 0.0;
-  if (unsat > kEps) {
-    const double a = k1 * pow_pos(unsat, 1.0 + alpha), b = unsat - kEps;
-    slow = (b < a) ? b : a;
-  }
+  if (unsat > kEps) slow = vmin(unsat - kEps, k1 * pow_pos(unsat, 1.0 + alpha));
   unsat = unsat - slow;
   perc = kp * unsat;
-  if (unsat > perc) {
-    unsat = unsat - perc;
-    sat = sat + perc * karst;
-  } else {
-    sat = sat + unsat * karst;
-    unsat = 0.0;
-  }
+  const double p = vmin(perc, unsat);  // the whole store when perc exceeds it
+  sat = sat + p * karst;
+  unsat = unsat - p;
 }
 
 // src/mHM/mo_runoff.f90:191-212
 __device__ __forceinline__ void runoff_sat_zone(double k2, double &sat, double &baseflow) {
-  if (sat > 0.0) {
-    baseflow = k2 * sat;
-    sat = sat - baseflow;
-  } else {
-    baseflow = 0.0;
-    sat = 0.0;
-  }
+  const double s0 = vmax(sat, 0.0);  // an empty or negative store drains nothing, ends at 0
+  baseflow = k2 * s0;
+  sat = s0 - baseflow;
 }
 
 // src/mHM/mo_runoff.f90:248-274
