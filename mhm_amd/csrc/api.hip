@@ -1223,7 +1223,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   if (o.route_mode < 0 || o.route_mode > 3) return x->fail(MHMHIP_EINVAL, "run: bad route_mode");
   if (o.steps_per_launch < 0) return x->fail(MHMHIP_EINVAL, "run: bad steps_per_launch");
   if (o.route_window < 0 || o.route_window > kTileWindowMax)
-    return x->fail(MHMHIP_EINVAL, "run: route_window must be 0..24");
+    return x->fail(MHMHIP_EINVAL, "run: route_window must be 0.." + std::to_string(kTileWindowMax));
   const bool route = routing_on(x);
   if (route && x->t_route != x->t_cell)
     return x->fail(MHMHIP_ESTATE, "run: previous step not routed (call mrm_step)");

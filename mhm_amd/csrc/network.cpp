@@ -86,10 +86,10 @@ namespace mhmhip {
 // their qTR sum runs in the order of L11_routing's accumulation.
 //
 // Tile partition, bottom-up (reverse breadth-first order): a node's open
-// tile takes its upstream nodes' open tiles while it stays within
-// kTileSlots slots and kTileHeight levels; otherwise the largest upstream
-// open tile (by slots, or by levels when the height is exceeded) is closed
-// -- it becomes a tile of its own and a proxy slot in the node's tile.
+// tile takes upstream open tiles within kTileSlots slots and kTileHeight
+// levels; an upstream open tile that is closed becomes a tile of its own
+// and a proxy slot in the node's tile.  Which ones are closed is chosen to
+// keep the longest chain of tiles short (the rule is at the partition).
 // Tiles are numbered breadth-first over the tile tree (tile depth tau
 // nondecreasing), laid out breadth-first inside (upstream slots of a slot
 // contiguous, in routing rank) and packed next-fit into bins of kTileSlots
@@ -182,54 +182,63 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
       return MHMHIP_EINVAL;
     }
     auto child_node = [&](int64_t i) { return (int64_t)fr[i] - 1; };
-    // -- tile partition (open tile size and height per node) --
-    std::vector<int32_t> osize(nd), oh(nd), ocut(nd);
+    // -- tile partition --
+    // Bottom-up, each node's open tile is chosen to minimise first oc, the
+    // number of tile boundaries on its longest upstream path, then oh, the
+    // open tile's height in levels (a proxy slot is one level), then osize,
+    // its slots.  Upstream tiles whose oc is below the largest (M) can be
+    // closed without raising the node's oc: those taller than the tallest
+    // oc == M tile are closed, so the height stays that tile's height plus
+    // one, and the largest of them are closed while the slots overflow.  When
+    // the oc == M tiles cannot stay open (height or slot limit), the node's
+    // oc becomes M + 1 and every upstream tile except single leaves is
+    // closed, which leaves the lowest possible height (2).  Under the height
+    // limit alone this greedy is optimal: no partition has fewer tile
+    // boundaries on its longest path, which is the number of launches a
+    // window waits for before it reaches the outlet.
+    std::vector<int32_t> osize(nd), oh(nd), oc(nd);
     std::vector<char> cut(nd, 0);
     for (int64_t h = nd - 1; h >= 0; --h) {
       const int64_t n = q[h];
-      int32_t sz = 1, ht = 1;
-      for (int64_t i : up[n]) {
-        sz += osize[child_node(i)];
-        ht = std::max(ht, 1 + oh[child_node(i)]);
-      }
-      // over the height limit: close the tallest upstream tiles
-      while (ht > kTileHeight) {
-        int64_t best = -1;
-        for (int64_t i : up[n]) {
-          const int64_t c = child_node(i);
-          if (!cut[c] && (best < 0 || oh[c] > oh[best])) best = c;
-        }
-        cut[best] = 1;
-        sz += 1 - osize[best];
-        ht = 1;
-        for (int64_t i : up[n]) {
-          const int64_t c = child_node(i);
-          ht = std::max(ht, 1 + (cut[c] ? 1 : oh[c]));
-        }
-      }
-      // over the slot limit: close the upstream tiles with the fewest cuts
-      // below them first (closing adds one to their paths), the largest
-      // among equals
-      while (sz > kTileSlots) {
-        int64_t best = -1;
-        for (int64_t i : up[n]) {
-          const int64_t c = child_node(i);
-          if (cut[c]) continue;
-          if (best < 0 || ocut[c] < ocut[best] || (ocut[c] == ocut[best] && osize[c] > osize[best]))
-            best = c;
-        }
-        if (best < 0) break;  // cannot happen: 1 + nchild <= kTileSlots
-        cut[best] = 1;
-        sz += 1 - osize[best];
-      }
-      int32_t oc = 0;
+      int32_t M = -1, hc = 1, sz = 1;
+      for (int64_t i : up[n]) M = std::max(M, oc[child_node(i)]);
       for (int64_t i : up[n]) {
         const int64_t c = child_node(i);
-        oc = std::max(oc, ocut[c] + (cut[c] ? 1 : 0));
+        if (oc[c] == M) hc = std::max(hc, oh[c]);
       }
-      ocut[n] = oc;
+      bool open = M < 0 || 1 + hc <= kTileHeight;
+      if (open) {
+        for (int64_t i : up[n]) {
+          const int64_t c = child_node(i);
+          cut[c] = oc[c] < M && oh[c] > hc;
+          sz += cut[c] ? 1 : osize[c];
+        }
+        while (sz > kTileSlots) {
+          int64_t best = -1;
+          for (int64_t i : up[n]) {
+            const int64_t c = child_node(i);
+            if (!cut[c] && oc[c] < M && (best < 0 || osize[c] > osize[best])) best = c;
+          }
+          if (best < 0) break;
+          cut[best] = 1;
+          sz += 1 - osize[best];
+        }
+        open = sz <= kTileSlots;
+      }
+      if (open) {
+        oc[n] = std::max(M, 0);
+        oh[n] = M < 0 ? 1 : 1 + hc;
+      } else {
+        sz = 1;  // 1 + nchild <= 16 slots, height 2
+        for (int64_t i : up[n]) {
+          const int64_t c = child_node(i);
+          cut[c] = oh[c] > 1;
+          sz += cut[c] ? 1 : osize[c];
+        }
+        oc[n] = M + 1;
+        oh[n] = 2;
+      }
       osize[n] = sz;
-      oh[n] = ht;
     }
     // -- tiles, breadth-first over the tile tree --
     struct TSlot {

@@ -11,7 +11,8 @@ invariant it relies on is checked here:
   * every proxy has exactly one tile root with its boundary row, whose tile
     lies one tile-level deeper; domain sinks head depth-0 tiles;
   * tiles stay within 256 slots and 16 levels (lag 0..15);
-  * the tile depth grows by one per tile boundary along every path.
+  * the tile depth grows by one per tile boundary along every path;
+  * the longest chain of tiles is as short as the height limit allows.
 Networks: synthetic D8 basins and hand-made edge cases (single node, a
 chain longer than the height limit, a 15-child star, several outlets).
 """
@@ -126,3 +127,20 @@ def test_deep_chain_is_cut_by_height():
     plan, _, sl = N.plan_network(Net(100, np.arange(2, 101), np.arange(1, 100)), slots=True)
     assert plan.tau_max == (100 - 1) // HEIGHT
     assert plan.n_tiles == -(-100 // HEIGHT)
+
+
+@pytest.mark.parametrize("n,seed", [(40, 4), (97, 9), (256, 1)])
+def test_tile_chain_is_shortest(n, seed):
+    """Along the deepest path every tile but the top one spends a level on the
+    proxy of the next tile up, so no partition has fewer than
+    ceil((dmax + 1 - H) / (H - 1)) tile boundaries on it; on D8 basins the
+    slot limit never binds and the partition meets this bound."""
+    plan, _, _ = N.plan_network(synth.make_domain(n, n, seed=seed))
+    bound = -(-max(0, plan.dmax + 1 - HEIGHT) // (HEIGHT - 1))
+    assert plan.tau_max == bound
+
+
+@pytest.mark.parametrize("name,net", _networks(), ids=[n for n, _ in _networks()])
+def test_tile_chain_lower_bound(name, net):
+    plan, _, _ = N.plan_network(net)
+    assert plan.tau_max >= -(-max(0, plan.dmax + 1 - HEIGHT) // (HEIGHT - 1))
