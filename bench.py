@@ -50,9 +50,9 @@ def bytes_per_unit(kernel, steps_per_launch, steps_per_day, window=16):
         return 600.0
     if kernel == "cell_fused":
         # forcing once per forcing day (32/F), qOUT to the ring every step (8);
-        # once per launch: params 256 + states 64 read, states 64 + 23 fluxes 184
-        # written (568/S)
-        return 32.0 / steps_per_day + 8.0 + 568.0 / steps_per_launch
+        # once per launch: params 256 + states 64 + ring lag 1 read, states 64 +
+        # 23 fluxes 184 written (569/S)
+        return 32.0 / steps_per_day + 8.0 + 569.0 / steps_per_launch
     if kernel == "route_tile":
         # node-step: qOUT from the ring 8; once per window of B steps: slot metadata
         # 16, C1 + C2 16, qTIN read + write 16, qTR read + write 16
@@ -312,7 +312,7 @@ def main():
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--days", type=int, default=90)
     ap.add_argument("--steps-per-launch", type=int, default=24)
-    ap.add_argument("--route-window", type=int, default=16,
+    ap.add_argument("--route-window", type=int, default=64,
                     help="routing window of the headline loop (ring memory grows with it)")
     ap.add_argument("--no-graph", action="store_true", help="headline loop without hipGraph")
     ap.add_argument("--no-plain", action="store_true")
@@ -371,7 +371,7 @@ def main():
     prof = model.profile()
     model.set_profiling(False)
     value = float(dom.n_cells) * n_steps * args.steps / elapsed
-    kernels = kernel_table(prof, S, 24, min(args.route_window, S))
+    kernels = kernel_table(prof, S, 24, args.route_window)
     dominant = max(prof.items(), key=lambda kv: kv[1][0])[0]
     hbm = hbm_roofline(dominant, prof[dominant], S, 24, traffic)
     fpu = flops.get(dominant, {}).get("flops_per_unit")

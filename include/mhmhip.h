@@ -167,9 +167,11 @@ typedef struct {
   int32_t block;         /* cell-kernel block size (0 = autotuned)               */
   int32_t use_graph;     /* capture the tile sequence in a hipGraph              */
   int32_t route_mode;    /* 0 auto (= 3), 1 grid diagonals, 2 workgroup-resident
-                            trees, 3 tiles (windows of up to 24 steps per launch) */
-  int32_t route_window;  /* tiled routing: steps per window, 1..24 (0 = 8); the
-                            runoff ring holds (tile depth + 1) windows + S steps */
+                            trees, 3 tiles (windows of up to 64 steps per launch) */
+  int32_t route_window;  /* tiled routing: steps per window, 1..64 (0 = 8); the
+                            runoff ring holds (tile depth + 1) windows + S + 1
+                            steps (at most the call's steps, rounded up to
+                            whole windows) + 16 rows                          */
   int32_t reserved[2];
 } mhmhip_run_opts;
 

@@ -12,7 +12,7 @@ from . import _native as N
 
 
 def candidates(model, S_values=(8, 24, 72, 168), blocks=(64, 128, 256), graphs=(False, True),
-               windows=(8, 24)):
+               windows=(8, 64)):
     """Fused-path settings worth trying for this model (the routing window
     only matters for the tiled routing)."""
     route_modes = [3, 1, 2] if _small_trees(model) else [3, 1]

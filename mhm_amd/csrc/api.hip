@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/mhmhip.h"
@@ -67,6 +68,9 @@ struct mhmhip_ctx {
   std::vector<int32_t> h_nout;
   int32_t n_gauges = 0, n_inflow = 0;
   int32_t *d_gauge_node = nullptr, *d_inflow_node = nullptr, *d_inflow_hw = nullptr;
+  int32_t *d_node_gauge = nullptr;  // [n_cells] first gauge at the node, -1: none
+  uint8_t *d_lag = nullptr;         // [n_cells] ring lag of each node (its tile slot's lag)
+  std::vector<std::pair<int32_t, int32_t>> gauge_dup;  // (gauge, earlier gauge at its node)
   std::vector<int32_t> h_inflow_node_dev, h_inflow_hw;
   double *d_inflow_q = nullptr;
   int64_t inflow_q_rows = 0;
@@ -89,12 +93,12 @@ struct mhmhip_ctx {
 
   // time loop captured as a hipGraph, replayed while its key matches
   struct GraphKey {
-    int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1, prof = -1;
+    int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1, prof = -1, B = -1;
     const void *ring = nullptr, *gauge = nullptr, *steps = nullptr, *inflow = nullptr;
     int64_t ring_len = 0;
     bool operator==(const GraphKey &o) const {
       return nsteps == o.nsteps && mode == o.mode && S == o.S && block == o.block && wg == o.wg &&
-             prof == o.prof && ring == o.ring && gauge == o.gauge && steps == o.steps &&
+             prof == o.prof && B == o.B && ring == o.ring && gauge == o.gauge && steps == o.steps &&
              inflow == o.inflow && ring_len == o.ring_len;
     }
   };
@@ -311,6 +315,8 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_bins);
   (void)hipFree(x->d_bnd);
   (void)hipFree(x->d_gauge_node);
+  (void)hipFree(x->d_node_gauge);
+  (void)hipFree(x->d_lag);
   (void)hipFree(x->d_inflow_node);
   (void)hipFree(x->d_inflow_hw);
   (void)hipFree(x->d_inflow_q);
@@ -669,33 +675,38 @@ int route_kind(int route_mode) {
   return MHMHIP_K_ROUTE_TILE;
 }
 
-// Steps per routing window of the tiled kernel for S steps per cell launch.
-// Steps per routing window: the requested one (0: 8), at most S of the fused
-// path and kTileWindowMax.  The ring spans (tau_max + 1) windows, so smaller
+// Steps per routing window of the tiled kernel: the requested one (0: 8), at
+// most kTileWindowMax.  The ring spans (tau_max + 1) windows, so smaller
 // windows cost launches and save memory.
 constexpr int32_t kTileWindowDefault = 8;
-int32_t tile_window(int S, int32_t req) {
+int32_t tile_window(int32_t req) {
   const int32_t b = req > 0 ? req : kTileWindowDefault;
-  return std::max(1, std::min(std::min(b, (int32_t)S), (int32_t)kTileWindowMax));
+  return std::max(1, std::min(b, (int32_t)kTileWindowMax));
 }
 
-// Ring rows the routing of a call needs: the pipelined diagonals read a
-// step up to dmax diagonals after its cell launch, the tiles up to tau_max
-// windows after.
-int64_t ring_need(const mhmhip_ctx *x, int kind, int S, int32_t B) {
-  if (kind == MHMHIP_K_ROUTE_TILE) return ((int64_t)x->lay.tau_max + 1) * B + S + 1;
-  return (int64_t)S + x->lay.dmax + 2;
+// Ring rows the routing of a call of nsteps steps uses: the pipelined
+// diagonals read a step up to dmax diagonals after its cell launch, the tiles
+// up to tau_max windows after; never more than the call's steps.  For the
+// tiles the count is a multiple of B, so that no window's rows wrap.
+int64_t ring_need(const mhmhip_ctx *x, int kind, int S, int32_t B, int64_t nsteps) {
+  if (kind == MHMHIP_K_ROUTE_TILE) {
+    const int64_t n = std::min(((int64_t)x->lay.tau_max + 1) * B + S + 1, nsteps);
+    return (n + B - 1) / B * B;
+  }
+  return std::min((int64_t)S + x->lay.dmax + 2, nsteps);
 }
 
 TileArgs tile_args(const mhmhip_ctx *x, int32_t B, int64_t nsteps) {
   TileArgs g{};
   g.slot = x->d_slots;
   g.bin = x->d_bins;
+  g.node_gauge = x->d_node_gauge;
   g.bnd = x->d_bnd;
   g.n_bnd = std::max<int64_t>(1, x->lay.n_bnd);
   g.tau_max = x->lay.tau_max;
   g.B = B;
   g.nwin = (int32_t)((nsteps + B - 1) / B);
+  g.bstride = kTileWindowMax;
   return g;
 }
 
@@ -826,6 +837,15 @@ int route_tiles(mhmhip_ctx *x, const RouteArgs &r, const TileArgs &g, int32_t L_
   return MHMHIP_OK;
 }
 
+// Gauges that share a node with an earlier gauge: copies of that gauge's
+// series in the downloaded rows (steps x sets).
+void copy_dup_gauges(const mhmhip_ctx *x, double *out, int64_t nsteps) {
+  if (x->gauge_dup.empty() || x->n_gauges <= 0) return;
+  const int64_t rows = nsteps * x->n_sets;
+  for (int64_t i = 0; i < rows; ++i)
+    for (const auto &d : x->gauge_dup) out[i * x->n_gauges + d.first] = out[i * x->n_gauges + d.second];
+}
+
 // node-steps routed, credited to the routing kernel that did them
 void credit_route(mhmhip_ctx *x, int kind, int64_t nsteps) {
   if (x->prof) x->prof_units[kind] += (double)x->n_cells * x->n_sets * (double)nsteps;
@@ -867,9 +887,12 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
   if (route) {
     if (x->t_route != x->t_cell)
       return x->fail(MHMHIP_ESTATE, "mhm_step: previous step not routed (call mrm_step)");
-    CHECK_RC(ensure_ring(x, 2));
+    CHECK_RC(ensure_ring(x, 1 + kTileHeight));
   }
   CellArgs a = cell_args(x, route);
+  // the step is step 0 of a one-step ring, routed by the tiles (mrm_step)
+  a.ring_len = 1;
+  a.lag = route ? x->d_lag : nullptr;
   CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
               return launch_cell_step(x->nh, a, d, 0, 256, x->n_sets, x->stream);
             }),
@@ -993,6 +1016,8 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->dfree(x->d_bnd, x->n_bnd_dev);
   x->n_child_dev = x->n_slots_dev = x->n_bins_dev = x->n_bnd_dev = 0;
   x->dfree(x->d_gauge_node, std::max(1, x->n_gauges));
+  x->dfree(x->d_node_gauge, x->n_cells);
+  x->dfree(x->d_lag, x->n_cells);
   x->dfree(x->d_inflow_node, std::max(1, x->n_inflow));
   x->dfree(x->d_inflow_hw, std::max(1, x->n_inflow));
   CHECK_RC(x->dalloc(&x->d_meta, x->n_cells, "alloc meta"));
@@ -1030,6 +1055,8 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->n_gauges = net->n_gauges;
   x->n_inflow = net->n_inflow;
   CHECK_RC(x->dalloc(&x->d_gauge_node, std::max(1, x->n_gauges), "alloc gauges"));
+  CHECK_RC(x->dalloc(&x->d_node_gauge, x->n_cells, "alloc gauges"));
+  CHECK_RC(x->dalloc(&x->d_lag, x->n_cells, "alloc ring lags"));
   CHECK_RC(x->dalloc(&x->d_inflow_node, std::max(1, x->n_inflow), "alloc inflow"));
   CHECK_RC(x->dalloc(&x->d_inflow_hw, std::max(1, x->n_inflow), "alloc inflow"));
 
@@ -1047,6 +1074,21 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   CHECK_HIP(x, sync_copy(x, x->d_dom_start, dstart.data(), sizeof(int64_t) * (x->n_domains + 1), hipMemcpyHostToDevice), "upload domains");
   CHECK_HIP(x, sync_copy(x, x->d_dom_dmax, lay.dom_dmax.data(), sizeof(int32_t) * x->n_domains, hipMemcpyHostToDevice), "upload domains");
   CHECK_HIP(x, sync_copy(x, x->d_gauge_node, gnode.data(), sizeof(int32_t) * gnode.size(), hipMemcpyHostToDevice), "upload gauges");
+  // the tiled kernel writes a node's first gauge; later gauges at the same
+  // node are copies of it
+  std::vector<int32_t> node_gauge(x->n_cells, -1);
+  x->gauge_dup.clear();
+  for (int g = 0; g < net->n_gauges; ++g) {
+    int32_t &first = node_gauge[gnode[g]];
+    if (first < 0) first = g;
+    else x->gauge_dup.emplace_back(g, first);
+  }
+  CHECK_HIP(x, sync_copy(x, x->d_node_gauge, node_gauge.data(), sizeof(int32_t) * x->n_cells, hipMemcpyHostToDevice), "upload gauges");
+  // ring lag of each node: the lag of its real slot
+  std::vector<uint8_t> lag(x->n_cells, 0);
+  for (const SlotMeta &m : lay.slots)
+    if (m.a & SF_REAL) lag[m.pos] = (uint8_t)(m.a & 0xFFF);
+  CHECK_HIP(x, sync_copy(x, x->d_lag, lag.data(), x->n_cells, hipMemcpyHostToDevice), "upload ring lags");
   CHECK_HIP(x, sync_copy(x, x->d_inflow_node, inode.data(), sizeof(int32_t) * inode.size(), hipMemcpyHostToDevice), "upload inflow");
   CHECK_HIP(x, sync_copy(x, x->d_inflow_hw, ihw.data(), sizeof(int32_t) * ihw.size(), hipMemcpyHostToDevice), "upload inflow");
   x->h_inflow_node_dev = inode;
@@ -1186,17 +1228,15 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "mrm_step");
   CHECK_RC(ensure_gauge_buf(x, 1));
   CHECK_RC(normalize_qtr(x));
-  // this step is step 0 of the call: mhm_step wrote its qOUT to ring slot 0
+  // this step is step 0 of a one-step ring: mhm_step wrote its qOUT to ring
+  // row lag (the tiles' layout)
   RouteArgs r = route_args(x, 0, 1, 0);
-  const int kind = route_kind(0);
-  if (kind == MHMHIP_K_ROUTE_TILE) {
-    const TileArgs g = tile_args(x, 1, 1);
-    CHECK_RC(route_tiles(x, r, g, 0, g.nwin + g.tau_max));
-  } else {
-    const bool wg = kind == MHMHIP_K_ROUTE_WG;
-    CHECK_RC(route_diagonals(x, r, 0, 1, false, wg));
-    CHECK_RC(route_diagonals(x, r, 1, 0, true, wg));
-  }
+  r.ring_len = 1;
+  const int kind = MHMHIP_K_ROUTE_TILE;
+  const TileArgs g = tile_args(x, 1, 1);
+  if (x->n_inflow > 0)
+    CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, 0, 1, x->stream), "inflow_patch");
+  CHECK_RC(route_tiles(x, r, g, 0, g.nwin + g.tau_max));
   credit_route(x, kind, 1);
   x->qtr_slot = 0;
   x->t_route++;
@@ -1207,6 +1247,7 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
               "download gauges");
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "mrm_step");
+  if (gauge_out) copy_dup_gauges(x, gauge_out, 1);
   return prof_collect(x);
 }
 
@@ -1240,18 +1281,23 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   const int kind = route_kind(o.route_mode);
   const bool wg = kind == MHMHIP_K_ROUTE_WG;
   // tiled routing: windows of B steps (the plain path routes in windows too)
-  const int32_t B = tile_window(o.mode == MHMHIP_RUN_PLAIN ? kTileWindowMax : S, o.route_window);
+  const int32_t B = tile_window(o.route_window);
   CHECK_RC(ensure_steps(x, nsteps));
   CHECK_HIP(x, hipMemcpyAsync(x->d_steps, hs.data(), sizeof(StepDev) * nsteps,
                               hipMemcpyHostToDevice, x->stream),
             "upload steps");
+  const int64_t ring_rows = route ? ring_need(x, kind, S, B, nsteps) : 0;
   if (route) {
-    CHECK_RC(ensure_ring(x, ring_need(x, kind, S, B)));
+    CHECK_RC(ensure_ring(x, ring_rows + kTileHeight));  // ring_row's rows past R
     CHECK_RC(ensure_gauge_buf(x, nsteps));
   }
   if (route) CHECK_RC(normalize_qtr(x));
   CellArgs a = cell_args(x, route);
   RouteArgs r = route_args(x, 0, nsteps, 0);
+  if (route) {
+    a.ring_len = r.ring_len = ring_rows;  // this call's rows of the ring
+    if (kind == MHMHIP_K_ROUTE_TILE) a.lag = x->d_lag;  // qOUT written lag rows late
+  }
   const TileArgs g = tile_args(x, B, nsteps);
   const int32_t L_end = g.nwin + g.tau_max;
   // the whole loop, in steps relative to this call (replayable as a graph)
@@ -1271,6 +1317,9 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
                   "cell_fused");
       }
       if (!route) continue;
+      if (kind == MHMHIP_K_ROUTE_TILE && x->n_inflow > 0)
+        CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, k, k + n, x->stream),
+                  "inflow_patch");
       if (kind == MHMHIP_K_ROUTE_TILE) {
         // launch L needs the steps of window L (its deepest tiles' window)
         int32_t L1 = L;
@@ -1298,7 +1347,8 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     key.gauge = route ? x->d_gauge_buf : nullptr;
     key.steps = x->d_steps;
     key.inflow = x->d_inflow_q;
-    key.ring_len = x->ring_len;
+    key.ring_len = ring_rows;
+    key.B = B;
     if (!x->graph_exec || !(key == x->gkey)) {
       drop_graph(x);
       CHECK_HIP(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -1333,6 +1383,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
               "download gauges");
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "run");
+  if (route && gauge_out) copy_dup_gauges(x, gauge_out, nsteps);
   if (graph && x->prof) CHECK_RC(prof_collect_graph(x));
   return prof_collect(x);
 }

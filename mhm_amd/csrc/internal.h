@@ -10,7 +10,9 @@
 //   S[row][ld]   states
 //   X[row][ld]   fluxes of the last step
 //   F[var][day][sstride]  forcing, shared by all parameter sets
-//   ring[R][ld]  qOUT per step (L1 total runoff in m3/s), consumed by the routing
+//   ring[R + kTileHeight][ld]  qOUT per step (L1 total runoff in m3/s),
+//                consumed by the routing: step t of node p in row
+//                (t mod R) + lag[p] (ring_row), lag 0 for the diagonal kernels
 #pragma once
 
 #include <stdint.h>
@@ -78,6 +80,7 @@ struct CellArgs {
   const double *F;
   double *ring;            // nullptr: no routing input wanted
   const double *area;      // [n_cells] L1 cell area [km2] (routing input)
+  const uint8_t *lag;      // [n_cells] ring lag of the node (tiled routing), null: 0
   double tst;              // HourSecs * timestep [s]
   double rtst;             // RN(1 / tst)
   const StepDev *steps;    // device step table (fused kernel)
@@ -124,11 +127,21 @@ constexpr int32_t NF_MASK = NF_LINK | NF_QOUT | NF_CUT | NF_GAUGE | NF_INFLOW;
 #define MHMHIP_TILE_HEIGHT 16
 #endif
 #ifndef MHMHIP_TILE_WINDOW
-#define MHMHIP_TILE_WINDOW 24
+#define MHMHIP_TILE_WINDOW 64
 #endif
 constexpr int kTileSlots = MHMHIP_TILE_SLOTS;
 constexpr int kTileHeight = MHMHIP_TILE_HEIGHT;
-constexpr int kTileWindowMax = MHMHIP_TILE_WINDOW;  // steps per window (LDS: (B + 2) * slots * 8 B)
+constexpr int kTileWindowMax = MHMHIP_TILE_WINDOW;  // steps per window, at most
+static_assert(kTileHeight <= 256, "slot lags are stored in 8 bits");
+
+// Ring row of step t (relative to the call) of a node with lag g, in a ring
+// of ring_len logical rows (plus kTileHeight rows past them).  The tiled
+// routing's slot with lag g handles window step j on diagonal j + g, so
+// writing step t of the node g rows late puts everything a bin reads on one
+// diagonal into one row.  The diagonal kernels use g = 0.
+__host__ __device__ inline int64_t ring_row(int64_t t, int32_t g, int64_t ring_len) {
+  return t % ring_len + g;
+}
 
 // One slot of a bin (16 bytes).
 //   pos: device node index of a real slot, -1 for proxies and padding
@@ -163,7 +176,7 @@ struct RouteArgs {
   double *qTIN;            // [ld]   qTIN(:,IT1)
   double *pub;             // [2][ld] qTR by step parity
   double *qOUT;            // [ld]   last qOUT (reference L11_qOUT), may be null
-  const double *ring;      // [ring_len][ld]
+  const double *ring;      // [ring_len + kTileHeight][ld] (ring_row)
   double *gauge_out;       // [nsteps][n_sets][n_gauges] relative to t_base
   const int32_t *gauge_node;  // [n_gauges] device node index
   const int32_t *inflow_node; // [n_inflow] device node index
@@ -185,12 +198,13 @@ struct RouteArgs {
 struct TileArgs {
   const SlotMeta *slot;    // [n_bins * kTileSlots]
   const BinInfo *bin;      // [n_bins]
-  double *bnd;             // [2][n_sets][n_bnd][kTileWindowMax] tile-root outflow by window parity
+  const int32_t *node_gauge;  // [n_nodes] first gauge at the node (device index), -1: none
+  double *bnd;             // [2][n_sets][n_bnd][bstride] tile-root outflow by window parity
   int64_t n_bnd;
   int32_t tau_max;         // deepest tile
   int32_t B;               // steps per window
   int32_t nwin;            // windows in [t_lo, t_hi)
-  int32_t pad;
+  int32_t bstride;         // boundary series length (kTileWindowMax)
 };
 
 }  // namespace mhmhip

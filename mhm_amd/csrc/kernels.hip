@@ -255,7 +255,9 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
   chain_step<NH, JV>(a, r, x, maxinter, 1.0 / maxinter, st.evap, st.revap);
   store_states<NH>(a, bv, r);
   store_fluxes<NH>(a, bv, x);
-  if (a.ring) str(a.ring + (t % a.ring_len) * a.ld, bv, runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst));
+  if (a.ring)
+    a.ring[ring_row(t, a.lag ? a.lag[c] : 0, a.ring_len) * a.ld + bv / 8u] =
+        runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,7 +287,13 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
   int cached_day = -1, cached_doy = -1, cached_month = -1;
   double pre = 0.0, tavg = 0.0, pd = 0.0, maxinter = 0.0, rmaxinter = 0.0;
   const ConstStep *steps = (const ConstStep *)(a.steps + step0);
-  int64_t ring_row = a.ring ? t0 % a.ring_len : 0;
+  // ring: the lane's row offset (its lag) is fixed, the step's row uniform
+  double *rlane = nullptr;
+  int64_t roff = 0;
+  if (a.ring) {
+    rlane = a.ring + (int64_t)(a.lag ? a.lag[c] : 0) * a.ld + bv / 8u;
+    roff = t0 % a.ring_len * a.ld;
+  }
 #pragma nounroll
   for (int32_t s = 0; s < nsteps; ++s) {
     const StepDev st = read_step(steps + s);
@@ -314,8 +322,9 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
     }
     chain_step<NH, JV>(a, r, x, maxinter, rmaxinter, st.evap, st.revap);
     if (a.ring) {
-      str(a.ring + ring_row * a.ld, bv, runoff_acc(x.total, area, a.tst, a.rtst));
-      ring_row = ring_row + 1 == a.ring_len ? 0 : ring_row + 1;
+      rlane[roff] = runoff_acc(x.total, area, a.tst, a.rtst);
+      roff += a.ld;
+      if (roff == a.ring_len * a.ld) roff = 0;
     }
     if (s == nsteps - 1) store_fluxes<NH>(a, bv, x);
   }
@@ -413,145 +422,196 @@ __device__ __forceinline__ void lds_barrier() {
 // workgroup walks its own (depth, step) diagonals: slot with lag g handles
 // window step j on diagonal j + g, upstream slots (lag g - 1) produced step j
 // on the diagonal before, and a two-entry LDS parity buffer keeps it
-// readable while they write step j + 1.  qOUT (or a proxy's boundary
-// series) of the whole window is staged in LDS first.
+// readable while they write step j + 1.  Each slot streams its own series --
+// qOUT rows of the ring (real slots, inflow gauges already applied by
+// inflow_patch_kernel) or the upstream tile root's boundary series (proxies)
+// -- from memory, kTileAhead diagonals ahead of use, so LDS holds only the
+// parity buffers and the window length is not bounded by it.
+// The host keeps a window's ring rows contiguous (ring length and window
+// starts are multiples of B).
 // Same per-node arithmetic as route_node (mo_mrm_routing.f90:425-472).
+constexpr int kTileAhead = 8;
+
+struct TileLane {
+  const double *p;      // series element of diagonal 0 (real) or window step 0 (proxy)
+  uint32_t stride;      // bytes between consecutive elements of the series (0: idle)
+  double *bout;         // tile root: boundary series of this window, else null
+  int64_t t0;           // global step of window step 0
+  int32_t lag, nb, nc, c0;
+  uint32_t bv;          // byte offset of (set, node) in a row
+  int32_t a, pos, set;
+  int32_t gauge;        // first gauge at the node, -1: none
+  bool act, real, proxy;
+  double qtin1, qtr1, c1, c2;
+};
+
+// One sweep over the bin's nd diagonals; NCW bounds the upstream slots summed
+// (the wave's largest upstream count rounded up to 1, 3, 5 or 8 -- a node's
+// slots past its own count read a +0.0 slot, which leaves the running sum's
+// value unchanged).  LDS: qTR parity buffers [2][W], +0.0 at 2W and 3W, so
+// that buffer 1's offsets are buffer 0's plus W for upstream and zero slots
+// alike.
+template <int NCW>
+__device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, double *qtr, int nd) {
+  constexpr int W = kTileSlots;
+  const int s = threadIdx.x;
+  uint32_t up[NCW];  // byte offsets in buffer 0
+  const uint32_t zoff = (uint32_t)(2 * W) * 8u;
+#pragma unroll
+  for (int c = 0; c < NCW; ++c) up[c] = c < u.nc ? (uint32_t)(u.c0 + c) * 8u : zoff;
+  const bool link = (u.a & NF_LINK) != 0, qflag = (u.a & NF_QOUT) != 0, cut = (u.a & NF_CUT) != 0;
+  const bool rare = u.bout != nullptr || u.gauge >= 0 || r.qOUT != nullptr;
+  const char *qtr_b = reinterpret_cast<const char *>(qtr);
+  // Value for diagonal df: a real slot's qOUT sits in ring row df of the
+  // window (the ring is written lag rows late), the same row for every real
+  // slot of the bin; a proxy's boundary element is its window step df - lag,
+  // clamped into the window.  The address is formed from the index each time:
+  // written as a conditional advance, the compiler reuses the previous
+  // element where the clamp holds it, and each fetch then waits for the one
+  // before.
+  const int32_t jmax = u.nb > 0 ? u.nb - 1 : 0;
+  auto fetch = [&](int df) -> double {
+    const uint32_t jc = u.real ? (uint32_t)df : (uint32_t)min(max(df - u.lag, 0), jmax);
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(u.p) +
+                                             (uint64_t)jc * u.stride);
+  };
+  double qq[kTileAhead];
+#pragma unroll
+  for (int i = 0; i < kTileAhead; ++i) qq[i] = 0.0;
+  double musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
+  // Diagonal d reads parity buffer (d + 1) & 1, written on diagonal d - 1 by
+  // the upstream slots (one level deeper, one diagonal earlier, same step),
+  // and writes buffer d & 1.  A slot writes every diagonal: an inactive
+  // slot's value is read only by a parent that is inactive for the same step.
+  auto diagonal = [&](int d, int wb, double qo) {
+    const int j = d - u.lag;
+    const bool on = u.act && (uint32_t)j < (uint32_t)u.nb;
+    const char *rb_b = qtr_b + (wb ^ 1) * W * 8;  // the buffer written on diagonal d - 1
+    double v[NCW];
+#pragma unroll
+    for (int c = 0; c < NCW; ++c) v[c] = *reinterpret_cast<const double *>(rb_b + up[c]);
+    // upstream qTR in routing rank
+    double sum = 0.0;
+#pragma unroll
+    for (int c = 0; c < NCW; ++c) sum = sum + v[c];
+    if (NCW == 8 && u.nc > 8) {
+      const double *rb = qtr + (wb ^ 1) * W;
+      for (int c = 8; c < u.nc; ++c) sum = sum + rb[u.c0 + c];
+    }
+    if (qflag) sum = sum + qo;
+    // musk = qTR(IT1) + C1 (qTIN(IT1) - qTR(IT1)), formed on the diagonal before
+    double q = musk + u.c2 * (sum - u.qtin1);
+    if (cut) q = 0.0;
+    qtr[wb * W + s] = u.proxy ? qo : q;
+    const bool upd = on && u.real;
+    if (upd && link) u.qtr1 = q;
+    if (upd) u.qtin1 = sum;
+    musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
+    // at most three stores, none in a loop: the loads in flight stay countable
+    if (upd && rare) {
+      if (u.bout) u.bout[j] = q;
+      const int64_t t = u.t0 + j;
+      if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, u.bv, qo);
+      if (u.gauge >= 0) r.gauge_out[((t - r.t_base) * r.n_sets + u.set) * r.n_gauges + u.gauge] = sum;
+    }
+  };
+  // The first pass only fetches.  Fetches stay in diagonal order, one per
+  // diagonal, in every pass, so that a value is waited for kTileAhead - 1
+  // fetches after its own (a prologue whose loads the compiler reorders would
+  // make the loop wait for all of them).
+  for (int d0 = -kTileAhead; d0 < nd; d0 += kTileAhead) {
+#pragma unroll
+    for (int i = 0; i < kTileAhead; ++i) {
+      const int d = d0 + i;
+      if (d >= 0 && d < nd) {
+        diagonal(d, i & 1, qq[i]);
+        lds_barrier();
+      }
+      qq[i] = fetch(min(d + kTileAhead, nd - 1));
+    }
+  }
+}
+
 __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, TileArgs g,
                                                                 int32_t L, int32_t bin0) {
-  extern __shared__ double lds[];
-  const int W = kTileSlots;
+  extern __shared__ double lds[];  // qTR parity buffers [2][W], +0.0 at 2W and 3W
+  constexpr int W = kTileSlots;
   const int bin = bin0 + blockIdx.x, set = blockIdx.y, s = threadIdx.x;
   const BinInfo bi = g.bin[bin];
   // whole bin idle in this launch: every thread leaves before any barrier
   if (L - (g.tau_max - bi.tau_hi) < 0 || L - (g.tau_max - bi.tau_lo) >= g.nwin) return;
   const SlotMeta m = g.slot[(int64_t)bin * W + s];
+  TileLane u;
   const int32_t k = L - (g.tau_max - (m.b >> 16));
-  const bool real = (m.a & SF_REAL) != 0, proxy = (m.a & SF_PROXY) != 0;
-  const bool act = (real || proxy) && k >= 0 && k < g.nwin;
-  const int64_t t0 = r.t_lo + (int64_t)k * g.B;
-  const int nb = act ? (int)min((int64_t)g.B, r.t_hi - t0) : 0;
+  u.a = m.a;
+  u.pos = m.pos;
+  u.set = set;
+  u.real = (m.a & SF_REAL) != 0;
+  u.proxy = (m.a & SF_PROXY) != 0;
+  u.act = (u.real || u.proxy) && k >= 0 && k < g.nwin;
+  u.t0 = r.t_lo + (int64_t)k * g.B;
+  u.nb = u.act ? (int)min((int64_t)g.B, r.t_hi - u.t0) : 0;
+  u.lag = m.a & 0xFFF;
+  u.nc = (m.a >> 12) & 0xF;
+  u.c0 = m.b & 0xFFFF;
   const int64_t ld = (int64_t)r.n_sets * r.sstride;
-  const uint32_t bv = real ? (uint32_t)(set * r.sstride + m.pos) * 8u : 0u;
-  double *qin = lds;             // [B][W]
-  double *qtr = lds + g.B * W;   // [2][W], then one +0.0
-  double *zero = qtr + 2 * W;
-  if (s == 0) *zero = 0.0;
-  double qtin1 = 0.0, qtr1 = 0.0, c1 = 0.0, c2 = 0.0;
-  if (act) {
-    // all loads of the window in flight at once, then to LDS
-    double v[kTileWindowMax];
-    if (real) {
-      int64_t row = t0 % r.ring_len;
-      const double *base = r.ring + (int64_t)set * r.sstride + m.pos;
-#pragma unroll
-      for (int j = 0; j < kTileWindowMax; ++j) {
-        if (j < nb) v[j] = base[row * ld];
-        row = row + 1 == r.ring_len ? 0 : row + 1;
-      }
-      qtin1 = ldr(r.qTIN, bv);
-      if (m.a & NF_LINK) {
-        qtr1 = ldr(r.pub + ((t0 - 1) & 1) * ld, bv);
-        c1 = ldr(r.C1, bv);
-        c2 = ldr(r.C2, bv);
-      }
-    } else {
-      const double *src =
-          g.bnd + (((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd + m.bnd) * kTileWindowMax;
-#pragma unroll
-      for (int j = 0; j < kTileWindowMax; ++j)
-        if (j < nb) v[j] = src[j];
+  u.bv = u.real ? (uint32_t)(set * r.sstride + m.pos) * 8u : 0u;
+  u.qtin1 = u.qtr1 = u.c1 = u.c2 = 0.0;
+  // series of the slot: ring rows t0 mod R + d, d < nd (t0 and R are
+  // multiples of B, and the ring has kTileHeight rows past R) or the
+  // boundary series of the proxy's tile root; idle slots re-read one valid
+  // element
+  const int64_t bseries = ((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd;
+  if (u.act && u.real) {
+    u.p = r.ring + ring_row(u.t0, 0, r.ring_len) * ld + (int64_t)set * r.sstride + m.pos;
+    u.stride = (uint32_t)(ld * 8);
+    u.qtin1 = ldr(r.qTIN, u.bv);
+    if (m.a & NF_LINK) {
+      u.qtr1 = ldr(r.pub + ((u.t0 - 1) & 1) * ld, u.bv);
+      u.c1 = ldr(r.C1, u.bv);
+      u.c2 = ldr(r.C2, u.bv);
     }
-#pragma unroll
-    for (int j = 0; j < kTileWindowMax; ++j)
-      if (j < nb) qin[j * W + s] = v[j];
-    if (real && (m.a & NF_INFLOW)) {
-      for (int j = 0; j < nb; ++j) {
-        const int64_t t = t0 + j;
-        double q = qin[j * W + s];
-        for (int32_t u = 0; u < r.n_inflow; ++u) {
-          if (r.inflow_node[u] == m.pos) {
-            const double qi = r.inflow_q[(t - r.t_base) * r.n_inflow + u];
-            q = r.inflow_hw[u] ? q + qi : qi;
-          }
-        }
-        qin[j * W + s] = q;
-      }
-    }
+  } else if (u.act) {
+    u.p = g.bnd + (bseries + m.bnd) * g.bstride;
+    u.stride = 8;
+  } else {
+    u.p = g.bnd;
+    u.stride = 0;
   }
+  u.bout = (u.act && (m.a & SF_TROOT)) ? g.bnd + (bseries + m.bnd) * g.bstride : nullptr;
+  // gauges: the node's first gauge only (the host copies it to the others)
+  u.gauge = (u.act && u.real && (m.a & NF_GAUGE)) ? g.node_gauge[m.pos] : -1;
+  double *qtr = lds;
+  if (s == 0) qtr[2 * W] = qtr[3 * W] = 0.0;
   lds_barrier();
-  const int lag = m.a & 0xFFF, nc = (m.a >> 12) & 0xF, c0 = m.b & 0xFFFF;
-  double *bout = (m.a & SF_TROOT)
-                     ? g.bnd + (((int64_t)(k & 1) * r.n_sets + set) * g.n_bnd + m.bnd) * kTileWindowMax
-                     : nullptr;
   const int nd = bi.height + g.B - 1;
-  double musk = qtr1 + c1 * (qtin1 - qtr1);
-  // LDS byte offsets of the upstream slots in parity buffer 0 and 1 (past the
-  // node's upstream count: the +0.0 slot, which leaves the running sum's
-  // value unchanged)
-  uint32_t up0[8], up1[8];
-  const uint32_t zoff = (uint32_t)(2 * W) * 8u;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    up0[c] = c < nc ? (uint32_t)(c0 + c) * 8u : zoff;
-    up1[c] = c < nc ? (uint32_t)(W + c0 + c) * 8u : zoff;
+  if (__any(u.nc > 5)) tile_sweep<8>(r, u, qtr, nd);
+  else if (__any(u.nc > 3)) tile_sweep<5>(r, u, qtr, nd);
+  else if (__any(u.nc > 1)) tile_sweep<3>(r, u, qtr, nd);
+  else tile_sweep<1>(r, u, qtr, nd);
+  if (u.act && u.real) {
+    str(r.qTIN, u.bv, u.qtin1);
+    if (m.a & NF_LINK) str(r.pub + ((u.t0 + u.nb - 1) & 1) * ld, u.bv, u.qtr1);
   }
-  const bool link = (m.a & NF_LINK) != 0, qflag = (m.a & NF_QOUT) != 0, cut = (m.a & NF_CUT) != 0;
-  const bool rare = (m.a & (NF_GAUGE | SF_TROOT)) != 0 || r.qOUT != nullptr;
-  const char *qtr_b = reinterpret_cast<const char *>(qtr);
-  // Diagonal d reads parity buffer (d + 1) & 1, written on diagonal d - 1 by
-  // the upstream slots (one level deeper, one diagonal earlier, same step),
-  // and writes buffer d & 1.  A slot writes every diagonal: an inactive
-  // slot's value is read only by a parent that is inactive for the same step.
-  auto diagonal = [&](int d, const uint32_t *up, int wb) {
-    const int j = d - lag;
-    const bool on = act && j >= 0 && j < nb;
-    const int jj = j < 0 ? 0 : (j < g.B ? j : g.B - 1);
-    const double qo = qin[jj * W + s];
-    double v[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = *reinterpret_cast<const double *>(qtr_b + up[c]);
-    // upstream qTR in routing rank (D8: at most 8; the header allows 15)
-    double sum = 0.0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) sum = sum + v[c];
-    if (nc > 8) {
-      const double *rb = qtr + (wb ^ 1) * W;
-      for (int c = 8; c < nc; ++c) sum = sum + rb[c0 + c];
-    }
-    if (qflag) sum = sum + qo;
-    // musk = qTR(IT1) + C1 (qTIN(IT1) - qTR(IT1)), formed on the diagonal before
-    double q = musk + c2 * (sum - qtin1);
-    if (cut) q = 0.0;
-    qtr[wb * W + s] = proxy ? qo : q;
-    const bool upd = on && real;
-    if (upd && link) qtr1 = q;
-    if (upd) qtin1 = sum;
-    musk = qtr1 + c1 * (qtin1 - qtr1);
-    if (upd && rare) {
-      if (bout) bout[j] = q;
-      const int64_t t = t0 + j;
-      if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, bv, qo);
-      if (m.a & NF_GAUGE) {
-        for (int32_t u = 0; u < r.n_gauges; ++u)
-          if (r.gauge_node[u] == m.pos)
-            r.gauge_out[((t - r.t_base) * r.n_sets + set) * r.n_gauges + u] = sum;
-      }
-    }
-  };
-  int d = 0;
-  for (; d + 1 < nd; d += 2) {
-    diagonal(d, up1, 0);
-    lds_barrier();
-    diagonal(d + 1, up0, 1);
-    lds_barrier();
-  }
-  if (d < nd) {
-    diagonal(d, up1, 0);
-    lds_barrier();
-  }
-  if (act && real) {
-    str(r.qTIN, bv, qtin1);
-    if (m.a & NF_LINK) str(r.pub + ((t0 + nb - 1) & 1) * ld, bv, qtr1);
+}
+
+// Inflow gauges for the tiled routing (add_inflow, mo_mrm_pre_routing.f90:
+// 179-214, as route_node applies them): the qOUT of an inflow node, in the
+// ring rows of steps [t_lo, t_hi), is replaced by the gauge's discharge, or
+// increased by it for a headwater gauge, before the tiles read those rows.
+// One thread per (step, set) applies the gauges in their order.
+__global__ __launch_bounds__(256) void inflow_patch_kernel(RouteArgs r, double *ring,
+                                                           const uint8_t *lag, int64_t t_lo,
+                                                           int64_t t_hi) {
+  const int64_t t = t_lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t set = blockIdx.y;
+  if (t >= t_hi) return;
+  const int64_t ld = (int64_t)r.n_sets * r.sstride;
+  for (int32_t g = 0; g < r.n_inflow; ++g) {
+    const int32_t p = r.inflow_node[g];
+    const double qi = r.inflow_q[(t - r.t_base) * r.n_inflow + g];
+    double &q = ring[ring_row(t, lag[p], r.ring_len) * ld + (int64_t)set * r.sstride + p];
+    q = r.inflow_hw[g] ? q + qi : qi;
   }
 }
 
@@ -618,12 +678,24 @@ hipError_t launch_route_diag(const RouteArgs &r, int64_t d, int64_t p_lo, int64_
 
 hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_lo, int64_t bin_hi,
                              int64_t n_bins, int32_t L, hipStream_t s) {
-  if (bin_lo < 0 || bin_hi > n_bins || bin_hi <= bin_lo || g.B < 1 || g.B > kTileWindowMax ||
-      g.nwin < 1)
+  // a window's rows do not wrap (ring length and window starts are multiples
+  // of B; the ring has kTileHeight rows past its length), and a ring row's
+  // bytes fit the kernel's 32-bit stride
+  if (bin_lo < 0 || bin_hi > n_bins || bin_hi <= bin_lo || g.B < 1 || g.B > g.bstride ||
+      g.bstride > kTileWindowMax || g.nwin < 1 || r.ring_len % g.B != 0 || r.t_lo % g.B != 0 ||
+      (uint64_t)r.n_sets * (uint64_t)r.sstride * 8u > 0xFFFFFFFFull)
     return hipErrorInvalidValue;
   dim3 grid((unsigned)(bin_hi - bin_lo), (unsigned)r.n_sets);
-  const size_t shm = sizeof(double) * ((size_t)(g.B + 2) * kTileSlots + 1);
+  const size_t shm = sizeof(double) * (3 * (size_t)kTileSlots + 1);
   hipLaunchKernelGGL(route_tile_kernel, grid, dim3(kTileSlots), shm, s, r, g, L, (int32_t)bin_lo);
+  return hipGetLastError();
+}
+
+hipError_t launch_inflow_patch(const RouteArgs &r, double *ring, const uint8_t *lag, int64_t t_lo,
+                               int64_t t_hi, hipStream_t s) {
+  if (t_hi <= t_lo || t_lo < r.t_base || r.n_inflow < 1 || !lag) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((t_hi - t_lo + 255) / 256), (unsigned)r.n_sets);
+  hipLaunchKernelGGL(inflow_patch_kernel, grid, dim3(256), 0, s, r, ring, lag, t_lo, t_hi);
   return hipGetLastError();
 }
 

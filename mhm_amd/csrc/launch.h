@@ -16,6 +16,8 @@ hipError_t launch_route_diag(const RouteArgs &r, int64_t d, int64_t p_lo, int64_
                              hipStream_t s);
 hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_lo, int64_t bin_hi,
                              int64_t n_bins, int32_t L, hipStream_t s);
+hipError_t launch_inflow_patch(const RouteArgs &r, double *ring, const uint8_t *lag, int64_t t_lo,
+                               int64_t t_hi, hipStream_t s);
 hipError_t launch_route_wg(const RouteArgs &r, const int64_t *dom_start, const int32_t *dom_dmax,
                            int n_domains, int64_t d_lo, int64_t d_hi, int drain, int block,
                            hipStream_t s);

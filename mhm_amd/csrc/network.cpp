@@ -93,7 +93,8 @@ namespace mhmhip {
 // Tiles are numbered breadth-first over the tile tree (tile depth tau
 // nondecreasing), laid out breadth-first inside (upstream slots of a slot
 // contiguous, in routing rank) and packed next-fit into bins of kTileSlots
-// slots; device indices follow the real slots of the bins in order.
+// slots, lag-major inside a bin; device indices follow the real slots of the
+// bins in order.
 int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
   const int64_t nN = in.n_nodes;
   out.pos.assign(nN, -1);
@@ -305,30 +306,43 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
       }
       const size_t sbase = out.slots.size();
       out.slots.resize(sbase + kTileSlots, SlotMeta{-1, -1, 0, 0});
-      int32_t off = 0;
+      // Slots of the bin in lag-major order (largest lag first; within a
+      // lag, tile order, then breadth-first order): the nodes of one lag,
+      // whose qOUT sits in one ring row on every diagonal, get consecutive
+      // device indices.  A slot's upstream slots (one lag lower, same tile,
+      // consecutive breadth-first) stay contiguous.
+      struct Ent {
+        int32_t lag, t, k;
+      };
+      std::vector<Ent> ent;
+      std::vector<std::vector<int32_t>> idx(t1 - t0);
       for (size_t t = t0; t < t1; ++t) {
-        const Tile &T = tiles[t];
-        for (size_t k = 0; k < T.s.size(); ++k) {
-          const TSlot &u = T.s[k];
-          SlotMeta &m = out.slots[sbase + off + k];
-          const int32_t lag = T.height - 1 - u.ldepth;
-          if (u.proxy) {
+        idx[t - t0].resize(tiles[t].s.size());
+        for (size_t k = 0; k < tiles[t].s.size(); ++k)
+          ent.push_back({tiles[t].height - 1 - tiles[t].s[k].ldepth, (int32_t)(t - t0), (int32_t)k});
+      }
+      std::stable_sort(ent.begin(), ent.end(), [](const Ent &a, const Ent &b) { return a.lag > b.lag; });
+      for (size_t i = 0; i < ent.size(); ++i) idx[ent[i].t][ent[i].k] = (int32_t)i;
+      for (size_t i = 0; i < ent.size(); ++i) {
+        const Tile &T = tiles[t0 + ent[i].t];
+        const size_t k = (size_t)ent[i].k;
+        const TSlot &u = T.s[k];
+        SlotMeta &m = out.slots[sbase + i];
+        if (u.proxy) {
+          m.bnd = (int32_t)bnd_row[u.node];
+          m.a = ent[i].lag | SF_PROXY;
+        } else {
+          const int64_t p = next_pos++;
+          out.pos[n0 + u.node] = p;
+          m.pos = (int32_t)p;
+          m.a = ent[i].lag | (u.nc << 12) | SF_REAL;
+          if (k == 0 && cut[u.node]) {
             m.bnd = (int32_t)bnd_row[u.node];
-            m.a = lag | SF_PROXY;
-          } else {
-            const int64_t p = next_pos++;
-            out.pos[n0 + u.node] = p;
-            m.pos = (int32_t)p;
-            m.a = lag | (u.nc << 12) | SF_REAL;
-            if (k == 0 && cut[u.node]) {
-              m.bnd = (int32_t)bnd_row[u.node];
-              m.a |= SF_TROOT;
-            }
+            m.a |= SF_TROOT;
           }
-          m.b = (u.proxy ? 0 : (off + u.first)) | (T.tau << 16);
-          out.tau_max = std::max(out.tau_max, T.tau);
         }
-        off += (int32_t)T.s.size();
+        m.b = (u.proxy || u.nc == 0 ? 0 : idx[ent[i].t][u.first]) | (T.tau << 16);
+        out.tau_max = std::max(out.tau_max, T.tau);
       }
       out.bins.push_back(bi);
       out.n_tiles += (int64_t)(t1 - t0);

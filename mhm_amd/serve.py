@@ -56,6 +56,10 @@ class Result:
         return self.finished - self.arrival
 
 
+TILE_WINDOW_MAX = 64  # longest routing window; boundary series are allocated at this length
+TILE_HEIGHT = 16  # levels per routing tile; the runoff ring has this many rows past its length
+
+
 def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_steps, n_steps,
                    n_gauges=1, n_outlets=1):
     """Device footprint of one context run in chunks of `chunk_steps` fused
@@ -65,12 +69,16 @@ def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_step
     ld = sstride * n_sets
     rows = (19 + 5 * n_horizons + 24) + (5 + n_horizons) + (17 + 2 * n_horizons)
     cells = rows * ld * 8 + 5 * n_forcing_days * sstride * 8
-    window = max(1, min(chunk_steps, 8))  # default routing window (mhmhip_run_opts)
-    ring = ((plan.tau_max + 1) * window + chunk_steps + 1) * ld * 8
-    routing = (16 * n_cells + 6 * ld * 8 + 20 + 4 * max(1, n_cells - n_outlets)
-               + 16 * plan.n_slots + 16 * plan.n_bins + 2 * n_sets * max(1, plan.n_bnd) * 24 * 8
+    window = 8  # default routing window (mhmhip_run_opts)
+    rows_run = min(chunk_steps, n_steps)  # steps of the longest call
+    # ring rows of a call: (tile depth + 1) windows + S + 1, at most the
+    # call's steps, rounded up to whole windows, and the lag rows past them
+    ring_rows = min((plan.tau_max + 1) * window + chunk_steps + 1, rows_run)
+    ring = (-(-ring_rows // window) * window + TILE_HEIGHT) * ld * 8
+    routing = (21 * n_cells + 6 * ld * 8 + 20 + 4 * max(1, n_cells - n_outlets)
+               + 16 * plan.n_slots + 16 * plan.n_bins
+               + 2 * n_sets * max(1, plan.n_bnd) * TILE_WINDOW_MAX * 8
                + 4 * max(1, n_gauges) + 8)
-    rows_run = min(chunk_steps, n_steps)
     series = rows_run * 96 + rows_run * n_sets * max(1, n_gauges) * 8
     return cells + ring + routing + series
 

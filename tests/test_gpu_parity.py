@@ -46,6 +46,26 @@ def test_step_by_step_matches_oracle():
     model.close()
 
 
+@pytest.mark.parametrize("mode,window,S", [(1, 1, 5), (1, 7, 24), (1, 64, 24), (1, 40, 48),
+                                           (0, 64, 0)])
+def test_route_windows_and_shared_gauge_nodes(mode, window, S):
+    """Tiled routing at windows shorter and longer than the cell launches, and
+    gauges that share a node (the kernel writes a node's first gauge, the
+    host copies it to the others)."""
+    _log(f"case: tile windows mode={mode} window={window} S={S}, shared gauge nodes")
+    dom = synth.make_domain(30, 26, seed=21)
+    outlet, inner = int(dom.gauge_nodes[0]), int(dom.toN[5])
+    dom.gauge_nodes = np.array([outlet, inner, outlet, inner, inner], np.int32)
+    model, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=70, seed=8)
+    g = model.run(steps, mode=mode, steps_per_launch=S, route_mode=3, route_window=window)
+    go = _oracle_series(sim, recs)
+    ok, worst = rel_close(g, go, RTOL, ATOL)
+    assert ok, worst
+    assert np.array_equal(g[:, :, 0], g[:, :, 2]) and np.array_equal(g[:, :, 1], g[:, :, 4])
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    model.close()
+
+
 @pytest.mark.parametrize("mode,S,route_mode", [(0, 0, 1), (1, 24, 1), (1, 5, 2), (1, 48, 2),
                                                (0, 0, 3), (1, 24, 3), (1, 5, 3), (1, 48, 3)])
 def test_run_matches_oracle(mode, S, route_mode):
