@@ -96,18 +96,28 @@ def hbm_roofline(name, stats, steps_per_launch, steps_per_day, traffic):
             "algorithmic_bytes_per_launch": round(units * bpu / launches, 1)}
 
 
-def fp64_roofline(name, stats, flops_per_unit):
+N_CU, CLOCK_GHZ = 256, 2.4
+
+
+def fp64_roofline(name, stats, flops_per_unit, valu_per_cell_step=None):
     """FP64 VALU roofline: flops per cell-step (PMC count of the FP64 VALU
     instructions of this kernel on this workload, 64 lanes each, FMA = 2)
-    x cell-steps per launch / measured launch time."""
+    x cell-steps per launch / measured launch time.  With the PMC count of
+    all VALU wave instructions per cell-step, also the share of the CUs'
+    VALU issue rate (one wave instruction per CU and cycle) the kernel uses."""
     ms, launches, units = stats
     if launches == 0 or ms <= 0 or not flops_per_unit:
         return None
     achieved = units * flops_per_unit / (ms * 1e-3) / 1e12
-    return {"bound": "fp64-valu", "kernel": name, "achieved": round(achieved, 2),
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "flops_per_cell_step": round(flops_per_unit, 1),
-            "avg_launch_us": round(ms * 1e3 / launches, 3)}
+    out = {"bound": "fp64-valu", "kernel": name, "achieved": round(achieved, 2),
+           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+           "flops_per_cell_step": round(flops_per_unit, 1),
+           "avg_launch_us": round(ms * 1e3 / launches, 3)}
+    if valu_per_cell_step:
+        rate = units * valu_per_cell_step / (ms * 1e-3)  # wave instructions per second
+        out["valu_issue"] = {"wave_insts_per_cell_step": round(valu_per_cell_step, 4),
+                             "frac": round(rate / (N_CU * CLOCK_GHZ * 1e9), 4)}
+    return out
 
 
 def cpu_baseline(dom, gam, n_days, sample_steps, threads):
@@ -204,19 +214,27 @@ def bench_serving(serve, n_req, qps, seed=11):
 # -- PMC traffic ------------------------------------------------------------------
 
 def pmc_workload(args):
-    """Short run for counter collection: the headline kernels over two days of
-    the same basin, then a plain device copy of COPY_CALIB_BYTES."""
+    """Short run for counter collection: the headline kernels on the same
+    basin (2 plain steps, then 6 fused launches routed in the headline's
+    windows), then a plain device copy of COPY_CALIB_BYTES."""
     import torch
     from mhm_amd import _native as N, workloads
-    model, steps, dom, _ = workloads.plain_domain(args.grid, args.grid, 48, seed=1)
-    model.run(steps, mode=N.RUN_FUSED, steps_per_launch=args.steps_per_launch)
+    n_steps = 2 + 6 * args.steps_per_launch  # 2 plain steps, then whole fused launches
+    model, steps, dom, _ = workloads.plain_domain(args.grid, args.grid, n_steps, seed=1)
     model.run(steps[:2], mode=N.RUN_PLAIN)
+    model.run(steps[2:], mode=N.RUN_FUSED, steps_per_launch=args.steps_per_launch,
+              route_window=args.route_window)
     model.sync()
     model.close()
+    plan, _, _ = N.plan_network(dom)
+    n_fused = n_steps - 2
     os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
     with open(os.path.join(ROOT, "gpurun_out", "pmc_meta.json"), "w") as f:
         json.dump({"cells": dom.n_cells, "steps_per_launch": args.steps_per_launch,
-                   "fused_launches": len(steps) // args.steps_per_launch, "plain_launches": 2}, f)
+                   "fused_launches": -(-n_fused // args.steps_per_launch), "plain_launches": 2,
+                   "route_window": args.route_window,
+                   "route_tile_fused_launches": -(-n_fused // args.route_window) + plan.tau_max,
+                   "route_node_steps": dom.n_cells * n_fused}, f)
     a = torch.empty(COPY_CALIB_BYTES // 8, dtype=torch.float64, device="cuda")
     a.fill_(1.0)
     b = torch.empty_like(a)
@@ -237,15 +255,15 @@ def _pmc_rows(d):
 
 
 def _per_kernel(rows, counter):
-    """{kernel short name: [value per dispatch]} for one counter."""
+    """{kernel short name: [value per dispatch, in dispatch order]} for one counter."""
     out = {}
-    for r in rows:
+    for r in sorted(rows, key=lambda r: int(r.get("Dispatch_Id", 0))):
         if r.get("Counter_Name") != counter:
             continue
         name = r["Kernel_Name"]
         short = ("copy" if ("copy" in name.lower() or "elementwise" in name.lower())
                  and "mhm" not in name.lower() else None)
-        for k in ("cell_fused", "cell_step", "route_diag", "route_wg"):
+        for k in ("cell_fused", "cell_step", "route_diag", "route_wg", "route_tile"):
             if k in name:
                 short = k
         if short:
@@ -271,6 +289,17 @@ def summarize_pmc(fetch_dir, write_dir, out_path):
         wv = sum(write[k]) / len(write[k]) * w_scale * 1024.0
         res[k] = {"bytes_per_launch": round(fv + wv), "fetch_bytes": round(fv),
                   "write_bytes": round(wv), "dispatches": len(fetch[k])}
+    meta_path = os.path.join(os.path.dirname(os.path.abspath(fetch_dir)), "pmc_meta.json")
+    if os.path.exists(meta_path) and "route_tile" in fetch and "route_tile" in write:
+        # the fused run's tile launches are the last ones: bytes per node-step
+        meta = load_json(meta_path)
+        n = meta["route_tile_fused_launches"]
+        fv = sum(fetch["route_tile"][-n:]) * f_scale * 1024.0
+        wv = sum(write["route_tile"][-n:]) * w_scale * 1024.0
+        res["route_tile_fused"] = {"bytes_per_node_step": (fv + wv) / meta["route_node_steps"],
+                                   "fetch_bytes": round(fv), "write_bytes": round(wv),
+                                   "dispatches": n, "node_steps": meta["route_node_steps"],
+                                   "window": meta["route_window"]}
     with open(out_path, "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res))
@@ -299,6 +328,9 @@ def summarize_flops(pmc_dir, meta_path, out_path):
         if parts:
             res[k] = {"flops_per_unit": tot / u, "unit": "cell-step",
                       "wave_insts_per_dispatch": parts}
+            valu = _per_kernel(rows, "SQ_INSTS_VALU").get(k)
+            if valu:
+                res[k]["valu_wave_insts_per_unit"] = sum(valu) / len(valu) / u
     with open(out_path, "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res))
@@ -375,7 +407,8 @@ def main():
     dominant = max(prof.items(), key=lambda kv: kv[1][0])[0]
     hbm = hbm_roofline(dominant, prof[dominant], S, 24, traffic)
     fpu = flops.get(dominant, {}).get("flops_per_unit")
-    fp64 = fp64_roofline(dominant, prof[dominant], fpu)
+    fp64 = fp64_roofline(dominant, prof[dominant], fpu,
+                         flops.get(dominant, {}).get("valu_wave_insts_per_unit"))
     roof = fp64 if (fp64 and dominant == "cell_fused") else hbm
     log(f"fused: {value:.4g} cell-steps/s  kernels={json.dumps(kernels)}")
 
