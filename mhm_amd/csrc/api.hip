@@ -70,6 +70,8 @@ struct mhmhip_ctx {
   int32_t *d_gauge_node = nullptr, *d_inflow_node = nullptr, *d_inflow_hw = nullptr;
   int32_t *d_node_gauge = nullptr;  // [n_cells] first gauge at the node, -1: none
   uint8_t *d_lag = nullptr;         // [n_cells] ring lag of each node (its tile slot's lag)
+  int32_t *d_slot_out = nullptr;    // [n_slots] LDS output rows of the tiled kernel
+  int32_t tile_out_rows = 0;        // most output rows of a bin
   std::vector<std::pair<int32_t, int32_t>> gauge_dup;  // (gauge, earlier gauge at its node)
   std::vector<int32_t> h_inflow_node_dev, h_inflow_hw;
   double *d_inflow_q = nullptr;
@@ -317,6 +319,7 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_gauge_node);
   (void)hipFree(x->d_node_gauge);
   (void)hipFree(x->d_lag);
+  (void)hipFree(x->d_slot_out);
   (void)hipFree(x->d_inflow_node);
   (void)hipFree(x->d_inflow_hw);
   (void)hipFree(x->d_inflow_q);
@@ -701,6 +704,8 @@ TileArgs tile_args(const mhmhip_ctx *x, int32_t B, int64_t nsteps) {
   g.slot = x->d_slots;
   g.bin = x->d_bins;
   g.node_gauge = x->d_node_gauge;
+  g.slot_out = x->d_slot_out;
+  g.out_rows = x->tile_out_rows;
   g.bnd = x->d_bnd;
   g.n_bnd = std::max<int64_t>(1, x->lay.n_bnd);
   g.tau_max = x->lay.tau_max;
@@ -1012,6 +1017,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->dfree(x->d_dom_dmax, x->n_domains);
   x->dfree(x->d_child_list, x->n_child_dev);
   x->dfree(x->d_slots, x->n_slots_dev);
+  x->dfree(x->d_slot_out, x->n_slots_dev);
   x->dfree(x->d_bins, x->n_bins_dev);
   x->dfree(x->d_bnd, x->n_bnd_dev);
   x->n_child_dev = x->n_slots_dev = x->n_bins_dev = x->n_bnd_dev = 0;
@@ -1032,12 +1038,33 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   // the slot flags carry the node flags (gauges, inflow, cut links set above)
   for (SlotMeta &m : lay.slots)
     if (m.a & SF_REAL) m.a |= lay.packed[m.pos] & NF_MASK;
+  // output rows of the tiled kernel, per bin: its tile roots, then its gauge
+  // nodes (the kernel stages their series in LDS)
+  std::vector<int32_t> slot_out(lay.slots.size(), -1);  // both fields 0xFFFF
+  int32_t out_rows = 0;
+  for (size_t b = 0; b < lay.bins.size(); ++b) {
+    int32_t n = 0;
+    for (int i = 0; i < kTileSlots; ++i)
+      if (lay.slots[b * kTileSlots + i].a & SF_TROOT)
+        slot_out[b * kTileSlots + i] = (int32_t)(0xFFFF0000u | (uint32_t)n++);
+    for (int i = 0; i < kTileSlots; ++i) {
+      const int32_t a = lay.slots[b * kTileSlots + i].a;
+      if ((a & SF_REAL) && (a & NF_GAUGE)) {
+        int32_t &v = slot_out[b * kTileSlots + i];
+        v = (int32_t)(((uint32_t)v & 0xFFFFu) | ((uint32_t)n++ << 16));
+      }
+    }
+    lay.bins[b].out_rows = n;
+    out_rows = std::max(out_rows, n);
+  }
   const int64_t n_child = std::max<int64_t>(1, (int64_t)lay.child_list.size());
   const int64_t n_bnd = 2 * (int64_t)x->n_sets * std::max<int64_t>(1, lay.n_bnd) * kTileWindowMax;
   CHECK_RC(x->dalloc(&x->d_child_list, n_child, "alloc child list"));
   x->n_child_dev = n_child;
   CHECK_RC(x->dalloc(&x->d_slots, (int64_t)lay.slots.size(), "alloc tiles"));
+  CHECK_RC(x->dalloc(&x->d_slot_out, (int64_t)lay.slots.size(), "alloc tiles"));
   x->n_slots_dev = (int64_t)lay.slots.size();
+  x->tile_out_rows = out_rows;
   CHECK_RC(x->dalloc(&x->d_bins, (int64_t)lay.bins.size(), "alloc tiles"));
   x->n_bins_dev = (int64_t)lay.bins.size();
   CHECK_RC(x->dalloc(&x->d_bnd, n_bnd, "alloc tile boundaries"));
@@ -1046,6 +1073,9 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
     CHECK_HIP(x, sync_copy(x, x->d_child_list, lay.child_list.data(),
                            sizeof(int32_t) * lay.child_list.size(), hipMemcpyHostToDevice),
               "upload child list");
+  CHECK_HIP(x, sync_copy(x, x->d_slot_out, slot_out.data(), sizeof(int32_t) * slot_out.size(),
+                         hipMemcpyHostToDevice),
+            "upload tiles");
   CHECK_HIP(x, sync_copy(x, x->d_slots, lay.slots.data(), sizeof(SlotMeta) * lay.slots.size(),
                          hipMemcpyHostToDevice),
             "upload tiles");

@@ -162,11 +162,14 @@ struct SlotMeta {
   int32_t b;
 };
 struct BinInfo {
-  int32_t height;  // tallest tile in the bin
-  int32_t tau_lo;  // smallest / largest tile depth in the bin
+  int32_t height;    // tallest tile in the bin
+  int32_t tau_lo;    // smallest / largest tile depth in the bin
   int32_t tau_hi;
-  int32_t pad;
+  int32_t out_rows;  // tile roots + gauge nodes of the bin (LDS output rows)
 };
+// LDS a tiled launch may use to stage its tile-root series and gauge sums;
+// past it the kernel stores them directly
+constexpr size_t kTileStageLds = 64 * 1024;
 
 struct RouteArgs {
   const NodeMeta *meta;    // [n_nodes]
@@ -199,12 +202,16 @@ struct TileArgs {
   const SlotMeta *slot;    // [n_bins * kTileSlots]
   const BinInfo *bin;      // [n_bins]
   const int32_t *node_gauge;  // [n_nodes] first gauge at the node (device index), -1: none
+  const int32_t *slot_out;    // [n_bins * kTileSlots] output rows: tile root (bits 0..15),
+                              // gauge (16..31), 0xFFFF: none
   double *bnd;             // [2][n_sets][n_bnd][bstride] tile-root outflow by window parity
   int64_t n_bnd;
   int32_t tau_max;         // deepest tile
   int32_t B;               // steps per window
   int32_t nwin;            // windows in [t_lo, t_hi)
   int32_t bstride;         // boundary series length (kTileWindowMax)
+  int32_t out_rows;        // largest BinInfo.out_rows
+  int32_t pad;
 };
 
 }  // namespace mhmhip

@@ -441,8 +441,17 @@ struct TileLane {
   uint32_t bv;          // byte offset of (set, node) in a row
   int32_t a, pos, set;
   int32_t gauge;        // first gauge at the node, -1: none
+  int32_t orow_t, orow_g;  // LDS output rows of the tile-root series / gauge sums, -1: none
   bool act, real, proxy;
   double qtin1, qtr1, c1, c2;
+  double qlast;         // qOUT of the run's last step (when it falls in this window)
+};
+
+// Output row of the tiled kernel's LDS staging: where its B values go.
+struct TileOutRow {
+  double *base;
+  int32_t stride;  // elements between consecutive steps
+  int32_t nb;      // steps to write
 };
 
 // One sweep over the bin's nd diagonals; NCW bounds the upstream slots summed
@@ -450,9 +459,13 @@ struct TileLane {
 // slots past its own count read a +0.0 slot, which leaves the running sum's
 // value unchanged).  LDS: qTR parity buffers [2][W], +0.0 at 2W and 3W, so
 // that buffer 1's offsets are buffer 0's plus W for upstream and zero slots
-// alike.
-template <int NCW>
-__device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, double *qtr, int nd) {
+// alike.  STAGE: the tile-root series and gauge sums go to LDS rows (out,
+// B values each) and leave after the sweep, so that the loop holds no global
+// store: stores count in the same counter as the loads in flight, and each
+// one would shorten the wait for a fetched value.
+template <int NCW, bool STAGE>
+__device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, double *qtr, int nd,
+                                           double *out, int B) {
   constexpr int W = kTileSlots;
   const int s = threadIdx.x;
   uint32_t up[NCW];  // byte offsets in buffer 0
@@ -460,7 +473,7 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
 #pragma unroll
   for (int c = 0; c < NCW; ++c) up[c] = c < u.nc ? (uint32_t)(u.c0 + c) * 8u : zoff;
   const bool link = (u.a & NF_LINK) != 0, qflag = (u.a & NF_QOUT) != 0, cut = (u.a & NF_CUT) != 0;
-  const bool rare = u.bout != nullptr || u.gauge >= 0 || r.qOUT != nullptr;
+  const bool rare = u.bout != nullptr || u.gauge >= 0;
   const char *qtr_b = reinterpret_cast<const char *>(qtr);
   // Value for diagonal df: a real slot's qOUT sits in ring row df of the
   // window (the ring is written lag rows late), the same row for every real
@@ -479,6 +492,17 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
 #pragma unroll
   for (int i = 0; i < kTileAhead; ++i) qq[i] = 0.0;
   double musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
+  // STAGE: a diagonal's outputs are written to LDS on the next diagonal, after
+  // its upstream reads are issued, not before the barrier that ends it
+  double pq = 0.0, ps = 0.0;
+  int pj = -1;
+  // rows are B + 1 apart: the lanes of a wave write different banks
+  auto put_rows = [&]() {
+    if (pj >= 0) {
+      if (u.orow_t >= 0) out[u.orow_t * (B + 1) + pj] = pq;
+      if (u.orow_g >= 0) out[u.orow_g * (B + 1) + pj] = ps;
+    }
+  };
   // Diagonal d reads parity buffer (d + 1) & 1, written on diagonal d - 1 by
   // the upstream slots (one level deeper, one diagonal earlier, same step),
   // and writes buffer d & 1.  A slot writes every diagonal: an inactive
@@ -490,6 +514,7 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
     double v[NCW];
 #pragma unroll
     for (int c = 0; c < NCW; ++c) v[c] = *reinterpret_cast<const double *>(rb_b + up[c]);
+    if (STAGE) put_rows();
     // upstream qTR in routing rank
     double sum = 0.0;
 #pragma unroll
@@ -507,12 +532,16 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
     if (upd && link) u.qtr1 = q;
     if (upd) u.qtin1 = sum;
     musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
-    // at most three stores, none in a loop: the loads in flight stay countable
-    if (upd && rare) {
+    if (upd && u.t0 + j == r.t_hi - 1) u.qlast = qo;
+    if (STAGE) {
+      pj = upd && rare ? j : -1;
+      pq = q;
+      ps = sum;
+    } else if (upd && rare) {
+      // at most two stores, none in a loop
       if (u.bout) u.bout[j] = q;
-      const int64_t t = u.t0 + j;
-      if (r.qOUT && t == r.t_hi - 1) str(r.qOUT, u.bv, qo);
-      if (u.gauge >= 0) r.gauge_out[((t - r.t_base) * r.n_sets + u.set) * r.n_gauges + u.gauge] = sum;
+      if (u.gauge >= 0)
+        r.gauge_out[((u.t0 + j - r.t_base) * r.n_sets + u.set) * r.n_gauges + u.gauge] = sum;
     }
   };
   // The first pass only fetches.  Fetches stay in diagonal order, one per
@@ -530,11 +559,18 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
       qq[i] = fetch(min(d + kTileAhead, nd - 1));
     }
   }
+  if (STAGE) {
+    put_rows();
+    lds_barrier();
+  }
 }
 
+// LDS: qTR parity buffers [2][W], +0.0 at 2W and 3W, then (STAGE) the bin's
+// output rows: g.out_rows TileOutRow, then g.out_rows x (B + 1) values.
+template <bool STAGE>
 __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, TileArgs g,
                                                                 int32_t L, int32_t bin0) {
-  extern __shared__ double lds[];  // qTR parity buffers [2][W], +0.0 at 2W and 3W
+  extern __shared__ double lds[];
   constexpr int W = kTileSlots;
   const int bin = bin0 + blockIdx.x, set = blockIdx.y, s = threadIdx.x;
   const BinInfo bi = g.bin[bin];
@@ -581,17 +617,46 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
   u.bout = (u.act && (m.a & SF_TROOT)) ? g.bnd + (bseries + m.bnd) * g.bstride : nullptr;
   // gauges: the node's first gauge only (the host copies it to the others)
   u.gauge = (u.act && u.real && (m.a & NF_GAUGE)) ? g.node_gauge[m.pos] : -1;
+  u.orow_t = u.orow_g = -1;
+  u.qlast = 0.0;
   double *qtr = lds;
+  TileOutRow *rows = reinterpret_cast<TileOutRow *>(lds + 3 * W + 1);
+  double *out = reinterpret_cast<double *>(rows + g.out_rows);
+  if (STAGE) {
+    // the slot's output rows (host-numbered inside the bin); an idle owner
+    // leaves its row empty
+    const int32_t so = g.slot_out[(int64_t)bin * W + s];
+    const int32_t rt = so & 0xFFFF, rg = (so >> 16) & 0xFFFF;
+    if (rt != 0xFFFF) {
+      u.orow_t = rt;
+      rows[rt] = TileOutRow{u.bout ? u.bout : g.bnd, 1, u.bout ? u.nb : 0};
+    }
+    if (rg != 0xFFFF) {
+      u.orow_g = rg;
+      const int64_t gi = ((u.t0 - r.t_base) * r.n_sets + set) * r.n_gauges + u.gauge;
+      rows[rg] = TileOutRow{u.gauge >= 0 ? r.gauge_out + gi : g.bnd, r.n_sets * r.n_gauges,
+                            u.gauge >= 0 ? u.nb : 0};
+    }
+  }
   if (s == 0) qtr[2 * W] = qtr[3 * W] = 0.0;
   lds_barrier();
   const int nd = bi.height + g.B - 1;
-  if (__any(u.nc > 5)) tile_sweep<8>(r, u, qtr, nd);
-  else if (__any(u.nc > 3)) tile_sweep<5>(r, u, qtr, nd);
-  else if (__any(u.nc > 1)) tile_sweep<3>(r, u, qtr, nd);
-  else tile_sweep<1>(r, u, qtr, nd);
+  if (__any(u.nc > 5)) tile_sweep<8, STAGE>(r, u, qtr, nd, out, g.B);
+  else if (__any(u.nc > 3)) tile_sweep<5, STAGE>(r, u, qtr, nd, out, g.B);
+  else if (__any(u.nc > 1)) tile_sweep<3, STAGE>(r, u, qtr, nd, out, g.B);
+  else tile_sweep<1, STAGE>(r, u, qtr, nd, out, g.B);
+  if (STAGE) {
+    // the sweep ended with a barrier after the last rows: they are complete
+    for (int idx = s; idx < bi.out_rows * g.B; idx += W) {
+      const int rr = idx / g.B, j = idx - rr * g.B;
+      const TileOutRow o = rows[rr];
+      if (j < o.nb) o.base[(int64_t)j * o.stride] = out[rr * (g.B + 1) + j];
+    }
+  }
   if (u.act && u.real) {
     str(r.qTIN, u.bv, u.qtin1);
     if (m.a & NF_LINK) str(r.pub + ((u.t0 + u.nb - 1) & 1) * ld, u.bv, u.qtr1);
+    if (r.qOUT && r.t_hi - 1 - u.t0 < u.nb) str(r.qOUT, u.bv, u.qlast);
   }
 }
 
@@ -686,8 +751,14 @@ hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_
       (uint64_t)r.n_sets * (uint64_t)r.sstride * 8u > 0xFFFFFFFFull)
     return hipErrorInvalidValue;
   dim3 grid((unsigned)(bin_hi - bin_lo), (unsigned)r.n_sets);
-  const size_t shm = sizeof(double) * (3 * (size_t)kTileSlots + 1);
-  hipLaunchKernelGGL(route_tile_kernel, grid, dim3(kTileSlots), shm, s, r, g, L, (int32_t)bin_lo);
+  const size_t base = sizeof(double) * (3 * (size_t)kTileSlots + 1);
+  const size_t stage = (size_t)g.out_rows * (sizeof(TileOutRow) + sizeof(double) * (g.B + 1));
+  if (base + stage <= kTileStageLds)
+    hipLaunchKernelGGL(route_tile_kernel<true>, grid, dim3(kTileSlots), base + stage, s, r, g, L,
+                       (int32_t)bin_lo);
+  else
+    hipLaunchKernelGGL(route_tile_kernel<false>, grid, dim3(kTileSlots), base, s, r, g, L,
+                       (int32_t)bin_lo);
   return hipGetLastError();
 }
 

@@ -76,7 +76,7 @@ def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_step
     ring_rows = min((plan.tau_max + 1) * window + chunk_steps + 1, rows_run)
     ring = (-(-ring_rows // window) * window + TILE_HEIGHT) * ld * 8
     routing = (21 * n_cells + 6 * ld * 8 + 20 + 4 * max(1, n_cells - n_outlets)
-               + 16 * plan.n_slots + 16 * plan.n_bins
+               + 20 * plan.n_slots + 16 * plan.n_bins
                + 2 * n_sets * max(1, plan.n_bnd) * TILE_WINDOW_MAX * 8
                + 4 * max(1, n_gauges) + 8)
     series = rows_run * 96 + rows_run * n_sets * max(1, n_gauges) * 8
