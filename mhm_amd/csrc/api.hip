@@ -41,6 +41,7 @@ struct mhmhip_ctx {
   std::vector<int64_t> dom_cell_start;
   MonthTables mt{};
   hipStream_t stream = nullptr;
+  hipStream_t rstream = nullptr;  // routing stream of a captured run (default priority)
   std::string err;
 
   double *P = nullptr, *S = nullptr, *X = nullptr, *F = nullptr;
@@ -119,6 +120,7 @@ struct mhmhip_ctx {
   // every replay.
   bool capturing = false;
   std::vector<Pending> graph_events;
+  std::vector<hipEvent_t> graph_sync;  // cross-stream edges of the captured graph
   int64_t graph_all[MHMHIP_K_COUNT] = {};
   bool prof = false;
   int prof_stride = 1;         // time every prof_stride-th routing launch
@@ -333,6 +335,7 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   }
   for (hipEvent_t e : x->ev_pool) (void)hipEventDestroy(e);
   drop_graph(x);
+  if (x->rstream) (void)hipStreamDestroy(x->rstream);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return MHMHIP_OK;
@@ -727,7 +730,8 @@ hipError_t ev_get(mhmhip_ctx *x, hipEvent_t *e) {
 // Launches f() on the context stream; with profiling on, brackets it with
 // events (routing launches: every prof_stride-th one).
 template <typename Fn>
-hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
+hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f, hipStream_t s = nullptr) {
+  if (!s) s = x->stream;
   if (!x->prof) return f();
   (x->capturing ? x->graph_all : x->prof_all)[kind] += 1;
   // routing launches come by the hundred or thousand: time one in prof_stride
@@ -749,9 +753,9 @@ hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f) {
   // under capture a plain record only marks a dependency; the external flag
   // makes it an event-record node that timestamps every replay
   const unsigned flags = x->capturing ? hipEventRecordExternal : 0u;
-  if ((e = hipEventRecordWithFlags(p.a, x->stream, flags)) != hipSuccess) return e;
+  if ((e = hipEventRecordWithFlags(p.a, s, flags)) != hipSuccess) return e;
   if ((e = f()) != hipSuccess) return e;
-  if ((e = hipEventRecordWithFlags(p.b, x->stream, flags)) != hipSuccess) return e;
+  if ((e = hipEventRecordWithFlags(p.b, s, flags)) != hipSuccess) return e;
   if (!x->capturing) x->pending.push_back(p);
   return hipSuccess;
 }
@@ -828,17 +832,31 @@ int route_diagonals(mhmhip_ctx *x, const RouteArgs &r, int64_t d_lo, int64_t d_h
 // Tiled routing launches [L_lo, L_hi) of a call.
 // Launch L handles window k = L - (tau_max - tau) of each tile: only the bins
 // holding tiles of depth tau_max - L .. tau_max - L + nwin - 1 are launched.
-int route_tiles(mhmhip_ctx *x, const RouteArgs &r, const TileArgs &g, int32_t L_lo, int32_t L_hi) {
+int route_tiles(mhmhip_ctx *x, const RouteArgs &r, const TileArgs &g, int32_t L_lo, int32_t L_hi,
+                hipStream_t s = nullptr) {
+  if (!s) s = x->stream;
   const int64_t nb = (int64_t)x->lay.bins.size();
   for (int32_t L = L_lo; L < L_hi; ++L) {
     int64_t lo, hi;
     tile_bin_range(x->lay, g.tau_max - L, g.tau_max - L + g.nwin - 1, lo, hi);
     if (hi <= lo) continue;
     CHECK_HIP(x, timed(x, MHMHIP_K_ROUTE_TILE, 0.0, [&] {
-                return launch_route_tile(r, g, lo, hi, nb, L, x->stream);
-              }),
+                return launch_route_tile(r, g, lo, hi, nb, L, s);
+              }, s),
               "route_tile");
   }
+  return MHMHIP_OK;
+}
+
+// Under capture: an edge from what stream `from` has enqueued so far to
+// what `to` enqueues next (the event is kept until the graph is dropped).
+int capture_edge(mhmhip_ctx *x, hipStream_t from, hipStream_t to, hipEvent_t *out = nullptr) {
+  hipEvent_t e = nullptr;
+  CHECK_HIP(x, hipEventCreateWithFlags(&e, hipEventDisableTiming), "capture edge");
+  x->graph_sync.push_back(e);
+  CHECK_HIP(x, hipEventRecord(e, from), "capture edge");
+  if (to) CHECK_HIP(x, hipStreamWaitEvent(to, e, 0), "capture edge");
+  if (out) *out = e;
   return MHMHIP_OK;
 }
 
@@ -874,6 +892,8 @@ void drop_graph(mhmhip_ctx *x) {
     (void)hipEventDestroy(p.b);
   }
   x->graph_events.clear();
+  for (hipEvent_t e : x->graph_sync) (void)hipEventDestroy(e);
+  x->graph_sync.clear();
   for (int k = 0; k < MHMHIP_K_COUNT; ++k) x->graph_all[k] = 0;
   x->graph_exec = nullptr;
   x->graph = nullptr;
@@ -1330,11 +1350,33 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   }
   const TileArgs g = tile_args(x, B, nsteps);
   const int32_t L_end = g.nwin + g.tau_max;
+  const bool graph = o.use_graph && o.mode == MHMHIP_RUN_FUSED;
+  // A captured fused run puts the tiled routing on a stream of its own: a
+  // routing launch waits for the cell launch that completes its window, and
+  // the cell launches run on while the routing of earlier windows proceeds.
+  // When the ring is shorter than the call, a cell launch also waits for the
+  // routing launch that last reads the rows it overwrites.
+  const bool two = graph && route && kind == MHMHIP_K_ROUTE_TILE;
+  if (two && !x->rstream)
+    CHECK_HIP(x, hipStreamCreateWithFlags(&x->rstream, hipStreamNonBlocking), "routing stream");
+  const bool wrap = ring_rows < nsteps;
   // the whole loop, in steps relative to this call (replayable as a graph)
   auto enqueue = [&]() -> int {
     int32_t L = 0;  // next tiled routing launch
+    hipStream_t rs = two ? x->rstream : x->stream;
+    std::vector<hipEvent_t> evL(two && wrap ? L_end : 0, nullptr);  // routing launch L done
+    if (two) CHECK_RC(capture_edge(x, x->stream, rs));  // the routing stream joins the capture
     for (int32_t k = 0; k < nsteps; k += S) {
       const int n = std::min<int32_t>(S, nsteps - k);
+      if (two && wrap && (int64_t)k + n - 1 - ring_rows >= 0) {
+        // rows of the steps up to k + n - 1 - R are overwritten: the last
+        // routing launch reading them is enqueued already
+        const int64_t Ln = ((int64_t)k + n - 1 - ring_rows) / B + g.tau_max;
+        if (Ln >= L) return x->fail(MHMHIP_EINVAL, "run: ring shorter than the routing lag");
+        int64_t Lw = Ln;
+        while (Lw >= 0 && !evL[Lw]) --Lw;  // a launch with no bin due reads nothing
+        if (Lw >= 0) CHECK_HIP(x, hipStreamWaitEvent(x->stream, evL[Lw], 0), "capture edge");
+      }
       if (o.mode == MHMHIP_RUN_PLAIN) {
         CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
                     return launch_cell_step(x->nh, a, hs[k], k, block, x->n_sets, x->stream);
@@ -1354,17 +1396,34 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
         // launch L needs the steps of window L (its deepest tiles' window)
         int32_t L1 = L;
         while (L1 < L_end && std::min<int64_t>((int64_t)(L1 + 1) * B, nsteps) <= k + n) ++L1;
-        CHECK_RC(route_tiles(x, r, g, L, L1));
+        if (two && L1 > L) {
+          CHECK_RC(capture_edge(x, x->stream, rs));
+          for (int32_t l = L; l < L1; ++l) {
+            CHECK_RC(route_tiles(x, r, g, l, l + 1, rs));
+            if (wrap) CHECK_RC(capture_edge(x, rs, nullptr, &evL[l]));
+          }
+        } else if (!two) {
+          CHECK_RC(route_tiles(x, r, g, L, L1));
+        }
         L = L1;
       } else {
         CHECK_RC(route_diagonals(x, r, k, k + n, false, wg));
       }
     }
-    if (route && kind == MHMHIP_K_ROUTE_TILE) CHECK_RC(route_tiles(x, r, g, L, L_end));
+    if (route && kind == MHMHIP_K_ROUTE_TILE) {
+      if (two) {
+        if (L < L_end) {
+          CHECK_RC(capture_edge(x, x->stream, rs));
+          CHECK_RC(route_tiles(x, r, g, L, L_end, rs));
+        }
+        CHECK_RC(capture_edge(x, rs, x->stream));  // join
+      } else {
+        CHECK_RC(route_tiles(x, r, g, L, L_end));
+      }
+    }
     if (route && kind != MHMHIP_K_ROUTE_TILE) CHECK_RC(route_diagonals(x, r, nsteps, 0, true, wg));
     return MHMHIP_OK;
   };
-  const bool graph = o.use_graph && o.mode == MHMHIP_RUN_FUSED;
   if (graph) {
     mhmhip_ctx::GraphKey key;
     key.nsteps = nsteps;

@@ -117,6 +117,28 @@ def test_graph_replays_match_eager_and_oracle():
     m2.close()
 
 
+@pytest.mark.parametrize("n_steps,S,window", [(200, 24, 8), (150, 7, 16), (96, 24, 64)])
+def test_graph_two_streams_match_eager(n_steps, S, window):
+    """A captured run routes on a stream of its own; with a ring shorter than
+    the run (the first two cases) cell launches also wait for the routing
+    that last reads the rows they overwrite.  Graph, eager and oracle agree
+    on every replay."""
+    _log(f"case: graph two streams n_steps={n_steps} S={S} window={window}")
+    dom = synth.make_domain(30, 26, seed=22)
+    m1, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=2 * n_steps, seed=9)
+    m2, _, _, _ = make_pair([dom], n_sets=2, n_steps=2 * n_steps, seed=9)
+    for a in (0, n_steps):
+        kw = dict(mode=1, steps_per_launch=S, route_mode=3, route_window=window)
+        g1 = m1.run(steps[a:a + n_steps], **kw)
+        g2 = m2.run(steps[a:a + n_steps], use_graph=True, **kw)
+        go = _oracle_series(sim, recs[a:a + n_steps])
+        assert np.array_equal(g1, g2), a
+        assert rel_close(g2, go, RTOL, ATOL)[0], a
+        assert compare_routing(m2, sim, RTOL, ATOL) == [], a
+    m1.close()
+    m2.close()
+
+
 def test_profiling_inside_graph_replays():
     """Kernel timing recorded as graph nodes: every replay adds its launches."""
     _log("case: graph profiling")
