@@ -66,6 +66,23 @@ def test_route_windows_and_shared_gauge_nodes(mode, window, S):
     model.close()
 
 
+@pytest.mark.parametrize("window", [8, 64])
+def test_gauge_at_every_node(window):
+    """Every node a gauge: at window 8 the tiled kernel stages hundreds of
+    output rows per bin in LDS; at window 64 they would not fit and the
+    kernel stores them directly.  Both agree with the oracle."""
+    _log(f"case: gauge at every node window={window}")
+    dom = synth.make_domain(16, 16, seed=23)
+    dom.gauge_nodes = np.arange(1, dom.n_cells + 1, dtype=np.int32)
+    model, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=40, seed=10)
+    g = model.run(steps, mode=1, steps_per_launch=8, route_mode=3, route_window=window)
+    go = _oracle_series(sim, recs)
+    ok, worst = rel_close(g, go, RTOL, ATOL)
+    assert ok, worst
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    model.close()
+
+
 @pytest.mark.parametrize("mode,S,route_mode", [(0, 0, 1), (1, 24, 1), (1, 5, 2), (1, 48, 2),
                                                (0, 0, 3), (1, 24, 3), (1, 5, 3), (1, 48, 3)])
 def test_run_matches_oracle(mode, S, route_mode):
