@@ -207,3 +207,37 @@ def test_process_variants_match_oracle(soil, pet, timestep):
     assert rel_close(g, go, RTOL, ATOL)[0]
     assert compare_cells(model, sim, RTOL, ATOL) == []
     model.close()
+
+
+@pytest.mark.parametrize("route_mode", [1, 3])
+def test_restart_round_trip(route_mode, tmp_path):
+    """Suspend and resume (the restart variable set: cell states and the
+    routing's qTIN/qTR): a run continued after its states went through a
+    file and into a fresh context equals the run that never stopped, bit
+    for bit (mo_restart.f90 / mo_mrm_restart.f90 hold the same set)."""
+    _log(f"case: restart round trip route_mode={route_mode}")
+    dom = synth.make_domain(24, 20, seed=24)
+    m1, _, steps, _ = make_pair([dom], n_sets=2, n_steps=60, seed=11)
+    kw = dict(mode=1, steps_per_launch=8, route_mode=route_mode)
+    m1.run(steps[:36], **kw)
+    snap = m1.snapshot()
+    path = tmp_path / "restart.npz"
+    flat = {f"cells_{s}_{k}": v for s, c in enumerate(snap["cells"]) for k, v in c.items()}
+    flat.update({f"routing_{s}_{k}": v for s, r in enumerate(snap["routing"]) for k, v in r.items()})
+    np.savez(path, **flat)
+    g1 = m1.run(steps[36:], **kw)
+    m2, _, _, _ = make_pair([dom], n_sets=2, n_steps=60, seed=11)
+    z = np.load(path)
+    back = {"cells": [{k: z[f"cells_{s}_{k}"] for k in m2.STATE_NAMES} for s in range(2)],
+            "routing": [{k: z[f"routing_{s}_{k}"] for k in ("qTIN", "qTR")} for s in range(2)]}
+    m2.restore(back)
+    g2 = m2.run(steps[36:], **kw)
+    assert np.array_equal(g1, g2)
+    for s in range(2):
+        for name in m1.STATE_NAMES:
+            assert np.array_equal(m1.get(name, s), m2.get(name, s)), (s, name)
+        a, b = m1.routing_state(s), m2.routing_state(s)
+        for k in ("qTIN", "qTR", "Qmod", "qOUT"):
+            assert np.array_equal(a[k], b[k]), (s, k)
+    m1.close()
+    m2.close()
