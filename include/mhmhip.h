@@ -260,6 +260,15 @@ int mhmhip_set_inflow(mhmhip_ctx *ctx, int32_t nsteps, const double *q);
  * C1, C2.  Any pointer may be NULL. */
 int mhmhip_get_routing_state(mhmhip_ctx *ctx, int32_t set, double *qOUT, double *qTIN,
                              double *qTR, double *Qmod, double *C1, double *C2);
+/* Output aggregation of L11_Qmod on the device (replaces the host-side
+ * updateVariable / writeVariableTimestep of src/common/mo_nc_output.f90:
+ * 140-175, called per step from mo_mhm_interface_run.f90:650-667): while on,
+ * every routed step adds each node's Qmod to a per-node sum, in step order.
+ * on = 1 starts (sums and count zeroed), on = 0 stops. */
+int mhmhip_qmod_accumulate(mhmhip_ctx *ctx, int32_t on);
+/* The sums of one set per node in reference order and the number of steps
+ * added; sum / count is the average writeVariableTimestep writes. */
+int mhmhip_get_qmod_sum(mhmhip_ctx *ctx, int32_t set, double *sum, int64_t *count);
 int mhmhip_set_routing_state(mhmhip_ctx *ctx, int32_t set, const double *qTIN,
                              const double *qTR);
 

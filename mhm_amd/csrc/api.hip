@@ -72,6 +72,9 @@ struct mhmhip_ctx {
   int32_t *d_node_gauge = nullptr;  // [n_cells] first gauge at the node, -1: none
   uint8_t *d_lag = nullptr;         // [n_cells] ring lag of each node (its tile slot's lag)
   int32_t *d_slot_out = nullptr;    // [n_slots] LDS output rows of the tiled kernel
+  double *d_qacc = nullptr;         // [ld] Qmod summed over routed steps (output aggregation)
+  bool qacc_on = false;
+  int64_t qacc_count = 0;
   int32_t tile_out_rows = 0;        // most output rows of a bin
   std::vector<std::pair<int32_t, int32_t>> gauge_dup;  // (gauge, earlier gauge at its node)
   std::vector<int32_t> h_inflow_node_dev, h_inflow_hw;
@@ -97,12 +100,13 @@ struct mhmhip_ctx {
   // time loop captured as a hipGraph, replayed while its key matches
   struct GraphKey {
     int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1, prof = -1, B = -1;
-    const void *ring = nullptr, *gauge = nullptr, *steps = nullptr, *inflow = nullptr;
+    const void *ring = nullptr, *gauge = nullptr, *steps = nullptr, *inflow = nullptr,
+               *qacc = nullptr;
     int64_t ring_len = 0;
     bool operator==(const GraphKey &o) const {
       return nsteps == o.nsteps && mode == o.mode && S == o.S && block == o.block && wg == o.wg &&
              prof == o.prof && B == o.B && ring == o.ring && gauge == o.gauge && steps == o.steps &&
-             inflow == o.inflow && ring_len == o.ring_len;
+             inflow == o.inflow && qacc == o.qacc && ring_len == o.ring_len;
     }
   };
   GraphKey gkey;
@@ -322,6 +326,7 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_node_gauge);
   (void)hipFree(x->d_lag);
   (void)hipFree(x->d_slot_out);
+  (void)hipFree(x->d_qacc);
   (void)hipFree(x->d_inflow_node);
   (void)hipFree(x->d_inflow_hw);
   (void)hipFree(x->d_inflow_q);
@@ -645,6 +650,7 @@ RouteArgs route_args(mhmhip_ctx *x, int64_t t_lo, int64_t t_hi, int64_t t_base) 
   r.qTIN = x->d_qTIN;
   r.pub = x->d_pub;
   r.qOUT = x->d_qOUT;
+  r.qacc = x->qacc_on ? x->d_qacc : nullptr;
   r.ring = x->d_ring;
   r.gauge_out = x->d_gauge_buf;
   r.gauge_node = x->d_gauge_node;
@@ -1044,6 +1050,9 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->dfree(x->d_gauge_node, std::max(1, x->n_gauges));
   x->dfree(x->d_node_gauge, x->n_cells);
   x->dfree(x->d_lag, x->n_cells);
+  x->dfree(x->d_qacc, x->ld);  // a new network restarts the output aggregation
+  x->qacc_on = false;
+  x->qacc_count = 0;
   x->dfree(x->d_inflow_node, std::max(1, x->n_inflow));
   x->dfree(x->d_inflow_hw, std::max(1, x->n_inflow));
   CHECK_RC(x->dalloc(&x->d_meta, x->n_cells, "alloc meta"));
@@ -1251,6 +1260,38 @@ extern "C" int mhmhip_get_routing_state(mhmhip_ctx *x, int32_t set, double *qOUT
   return MHMHIP_OK;
 }
 
+extern "C" int mhmhip_qmod_accumulate(mhmhip_ctx *x, int32_t on) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "qmod_accumulate: no network bound");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "qmod_accumulate");
+  x->qacc_on = on != 0;
+  if (!x->qacc_on) return MHMHIP_OK;
+  if (!x->d_qacc) {
+    CHECK_RC(x->dalloc(&x->d_qacc, x->ld, "alloc Qmod sums"));  // zeroed
+  } else {
+    CHECK_HIP(x, hipMemsetAsync(x->d_qacc, 0, sizeof(double) * x->ld, x->stream), "qmod_accumulate");
+  }
+  x->qacc_count = 0;
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_get_qmod_sum(mhmhip_ctx *x, int32_t set, double *sum, int64_t *count) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!x->d_qacc) return x->fail(MHMHIP_ESTATE, "get_qmod_sum: accumulation never started");
+  if (set < 0 || set >= x->n_sets) return x->fail(MHMHIP_EINVAL, "get_qmod_sum: bad set");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "get_qmod_sum");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "get_qmod_sum");
+  if (sum) {
+    std::vector<double> buf(x->sstride);
+    CHECK_HIP(x, sync_copy(x, buf.data(), x->d_qacc + (int64_t)set * x->sstride,
+                           sizeof(double) * x->sstride, hipMemcpyDeviceToHost),
+              "download");
+    for (int64_t k = 0; k < x->n_cells; ++k) sum[k] = buf[x->pos[k]];
+  }
+  if (count) *count = x->qacc_count;
+  return MHMHIP_OK;
+}
+
 extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double *qTIN,
                                         const double *qTR) {
   if (!x) return MHMHIP_EINVAL;
@@ -1289,6 +1330,7 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   CHECK_RC(route_tiles(x, r, g, 0, g.nwin + g.tau_max));
   credit_route(x, kind, 1);
   x->qtr_slot = 0;
+  if (x->qacc_on) x->qacc_count += 1;
   x->t_route++;
   if (gauge_out && x->n_gauges > 0) {
     CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf,
@@ -1438,6 +1480,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     key.inflow = x->d_inflow_q;
     key.ring_len = ring_rows;
     key.B = B;
+    key.qacc = r.qacc;
     if (!x->graph_exec || !(key == x->gkey)) {
       drop_graph(x);
       CHECK_HIP(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -1462,6 +1505,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   if (route) {
     credit_route(x, kind, nsteps);
     x->qtr_slot = (int)((nsteps - 1) & 1);
+    if (x->qacc_on) x->qacc_count += nsteps;
   }
   x->t_cell += nsteps;
   x->t_route = x->t_cell;

@@ -179,6 +179,7 @@ struct RouteArgs {
   double *qTIN;            // [ld]   qTIN(:,IT1)
   double *pub;             // [2][ld] qTR by step parity
   double *qOUT;            // [ld]   last qOUT (reference L11_qOUT), may be null
+  double *qacc;            // [ld]   Qmod summed over the routed steps, null: off
   const double *ring;      // [ring_len + kTileHeight][ld] (ring_row)
   double *gauge_out;       // [nsteps][n_sets][n_gauges] relative to t_base
   const int32_t *gauge_node;  // [n_gauges] device node index

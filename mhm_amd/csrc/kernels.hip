@@ -365,6 +365,7 @@ __device__ __forceinline__ void route_node(const RouteArgs &r, uint32_t p, int32
     str(r.pub + (t & 1) * ld, bv, qtr);
   }
   str(r.qTIN, bv, s);
+  if (r.qacc) str(r.qacc, bv, ldr(r.qacc, bv) + s);
   if (y & NF_GAUGE) {
     for (int32_t g = 0; g < r.n_gauges; ++g)
       if (r.gauge_node[g] == (int32_t)p)
@@ -445,6 +446,7 @@ struct TileLane {
   bool act, real, proxy;
   double qtin1, qtr1, c1, c2;
   double qlast;         // qOUT of the run's last step (when it falls in this window)
+  double qacc;          // Qmod summed over the steps (ACC)
 };
 
 // Output row of the tiled kernel's LDS staging: where its B values go.
@@ -463,7 +465,7 @@ struct TileOutRow {
 // B values each) and leave after the sweep, so that the loop holds no global
 // store: stores count in the same counter as the loads in flight, and each
 // one would shorten the wait for a fetched value.
-template <int NCW, bool STAGE>
+template <int NCW, bool STAGE, bool ACC>
 __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, double *qtr, int nd,
                                            double *out, int B) {
   constexpr int W = kTileSlots;
@@ -531,6 +533,7 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
     const bool upd = on && u.real;
     if (upd && link) u.qtr1 = q;
     if (upd) u.qtin1 = sum;
+    if (ACC && upd) u.qacc = u.qacc + sum;  // Qmod of step j, added in step order
     musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
     if (upd && u.t0 + j == r.t_hi - 1) u.qlast = qo;
     if (STAGE) {
@@ -567,7 +570,8 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
 
 // LDS: qTR parity buffers [2][W], +0.0 at 2W and 3W, then (STAGE) the bin's
 // output rows: g.out_rows TileOutRow, then g.out_rows x (B + 1) values.
-template <bool STAGE>
+// ACC: Qmod summed per node (r.qacc).
+template <bool STAGE, bool ACC>
 __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, TileArgs g,
                                                                 int32_t L, int32_t bin0) {
   extern __shared__ double lds[];
@@ -619,6 +623,7 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
   u.gauge = (u.act && u.real && (m.a & NF_GAUGE)) ? g.node_gauge[m.pos] : -1;
   u.orow_t = u.orow_g = -1;
   u.qlast = 0.0;
+  u.qacc = ACC && u.act && u.real ? ldr(r.qacc, u.bv) : 0.0;
   double *qtr = lds;
   TileOutRow *rows = reinterpret_cast<TileOutRow *>(lds + 3 * W + 1);
   double *out = reinterpret_cast<double *>(rows + g.out_rows);
@@ -641,10 +646,10 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
   if (s == 0) qtr[2 * W] = qtr[3 * W] = 0.0;
   lds_barrier();
   const int nd = bi.height + g.B - 1;
-  if (__any(u.nc > 5)) tile_sweep<8, STAGE>(r, u, qtr, nd, out, g.B);
-  else if (__any(u.nc > 3)) tile_sweep<5, STAGE>(r, u, qtr, nd, out, g.B);
-  else if (__any(u.nc > 1)) tile_sweep<3, STAGE>(r, u, qtr, nd, out, g.B);
-  else tile_sweep<1, STAGE>(r, u, qtr, nd, out, g.B);
+  if (__any(u.nc > 5)) tile_sweep<8, STAGE, ACC>(r, u, qtr, nd, out, g.B);
+  else if (__any(u.nc > 3)) tile_sweep<5, STAGE, ACC>(r, u, qtr, nd, out, g.B);
+  else if (__any(u.nc > 1)) tile_sweep<3, STAGE, ACC>(r, u, qtr, nd, out, g.B);
+  else tile_sweep<1, STAGE, ACC>(r, u, qtr, nd, out, g.B);
   if (STAGE) {
     // the sweep ended with a barrier after the last rows: they are complete
     for (int idx = s; idx < bi.out_rows * g.B; idx += W) {
@@ -657,6 +662,7 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
     str(r.qTIN, u.bv, u.qtin1);
     if (m.a & NF_LINK) str(r.pub + ((u.t0 + u.nb - 1) & 1) * ld, u.bv, u.qtr1);
     if (r.qOUT && r.t_hi - 1 - u.t0 < u.nb) str(r.qOUT, u.bv, u.qlast);
+    if (ACC) str(r.qacc, u.bv, u.qacc);
   }
 }
 
@@ -753,11 +759,19 @@ hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_
   dim3 grid((unsigned)(bin_hi - bin_lo), (unsigned)r.n_sets);
   const size_t base = sizeof(double) * (3 * (size_t)kTileSlots + 1);
   const size_t stage = (size_t)g.out_rows * (sizeof(TileOutRow) + sizeof(double) * (g.B + 1));
-  if (base + stage <= kTileStageLds)
-    hipLaunchKernelGGL(route_tile_kernel<true>, grid, dim3(kTileSlots), base + stage, s, r, g, L,
+  const bool st = base + stage <= kTileStageLds;
+  const size_t shm = st ? base + stage : base;
+  if (st && r.qacc)
+    hipLaunchKernelGGL((route_tile_kernel<true, true>), grid, dim3(kTileSlots), shm, s, r, g, L,
+                       (int32_t)bin_lo);
+  else if (st)
+    hipLaunchKernelGGL((route_tile_kernel<true, false>), grid, dim3(kTileSlots), shm, s, r, g, L,
+                       (int32_t)bin_lo);
+  else if (r.qacc)
+    hipLaunchKernelGGL((route_tile_kernel<false, true>), grid, dim3(kTileSlots), shm, s, r, g, L,
                        (int32_t)bin_lo);
   else
-    hipLaunchKernelGGL(route_tile_kernel<false>, grid, dim3(kTileSlots), base, s, r, g, L,
+    hipLaunchKernelGGL((route_tile_kernel<false, false>), grid, dim3(kTileSlots), shm, s, r, g, L,
                        (int32_t)bin_lo);
   return hipGetLastError();
 }

@@ -187,6 +187,18 @@ class Model:
         b = np.ascontiguousarray(qTR, dtype=np.float64)
         self._chk(N.lib().mhmhip_set_routing_state(self._ctx, set_, N.dptr(a), N.dptr(b)))
 
+    def qmod_accumulate(self, on=True):
+        """Start (sums zeroed) or stop summing every node's Qmod over the routed
+        steps on the device (the output aggregation of mo_nc_output.f90:140-175)."""
+        self._chk(N.lib().mhmhip_qmod_accumulate(self._ctx, 1 if on else 0))
+
+    def qmod_sum(self, set_=0):
+        """(per-node Qmod sum in reference node order, steps added)."""
+        out = np.zeros(self.n_cells)
+        cnt = C.c_int64()
+        self._chk(N.lib().mhmhip_get_qmod_sum(self._ctx, set_, N.dptr(out), C.byref(cnt)))
+        return out, cnt.value
+
     # -- stepping ----------------------------------------------------------------
     def mhm_step(self, step):
         self._chk(N.lib().mhmhip_mhm_step(self._ctx, C.byref(step)))

@@ -56,12 +56,17 @@ module mo_mhmhip
                           route_mode = 0, route_window = 0, reserved(2) = 0
   end type
 
+  type, bind(C), public :: mhmhip_plan
+    integer(c_int32_t) :: dmax, tau_max
+    integer(c_int64_t) :: n_tiles, n_bins, n_bnd, n_slots
+  end type
+
   public :: mhmhip_create, mhmhip_destroy, mhmhip_last_error, mhmhip_device_bytes, mhmhip_sync, &
             mhmhip_set_cell_var, mhmhip_get_cell_var, mhmhip_default_states, &
             mhmhip_set_forcing, mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, &
             mhmhip_set_routing_params, mhmhip_set_inflow, mhmhip_get_routing_state, &
             mhmhip_set_routing_state, mhmhip_mrm_step, mhmhip_run, mhmhip_set_profiling, &
-            mhmhip_get_profile
+            mhmhip_get_profile, mhmhip_plan_network, mhmhip_qmod_accumulate, mhmhip_get_qmod_sum
   public :: mhmhip_check, mhm_hip_put, mhm_hip_get, mhm_hip_do_time_step
 
   interface
@@ -178,6 +183,27 @@ module mo_mhmhip
       integer(c_int32_t), value :: kernel
       real(c_double), intent(out) :: ms, units
       integer(c_int64_t), intent(out) :: launches
+    end function
+    integer(c_int) function mhmhip_plan_network(n_nodes, n_outlets, netPerm, fromN, toN, plan, &
+                                                pos, slots, slots_cap) &
+        bind(C, name="mhmhip_plan_network")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, mhmhip_plan
+      integer(c_int64_t), value :: n_nodes, slots_cap
+      integer(c_int32_t), value :: n_outlets
+      type(c_ptr), value :: netPerm, fromN, toN, pos, slots
+      type(mhmhip_plan), intent(out) :: plan
+    end function
+    integer(c_int) function mhmhip_qmod_accumulate(ctx, on) bind(C, name="mhmhip_qmod_accumulate")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function mhmhip_get_qmod_sum(ctx, set, sum, count) &
+        bind(C, name="mhmhip_get_qmod_sum")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx, sum
+      integer(c_int32_t), value :: set
+      integer(c_int64_t), intent(out) :: count
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
