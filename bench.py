@@ -414,7 +414,7 @@ def main():
 
     extra = {"roofline_hbm": hbm}
     if not args.no_plain:
-        pkw = dict(mode=N.RUN_PLAIN, want_gauges=True)
+        pkw = dict(mode=N.RUN_PLAIN, want_gauges=True, route_window=args.route_window)
         model.run(steps[:48], **pkw)
         model.set_profiling(True, ROUTE_STRIDE)
         tp = timed_runs(model, steps, 1, **pkw)
@@ -422,10 +422,10 @@ def main():
         model.set_profiling(False)
         extra["plain"] = {"value": dom.n_cells * n_steps / tp, "unit": "cell-steps/s",
                           "workload": "same basin and steps, one launch per step, every flux "
-                                      "written every step",
+                                      f"written every step, tiled routing window {args.route_window}",
                           "cell_step_roofline": hbm_roofline("cell_step", pprof["cell_step"], 1,
                                                              24, traffic),
-                          "kernels": kernel_table(pprof, 1, 24)}
+                          "kernels": kernel_table(pprof, 1, 24, args.route_window)}
         log(f"plain: {extra['plain']['value']:.4g} cell-steps/s")
     model.close()
 
