@@ -467,7 +467,7 @@ struct TileOutRow {
 // one would shorten the wait for a fetched value.
 template <int NCW, bool STAGE, bool ACC>
 __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, double *qtr, int nd,
-                                           double *out, int B) {
+                                           double *out, int B, int out_cap) {
   constexpr int W = kTileSlots;
   const int s = threadIdx.x;
   uint32_t up[NCW];  // byte offsets in buffer 0
@@ -484,26 +484,33 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
   // written as a conditional advance, the compiler reuses the previous
   // element where the clamp holds it, and each fetch then waits for the one
   // before.
-  const int32_t jmax = u.nb > 0 ? u.nb - 1 : 0;
+  // (a real slot's index is df itself: no shift, no clamp below nd)
+  const int32_t fshift = u.real ? 0 : u.lag;
+  const int32_t fmax = u.real ? 0x7FFFFFFF : (u.nb > 0 ? u.nb - 1 : 0);
   auto fetch = [&](int df) -> double {
-    const uint32_t jc = u.real ? (uint32_t)df : (uint32_t)min(max(df - u.lag, 0), jmax);
+    const uint32_t jc = (uint32_t)min(max(df - fshift, 0), fmax);
     return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(u.p) +
                                              (uint64_t)jc * u.stride);
   };
+  // steps this slot updates (real and due), and the run's last step in the window
+  const uint32_t nupd = u.act && u.real ? (uint32_t)u.nb : 0u;
+  const int32_t jlast = (int32_t)min<int64_t>(r.t_hi - 1 - u.t0, 0x7FFFFFFF);
   double qq[kTileAhead];
 #pragma unroll
   for (int i = 0; i < kTileAhead; ++i) qq[i] = 0.0;
   double musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
   // STAGE: a diagonal's outputs are written to LDS on the next diagonal, after
-  // its upstream reads are issued, not before the barrier that ends it
+  // its upstream reads are issued, not before the barrier that ends it.
+  // Every lane writes both, without a branch: a lane with nothing to keep
+  // writes its own scratch entries (after the rows).  Rows are B + 1 apart:
+  // the lanes of a wave write different banks.
   double pq = 0.0, ps = 0.0;
-  int pj = -1;
-  // rows are B + 1 apart: the lanes of a wave write different banks
+  const int scratch_t = out_cap * (B + 1) + s;
+  const int scratch_g = scratch_t + W;
+  int it = scratch_t, ig = scratch_g;
   auto put_rows = [&]() {
-    if (pj >= 0) {
-      if (u.orow_t >= 0) out[u.orow_t * (B + 1) + pj] = pq;
-      if (u.orow_g >= 0) out[u.orow_g * (B + 1) + pj] = ps;
-    }
+    out[it] = pq;
+    out[ig] = ps;
   };
   // Diagonal d reads parity buffer (d + 1) & 1, written on diagonal d - 1 by
   // the upstream slots (one level deeper, one diagonal earlier, same step),
@@ -511,7 +518,6 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
   // slot's value is read only by a parent that is inactive for the same step.
   auto diagonal = [&](int d, int wb, double qo) {
     const int j = d - u.lag;
-    const bool on = u.act && (uint32_t)j < (uint32_t)u.nb;
     const char *rb_b = qtr_b + (wb ^ 1) * W * 8;  // the buffer written on diagonal d - 1
     double v[NCW];
 #pragma unroll
@@ -530,14 +536,15 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
     double q = musk + u.c2 * (sum - u.qtin1);
     if (cut) q = 0.0;
     qtr[wb * W + s] = u.proxy ? qo : q;
-    const bool upd = on && u.real;
+    const bool upd = (uint32_t)j < nupd;
     if (upd && link) u.qtr1 = q;
     if (upd) u.qtin1 = sum;
     if (ACC && upd) u.qacc = u.qacc + sum;  // Qmod of step j, added in step order
     musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
-    if (upd && u.t0 + j == r.t_hi - 1) u.qlast = qo;
+    if (upd && j == jlast) u.qlast = qo;
     if (STAGE) {
-      pj = upd && rare ? j : -1;
+      it = upd && u.orow_t >= 0 ? u.orow_t * (B + 1) + j : scratch_t;
+      ig = upd && u.orow_g >= 0 ? u.orow_g * (B + 1) + j : scratch_g;
       pq = q;
       ps = sum;
     } else if (upd && rare) {
@@ -569,7 +576,8 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
 }
 
 // LDS: qTR parity buffers [2][W], +0.0 at 2W and 3W, then (STAGE) the bin's
-// output rows: g.out_rows TileOutRow, then g.out_rows x (B + 1) values.
+// output rows: g.out_rows TileOutRow, then g.out_rows x (B + 1) values and
+// 2 W scratch values.
 // ACC: Qmod summed per node (r.qacc).
 template <bool STAGE, bool ACC>
 __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, TileArgs g,
@@ -646,10 +654,10 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
   if (s == 0) qtr[2 * W] = qtr[3 * W] = 0.0;
   lds_barrier();
   const int nd = bi.height + g.B - 1;
-  if (__any(u.nc > 5)) tile_sweep<8, STAGE, ACC>(r, u, qtr, nd, out, g.B);
-  else if (__any(u.nc > 3)) tile_sweep<5, STAGE, ACC>(r, u, qtr, nd, out, g.B);
-  else if (__any(u.nc > 1)) tile_sweep<3, STAGE, ACC>(r, u, qtr, nd, out, g.B);
-  else tile_sweep<1, STAGE, ACC>(r, u, qtr, nd, out, g.B);
+  if (__any(u.nc > 5)) tile_sweep<8, STAGE, ACC>(r, u, qtr, nd, out, g.B, g.out_rows);
+  else if (__any(u.nc > 3)) tile_sweep<5, STAGE, ACC>(r, u, qtr, nd, out, g.B, g.out_rows);
+  else if (__any(u.nc > 1)) tile_sweep<3, STAGE, ACC>(r, u, qtr, nd, out, g.B, g.out_rows);
+  else tile_sweep<1, STAGE, ACC>(r, u, qtr, nd, out, g.B, g.out_rows);
   if (STAGE) {
     // the sweep ended with a barrier after the last rows: they are complete
     for (int idx = s; idx < bi.out_rows * g.B; idx += W) {
@@ -758,7 +766,8 @@ hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_
     return hipErrorInvalidValue;
   dim3 grid((unsigned)(bin_hi - bin_lo), (unsigned)r.n_sets);
   const size_t base = sizeof(double) * (3 * (size_t)kTileSlots + 1);
-  const size_t stage = (size_t)g.out_rows * (sizeof(TileOutRow) + sizeof(double) * (g.B + 1));
+  const size_t stage = (size_t)g.out_rows * (sizeof(TileOutRow) + sizeof(double) * (g.B + 1)) +
+                       sizeof(double) * 2 * kTileSlots;
   const bool st = base + stage <= kTileStageLds;
   const size_t shm = st ? base + stage : base;
   if (st && r.qacc)
