@@ -18,7 +18,6 @@ namespace mhmhip {
 // src/common/mo_common_constants.f90: eps_dp = epsilon(1.0_dp)
 constexpr double kEps = DBL_EPSILON;
 // FORCES mo_constants: twothird_dp
-constexpr double kTwoThird = 0.6666666666666666666666666666666666667;
 // src/mHM/mo_mhm_constants.f90:36-39
 constexpr double kHarSamConst = 17.800;
 constexpr double kDuffieDr = 0.0330;
@@ -188,6 +187,42 @@ __host__ __device__ inline double log_pos(double x) { return log_pos_v<1>(DV<1>{
 __host__ __device__ inline double exp_fin(double z) { return exp_fin_v<1>(DV<1>{{z}}).v[0]; }
 __host__ __device__ inline double exp_ylogx(double y, double x) {
   return exp_ylogx_v<1>(DV<1>{{y}}, DV<1>{{x}}).v[0];
+}
+
+// x**(2/3) for finite x > 0 (the canopy's (I / Imax)**(2/3),
+// mo_canopy_interc.f90:121), without a log and an exp: x = m 2^e with
+// e = 3q + r (r in 0..2), a = m 2^r in [1/2, 4), and c = cbrt(a) from a
+// single-precision seed 2^(log2(a) / 3) (relative error below 2^-20),
+// refined by one Halley step written as a small correction,
+//   c' = c - c (c^3 - a) / (2 c^3 + a),
+// with c^2 split exactly (fma) so that c^3 - a keeps its digits: the seed's
+// error becomes 2^-60 before rounding; x**(2/3) = c'^2 2^(2q).  Within 3 ulp
+// of the exact power (against mpmath in the tests).  The seed comes from the
+// hardware's single-precision log2 / exp2 on the device and from libm on the
+// host, and the divisor's reciprocal from v_rcp_f64 and one Newton step, so
+// the two builds may differ in the last bit.
+__host__ __device__ inline double pow23(double x) {
+  int e;
+  const double m = frexp(x, &e);  // [1/2, 1)
+  const int q = (e + 1200) / 3 - 400;  // floor(e / 3) for e > -1200
+  const double a = ldexp(m, e - 3 * q);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double t =
+      (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)a) * (1.0f / 3.0f));
+#else
+  const double t = (double)exp2f(log2f((float)a) * (1.0f / 3.0f));
+#endif
+  const double t2h = t * t, t2l = fma(t, t, -t2h);     // t^2 exactly
+  const double d = fma(t2l, t, fma(t2h, t, -a));       // t^3 - a
+  const double den = fma(2.0 * t2h, t, a);             // 2 t^3 + a (to ~1 ulp)
+#if defined(__HIP_DEVICE_COMPILE__)
+  double rr = __builtin_amdgcn_rcp(den);
+  rr = fma(rr, fma(-den, rr, 1.0), rr);
+#else
+  const double rr = 1.0 / den;
+#endif
+  const double c = fma(-(t * d), rr, t);
+  return ldexp(c * c, 2 * q);
 }
 
 // x / d for a divisor whose correctly rounded reciprocal r = RN(1/d) is known:

@@ -199,26 +199,24 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
                                            double maxinter, double rmaxinter, double evap,
                                            double revap) {
   const double ratio = canopy_fill(maxinter, rmaxinter, x.prec, c.inter, x.thr);
-  // The canopy's (interc / interc_max)**(2/3) and every horizon's
-  // (sm / sat)**beta depend only on values known here: one call evaluates
-  // these NH + 1 powers side by side.  Arguments the reference never raises
-  // to a power are replaced by 1 and their results left unused.
-  DV<NH + 1> py, px;
-  py.v[0] = kTwoThird;
-  px.v[0] = ratio > 0.0 ? ratio : 1.0;
+  // Every horizon's (sm / sat)**beta depends only on values known here: one
+  // call evaluates these NH powers side by side.  Arguments the reference
+  // never raises to a power are replaced by 1 and their results left unused.
+  // The canopy's (interc / interc_max)**(2/3) is pow23.
+  DV<NH> py, px;
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
     const double q = div_rcp(c.sm[h], c.sat[h], c.rsat[h]);
-    py.v[h + 1] = c.bexp[h];
-    px.v[h + 1] = q > 0.0 ? q : 1.0;
+    py.v[h] = c.bexp[h];
+    px.v[h] = q > 0.0 ? q : 1.0;
   }
-  const DV<NH + 1> pw = exp_ylogx_v<NH + 1>(py, px);
-  canopy_evap(x.pet, maxinter, ratio, pw.v[0], c.inter, x.ecan);
+  const DV<NH> pw = exp_ylogx_v<NH>(py, px);
+  canopy_evap(x.pet, maxinter, ratio, ratio > 0.0 ? pow23(ratio) : 0.0, c.inter, x.ecan);
   snow_accum_melt(c.incr, c.dmaxs, c.dnps, c.ddthr, x.prec, x.temp, c.tth, x.thr, c.pack, x.ddf,
                   x.melt, x.peff, x.rain, x.snow);
   soil_moisture<NH, JV>(c.fS, c.thrS, c.rthrS, x.pet, evap, revap, c.sat, c.fr, c.fc, c.wp, c.rfw,
                         c.c1,
-                    x.ecan, x.peff, &pw.v[1], x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
+                    x.ecan, x.peff, &pw.v[0], x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
   runoff_unsat_zone(c.k1s, c.kps, c.k0s, c.alpha, c.karst, x.infil[NH - 1], c.thrU, c.satS,
                     c.unsat, x.slow, x.fast, x.perc);
   runoff_sat_zone(c.k2s, c.satS, x.bflow);

@@ -13,7 +13,10 @@ compiles for the host and for gfx950.  Pinned here on the host build:
     underflow, subnormal arguments;
   * planted mistakes (a dropped series term in exact-rounding Python copies
     of log_pos and exp_fin) fail the 1-ulp checks.
-The GPU test checks that the device build gives the host's bits.
+The GPU test checks that the device build gives the host's bits.  The
+canopy's x**(2/3) is pow23 (a cube root, no log and exp): within 3 ulp on
+the host and on the device (whose seed comes from hardware single-precision
+log2 / exp2, so the two may differ in the last bit), exact on exact cubes.
 """
 import ctypes as C
 import math
@@ -187,6 +190,45 @@ def test_copies_match_the_library_and_planted_mistakes_fail(lib):
     # a dropped term: log1p through u^6, exp through r^11
     assert max(_ulp_err(_log_copy(x, degree=6), mp.log(mp.mpf(x))) for x in xs) > 1.25
     assert max(_ulp_err(_exp_copy(z, top=11), mp.exp(mp.mpf(z))) for z in zs) > 1.0
+
+
+def _pow23_sample(rng, n):
+    return np.concatenate([np.exp(rng.uniform(-700, 0, n)), rng.uniform(0.0, 1.0, n),
+                           np.exp(rng.uniform(0, 700, n // 4)),
+                           np.array([5e-324, 2.2250738585072014e-308, 2**-1074 * 3, 0.5, 1.0,
+                                     1.0 - 2**-53, 2.0, 4.0, 8.0, 0.125, 1e-300, 1e300,
+                                     1.7976931348623157e308])])
+
+
+def test_pow23_within_three_ulp(lib):
+    """x**(2/3) by cube root (pow23, the canopy's power) against mpmath; exact
+    cubes come out exact."""
+    rng = np.random.default_rng(6)
+    x = _pow23_sample(rng, 3000)
+    x = x[x > 0]
+    got = _eval(lib, 6, x)
+    worst = max(_ulp_err(g, mp.power(mp.mpf(v), mp.mpf(2) / 3)) for g, v in zip(got, x))
+    assert worst <= 3.0, worst
+    cubes = np.array([2.0 ** (3 * k) for k in range(-340, 341)] + [27.0, 1 / 27.0, 125.0, 0.001])
+    want = np.array([2.0 ** (2 * k) for k in range(-340, 341)] + [9.0, 1 / 9.0, 25.0, 0.01])
+    got = _eval(lib, 6, cubes)
+    assert np.array_equal(got[:681], want[:681])
+    assert got[681] == 9.0 and got[683] == 25.0
+    assert max(_ulp_err(g, mp.power(mp.mpf(v), mp.mpf(2) / 3)) for g, v in zip(got, cubes)) <= 3.0
+
+
+@pytest.mark.gpu
+def test_device_pow23_within_three_ulp(lib):
+    """The device's pow23 seeds its cube root from the hardware's
+    single-precision log2 / exp2: held to the same bound as the host."""
+    print("case: pow23 on the device", flush=True)
+    rng = np.random.default_rng(7)
+    x = _pow23_sample(rng, 20000)
+    x = x[x > 0]
+    got = _eval(lib, 6, x, device=True)
+    sub = np.arange(0, x.shape[0], 7)
+    worst = max(_ulp_err(got[i], mp.power(mp.mpf(x[i]), mp.mpf(2) / 3)) for i in sub)
+    assert worst <= 3.0, worst
 
 
 @pytest.mark.gpu
