@@ -413,21 +413,37 @@ __device__ __forceinline__ void soil_moisture(double fS, double thrS, double rth
 }
 
 // src/mHM/mo_runoff.f90:74-154 (k0, k1, kp already c2TSTu / k)
-__device__ __forceinline__ void runoff_unsat_zone(double k1, double kp, double k0, double alpha,
-                                                  double karst, double pin, double thr,
-                                                  double &sat, double &unsat, double &slow,
-                                                  double &fast, double &perc) {
+// in two parts around the power unsat**(1 + alpha) of the slow interflow:
+// runoff_fast adds the infiltration and takes the fast interflow (the store
+// is then the power's base), runoff_slow_perc finishes with the power pw
+// (of unsat when unsat > 0).
+__device__ __forceinline__ void runoff_fast(double k0, double pin, double thr, double &unsat,
+                                            double &fast) {
   unsat = unsat + pin;
   fast = 0.0;
   if (unsat > thr) fast = vmin(unsat - kEps, k0 * (unsat - thr));
   unsat = unsat - fast;
+}
+
+__device__ __forceinline__ void runoff_slow_perc(double k1, double kp, double karst, double pw,
+                                                 double &sat, double &unsat, double &slow,
+                                                 double &perc) {
   slow = 0.0;
-  if (unsat > kEps) slow = vmin(unsat - kEps, k1 * pow_pos(unsat, 1.0 + alpha));
+  if (unsat > kEps) slow = vmin(unsat - kEps, k1 * pow_pos_from(unsat, pw));
   unsat = unsat - slow;
   perc = kp * unsat;
   const double p = vmin(perc, unsat);  // the whole store when perc exceeds it
   sat = sat + p * karst;
   unsat = unsat - p;
+}
+
+__device__ __forceinline__ void runoff_unsat_zone(double k1, double kp, double k0, double alpha,
+                                                  double karst, double pin, double thr,
+                                                  double &sat, double &unsat, double &slow,
+                                                  double &fast, double &perc) {
+  runoff_fast(k0, pin, thr, unsat, fast);
+  runoff_slow_perc(k1, kp, karst, unsat > 0.0 ? exp_ylogx(1.0 + alpha, unsat) : 0.0, sat, unsat,
+                   slow, perc);
 }
 
 // src/mHM/mo_runoff.f90:191-212

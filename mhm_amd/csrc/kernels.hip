@@ -117,7 +117,8 @@ __device__ __forceinline__ void store_states(const CellArgs &a, uint32_t bv, con
 }
 
 template <int NH>
-__device__ __forceinline__ void store_fluxes(const CellArgs &a, uint32_t bv, const FluxRegs<NH> &x) {
+__device__ __forceinline__ void store_fluxes_pre(const CellArgs &a, uint32_t bv,
+                                                 const FluxRegs<NH> &x) {
   auto X = [&](int r, double y) { stn(a.X + (int64_t)r * a.ld, bv, y); };
   X(XR_PET, x.pet);
   X(XR_TEMP, x.temp);
@@ -132,15 +133,28 @@ __device__ __forceinline__ void store_fluxes(const CellArgs &a, uint32_t bv, con
   X(XR_RUNOFFSEAL, x.rseal);
   X(XR_AETSEAL, x.aseal);
   X(XR_FAST, x.fast);
-  X(XR_SLOW, x.slow);
-  X(XR_PERC, x.perc);
-  X(XR_BASEFLOW, x.bflow);
-  X(XR_TOTAL, x.total);
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
     X(xr_infil(NH) + h, x.infil[h]);
     X(xr_aet(NH) + h, x.aet[h]);
   }
+}
+
+// the fluxes chain_post sets
+template <int NH>
+__device__ __forceinline__ void store_fluxes_post(const CellArgs &a, uint32_t bv,
+                                                  const FluxRegs<NH> &x) {
+  auto X = [&](int r, double y) { stn(a.X + (int64_t)r * a.ld, bv, y); };
+  X(XR_SLOW, x.slow);
+  X(XR_PERC, x.perc);
+  X(XR_BASEFLOW, x.bflow);
+  X(XR_TOTAL, x.total);
+}
+
+template <int NH>
+__device__ __forceinline__ void store_fluxes(const CellArgs &a, uint32_t bv, const FluxRegs<NH> &x) {
+  store_fluxes_pre<NH>(a, bv, x);
+  store_fluxes_post<NH>(a, bv, x);
 }
 
 // Daily PET before disaggregation (get_corrected_pet, mo_meteo_handler.f90:1050-1097).
@@ -192,35 +206,62 @@ __device__ __forceinline__ double runoff_acc(double q, double area, double tst, 
   return div_rcp((0.0 + q * area) * 1000.0, tst, rtst);
 }
 
-// One step of mHM() for one cell after the meteo getters (mo_mhm.f90:455-519);
-// JV: Jarvis soil-moisture stress (processCase(3) = 2, 3) instead of Feddes
-template <int NH, bool JV>
-__device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, FluxRegs<NH> &x,
-                                           double maxinter, double rmaxinter, double evap,
-                                           double revap) {
-  const double ratio = canopy_fill(maxinter, rmaxinter, x.prec, c.inter, x.thr);
-  // Every horizon's (sm / sat)**beta depends only on values known here: one
-  // call evaluates these NH powers side by side.  Arguments the reference
-  // never raises to a power are replaced by 1 and their results left unused.
-  // The canopy's (interc / interc_max)**(2/3) is pow23.
-  DV<NH> py, px;
+// Every horizon's (sm / sat)**beta depends only on the soil moisture at the
+// start of the step: the bases and exponents of these NH powers, into
+// slots off .. off + NH - 1.  Bases the reference never raises to a power
+// are replaced by 1 and their results left unused.
+template <int NH, int N>
+__device__ __forceinline__ void horizon_pow_args(const CellRegs<NH> &c, DV<N> &py, DV<N> &px,
+                                                 int off) {
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
     const double q = div_rcp(c.sm[h], c.sat[h], c.rsat[h]);
-    py.v[h] = c.bexp[h];
-    px.v[h] = q > 0.0 ? q : 1.0;
+    py.v[off + h] = c.bexp[h];
+    px.v[off + h] = q > 0.0 ? q : 1.0;
   }
-  const DV<NH> pw = exp_ylogx_v<NH>(py, px);
+}
+
+template <int NH>
+__device__ __forceinline__ DV<NH> horizon_pows(const CellRegs<NH> &c) {
+  DV<NH> py, px;
+  horizon_pow_args<NH, NH>(c, py, px, 0);
+  return exp_ylogx_v<NH>(py, px);
+}
+
+// One step of mHM() for one cell after the meteo getters (mo_mhm.f90:455-519),
+// in two parts around the slow interflow's power: chain_pre (canopy, snow,
+// soil moisture with the horizon powers pwh of the step's starting soil
+// moisture, fast interflow) and chain_post (slow interflow with pw_slow =
+// unsat**(1 + alpha), percolation, baseflow, total runoff).  The canopy's
+// (interc / interc_max)**(2/3) is pow23.
+// JV: Jarvis soil-moisture stress (processCase(3) = 2, 3) instead of Feddes
+template <int NH, bool JV>
+__device__ __forceinline__ void chain_pre(CellRegs<NH> &c, FluxRegs<NH> &x, double maxinter,
+                                          double rmaxinter, double evap, double revap,
+                                          const double *pwh) {
+  const double ratio = canopy_fill(maxinter, rmaxinter, x.prec, c.inter, x.thr);
   canopy_evap(x.pet, maxinter, ratio, ratio > 0.0 ? pow23(ratio) : 0.0, c.inter, x.ecan);
   snow_accum_melt(c.incr, c.dmaxs, c.dnps, c.ddthr, x.prec, x.temp, c.tth, x.thr, c.pack, x.ddf,
                   x.melt, x.peff, x.rain, x.snow);
   soil_moisture<NH, JV>(c.fS, c.thrS, c.rthrS, x.pet, evap, revap, c.sat, c.fr, c.fc, c.wp, c.rfw,
-                        c.c1,
-                    x.ecan, x.peff, &pw.v[0], x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
-  runoff_unsat_zone(c.k1s, c.kps, c.k0s, c.alpha, c.karst, x.infil[NH - 1], c.thrU, c.satS,
-                    c.unsat, x.slow, x.fast, x.perc);
+                        c.c1, x.ecan, x.peff, pwh, x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
+  runoff_fast(c.k0s, x.infil[NH - 1], c.thrU, c.unsat, x.fast);
+}
+
+template <int NH>
+__device__ __forceinline__ void chain_post(CellRegs<NH> &c, FluxRegs<NH> &x, double pw_slow) {
+  runoff_slow_perc(c.k1s, c.kps, c.karst, pw_slow, c.satS, c.unsat, x.slow, x.perc);
   runoff_sat_zone(c.k2s, c.satS, x.bflow);
   x.total = total_runoff(c.fS, x.fast, x.slow, x.bflow, x.rseal);
+}
+
+template <int NH, bool JV>
+__device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, FluxRegs<NH> &x,
+                                           double maxinter, double rmaxinter, double evap,
+                                           double revap) {
+  const DV<NH> pwh = horizon_pows<NH>(c);
+  chain_pre<NH, JV>(c, x, maxinter, rmaxinter, evap, revap, pwh.v);
+  chain_post<NH>(c, x, c.unsat > 0.0 ? exp_ylogx(1.0 + c.alpha, c.unsat) : 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -292,6 +333,12 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
     rlane = a.ring + (int64_t)(a.lag ? a.lag[c] : 0) * a.ld + bv / 8u;
     roff = t0 % a.ring_len * a.ld;
   }
+  // The horizon powers of a step depend only on its starting soil moisture,
+  // known when the step before reaches its slow interflow: one call
+  // evaluates that step's unsat**(1 + alpha) and the next step's NH horizon
+  // powers side by side (independent chains for the scheduler); the first
+  // step's come from here, a step that resets the soil moisture recomputes.
+  DV<NH> pwh = horizon_pows<NH>(r);
 #pragma nounroll
   for (int32_t s = 0; s < nsteps; ++s) {
     const StepDev st = read_step(steps + s);
@@ -317,14 +364,27 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
     if (st.init_sm) {
 #pragma unroll
       for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
+      pwh = horizon_pows<NH>(r);
     }
-    chain_step<NH, JV>(a, r, x, maxinter, rmaxinter, st.evap, st.revap);
+    chain_pre<NH, JV>(r, x, maxinter, rmaxinter, st.evap, st.revap, pwh.v);
+    // the last step's fluxes leave as soon as they are known, so that they
+    // are not held across the powers
+    const bool last = s == nsteps - 1;
+    if (last) store_fluxes_pre<NH>(a, bv, x);
+    DV<NH + 1> py, px;
+    py.v[0] = 1.0 + r.alpha;
+    px.v[0] = r.unsat > 0.0 ? r.unsat : 1.0;
+    horizon_pow_args<NH, NH + 1>(r, py, px, 1);
+    const DV<NH + 1> pw = exp_ylogx_v<NH + 1>(py, px);
+    chain_post<NH>(r, x, r.unsat > 0.0 ? pw.v[0] : 0.0);
+#pragma unroll
+    for (int h = 0; h < NH; ++h) pwh.v[h] = pw.v[h + 1];
     if (a.ring) {
       rlane[roff] = runoff_acc(x.total, area, a.tst, a.rtst);
       roff += a.ld;
       if (roff == a.ring_len * a.ld) roff = 0;
     }
-    if (s == nsteps - 1) store_fluxes<NH>(a, bv, x);
+    if (last) store_fluxes_post<NH>(a, bv, x);
   }
   store_states<NH>(a, bv, r);
 }
