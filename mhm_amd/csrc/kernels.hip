@@ -12,7 +12,11 @@
 // Its upstream nodes sit one level deeper, so they produced step t on
 // diagonal d-1; a two-slot parity buffer of routed outflow (qTR) keeps step t
 // readable while those nodes already write step t+1.  A whole run of T steps
-// takes T + dmax diagonals instead of T * dmax level sweeps.
+// takes T + dmax diagonals instead of T * dmax level sweeps.  The default
+// schedule (route_tile_kernel) runs the same diagonals inside tiles of the
+// tree, one workgroup per bin of tiles and window of steps, with the window
+// handed from tile to tile between launches; route_diag_kernel and
+// route_wg_kernel run them over the whole grid or inside one workgroup.
 #include <hip/hip_runtime.h>
 
 #include "chain.h"
@@ -304,8 +308,9 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
 // in registers.  Forcing and daily PET are read/evaluated once per forcing day
 // (the reference recomputes the same value every hour), maximum interception
 // once per month.  Only the routing input leaves the lane each step; states
-// and the last step's fluxes are written at the end.  Per-step fluxes are
-// scoped to the loop body so they do not stay live across steps.
+// are written at the end and the last step's fluxes as soon as each is known.
+// Per-step fluxes are scoped to the loop body so they do not stay live across
+// steps.
 template <int NH, bool JV>
 #ifndef MHMHIP_FUSED_WAVES
 #define MHMHIP_FUSED_WAVES 2
