@@ -32,6 +32,13 @@ __device__ __forceinline__ double ldr(const double *row, uint32_t boff) {
 __device__ __forceinline__ void str(double *row, uint32_t boff, double x) {
   *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + boff) = x;
 }
+// Waits for v's load here and makes v count as defined by this (empty)
+// statement.  A value loaded in a block that runs once per day or month and
+// used every step otherwise leaves the compiler's memory-counter tracking
+// unsure whether its load is still in flight where the paths merge, and the
+// wait it puts there (every step) also waits for the step before's ring
+// store.
+__device__ __forceinline__ void settle(double &v) { asm volatile("" : "+v"(v)); }
 // streaming store for outputs no kernel of the step reads back
 #ifndef MHMHIP_NT_STORE
 #define MHMHIP_NT_STORE 1
@@ -354,12 +361,17 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
       const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, bc) : 0.0;
       const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, bc) : 0.0;
       pd = pet_day<NH>(a, st, bv, bc, tavg, tmax, tmin);
+      settle(pre);
+      settle(tavg);
+      settle(pd);
       cached_day = st.day;
       cached_doy = st.doy;
     }
     if (st.month0 != cached_month) {
       maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld, bv);
       rmaxinter = 1.0 / maxinter;
+      settle(maxinter);
+      settle(rmaxinter);
       cached_month = st.month0;
     }
     FluxRegs<NH> x;
