@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-ROUTE_STRIDE = 16          # one routing launch in 16 is bracketed by events
+PROF_STRIDE = 16           # one launch in 16 of each kernel is bracketed by events
 COPY_CALIB_BYTES = 1 << 30  # 1 GiB read + 1 GiB written by the calibration copy
 
 
@@ -75,7 +75,8 @@ def kernel_table(prof, steps_per_launch, steps_per_day, window=8):
     for k, (kms, kn, ku) in prof.items():
         if kn:
             b = bytes_per_unit(k, steps_per_launch, steps_per_day, window)
-            detail[k] = {"ms": round(kms, 3), "launches": kn, "units": ku,
+            # the timed launches only (one in PROF_STRIDE of each kernel)
+            detail[k] = {"ms": round(kms, 3), "timed_launches": kn, "units": ku,
                          "avg_us": round(kms * 1e3 / kn, 3),
                          "bytes_per_unit": round(b, 3),
                          "GBs": round(ku * b / (kms * 1e-3) / 1e9, 1)}
@@ -161,10 +162,11 @@ def bench_ensemble(N, workloads, autotune, traffic):
                                       cands=list(autotune.candidates(em, S_values=(73, 365, 730))),
                                       repeats=2)
     snap = em.snapshot()
-    em.set_profiling(True, ROUTE_STRIDE)
+    # a few launches per run: time them all
+    em.set_profiling(True, 1)
     em.run(esteps[:730], want_gauges=False, **best)  # graph capture, if chosen
     em.restore(snap)
-    em.set_profiling(True, ROUTE_STRIDE)
+    em.set_profiling(True, 1)
     te = timed_runs(em, esteps, 1, **best)
     eprof = em.profile()
     em.close()
@@ -183,9 +185,9 @@ def bench_multi_basin(N, workloads, S):
     cells = sum(d.n_cells for d in mdoms)
     kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=True)
     mm.run(msteps, **kw)
-    mm.set_profiling(True, ROUTE_STRIDE)
+    mm.set_profiling(True, PROF_STRIDE)
     mm.run(msteps, **kw)
-    mm.set_profiling(True, ROUTE_STRIDE)
+    mm.set_profiling(True, PROF_STRIDE)
     tm = timed_runs(mm, msteps, 1, **kw)
     mprof = mm.profile()
     mm.close()
@@ -388,10 +390,10 @@ def main():
     kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=not args.no_graph,
               route_window=args.route_window)
     # profiling on before the warm-up so a captured graph holds its event nodes
-    model.set_profiling(True, ROUTE_STRIDE)
+    model.set_profiling(True, PROF_STRIDE)
     for _ in range(args.warmup):
         model.run(steps, **kw)
-    model.set_profiling(True, ROUTE_STRIDE)  # counters from zero for the timed region
+    model.set_profiling(True, PROF_STRIDE)  # counters from zero for the timed region
     model.sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -416,7 +418,7 @@ def main():
     if not args.no_plain:
         pkw = dict(mode=N.RUN_PLAIN, want_gauges=True, route_window=args.route_window)
         model.run(steps[:48], **pkw)
-        model.set_profiling(True, ROUTE_STRIDE)
+        model.set_profiling(True, PROF_STRIDE)
         tp = timed_runs(model, steps, 1, **pkw)
         pprof = model.profile()
         model.set_profiling(False)

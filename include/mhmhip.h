@@ -287,13 +287,16 @@ int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
 
 /* ---- measurement (no counterpart in the reference, which has only wall
  * timers, mo_mhm_interface.F90:157-162) -------------------------------------
- * With profiling on, every kernel launch is bracketed by HIP events on the
- * context's stream; the device time, launch count and work units (cell-steps
- * for the cell kernels, node-steps for the routing) accumulate per kernel. */
+ * With profiling on, kernel launches are bracketed by HIP events on the
+ * stream they run on; the device time, launch count and work units
+ * (cell-steps for the cell kernels, node-steps for the routing) of the timed
+ * launches accumulate per kernel. */
 enum mhmhip_kernel { MHMHIP_K_CELL_STEP = 0, MHMHIP_K_CELL_FUSED, MHMHIP_K_ROUTE_DIAG,
                      MHMHIP_K_ROUTE_WG, MHMHIP_K_ROUTE_TILE, MHMHIP_K_COUNT };
-/* on > 0 enables timing and resets the totals; routing-diagonal launches are
- * then timed one in `on` (1: all), the totals scaled to the timed ones. */
+/* on > 0 enables timing and resets the totals; each kernel's launches are
+ * then timed one in `on` (1: all; an event pair costs the stream a few
+ * microseconds).  In a captured time loop the timed launches are chosen at
+ * capture and every replay times the same ones. */
 int mhmhip_set_profiling(mhmhip_ctx *ctx, int32_t on);
 int mhmhip_get_profile(mhmhip_ctx *ctx, int32_t kernel, double *ms, int64_t *launches,
                        double *units);

@@ -127,8 +127,8 @@ struct mhmhip_ctx {
   std::vector<hipEvent_t> graph_sync;  // cross-stream edges of the captured graph
   int64_t graph_all[MHMHIP_K_COUNT] = {};
   bool prof = false;
-  int prof_stride = 1;         // time every prof_stride-th routing launch
-  int64_t route_launches = 0;  // routing launches seen while profiling
+  int prof_stride = 1;         // time every prof_stride-th launch of each kernel
+  int64_t prof_seq[MHMHIP_K_COUNT] = {};  // launches seen per kernel while profiling
   std::vector<Pending> pending;
   std::vector<hipEvent_t> ev_pool;
   double prof_ms[MHMHIP_K_COUNT] = {};
@@ -734,16 +734,16 @@ hipError_t ev_get(mhmhip_ctx *x, hipEvent_t *e) {
 }
 
 // Launches f() on the context stream; with profiling on, brackets it with
-// events (routing launches: every prof_stride-th one).
+// events: every prof_stride-th launch of each kernel (in a captured graph,
+// the choice is made at capture and every replay times the same launches).
 template <typename Fn>
 hipError_t timed(mhmhip_ctx *x, int kind, double units, Fn f, hipStream_t s = nullptr) {
   if (!s) s = x->stream;
   if (!x->prof) return f();
   (x->capturing ? x->graph_all : x->prof_all)[kind] += 1;
-  // routing launches come by the hundred or thousand: time one in prof_stride
-  if ((kind == MHMHIP_K_ROUTE_DIAG || kind == MHMHIP_K_ROUTE_TILE) &&
-      (x->route_launches++ % x->prof_stride) != 0)
-    return f();
+  // a time loop launches kernels by the hundred or thousand, and each event
+  // pair costs the stream a few microseconds: time one launch in prof_stride
+  if ((x->prof_seq[kind]++ % x->prof_stride) != 0) return f();
   mhmhip_ctx::Pending p{kind, nullptr, nullptr, units};
   hipError_t e;
   if (x->capturing) {
@@ -1528,7 +1528,7 @@ extern "C" int mhmhip_set_profiling(mhmhip_ctx *x, int32_t on) {
   CHECK_RC(prof_collect(x));
   x->prof = on > 0;
   x->prof_stride = on > 0 ? on : 1;
-  x->route_launches = 0;
+  for (int k = 0; k < MHMHIP_K_COUNT; ++k) x->prof_seq[k] = 0;
   if (x->prof)
     for (int k = 0; k < MHMHIP_K_COUNT; ++k) {
       x->prof_ms[k] = 0.0;
@@ -1543,8 +1543,13 @@ extern "C" int mhmhip_get_profile(mhmhip_ctx *x, int32_t kernel, double *ms, int
                                   double *units) {
   if (!x) return MHMHIP_EINVAL;
   if (kernel < 0 || kernel >= MHMHIP_K_COUNT) return x->fail(MHMHIP_EINVAL, "get_profile: bad kernel");
-  // units scaled to the timed launches when routing launches were sampled
-  const double frac = x->prof_all[kernel] ? (double)x->prof_n[kernel] / x->prof_all[kernel] : 1.0;
+  // routing units are counted for all launches (a call's node-steps), so
+  // they are scaled to the timed ones; cell units come with each timed launch
+  const bool routing = kernel == MHMHIP_K_ROUTE_DIAG || kernel == MHMHIP_K_ROUTE_WG ||
+                       kernel == MHMHIP_K_ROUTE_TILE;
+  const double frac = routing && x->prof_all[kernel]
+                          ? (double)x->prof_n[kernel] / x->prof_all[kernel]
+                          : 1.0;
   if (ms) *ms = x->prof_ms[kernel];
   if (launches) *launches = x->prof_n[kernel];
   if (units) *units = x->prof_units[kernel] * frac;

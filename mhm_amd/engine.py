@@ -243,9 +243,9 @@ class Model:
             self.set_routing_state(rs["qTIN"], rs["qTR"], s)
 
     # -- measurement ---------------------------------------------------------------
-    def set_profiling(self, on=True, route_stride=1):
-        """Kernel timing on/off; routing diagonals timed one in route_stride."""
-        self._chk(N.lib().mhmhip_set_profiling(self._ctx, max(1, int(route_stride)) if on else 0))
+    def set_profiling(self, on=True, stride=1):
+        """Kernel timing on/off; each kernel's launches timed one in stride."""
+        self._chk(N.lib().mhmhip_set_profiling(self._ctx, max(1, int(stride)) if on else 0))
 
     def profile(self):
         """{kernel: (device ms, launches, work units)} accumulated since set_profiling(True)."""

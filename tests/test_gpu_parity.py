@@ -170,10 +170,13 @@ def test_profiling_inside_graph_replays():
             assert np.array_equal(g1, g2)
             a += 12
     p = m2.profile()
-    assert p["cell_fused"][1] == 8 and p["cell_fused"][0] > 0.0
-    assert p["cell_fused"][2] == pytest.approx(8 * 6 * dom.n_cells)
+    # one launch in 3 of each kernel is timed, chosen when a graph is
+    # captured; the context keeps one graph, so the alternating routing modes
+    # recapture on every call: fused launches 0, 3 and 6 of the eight
+    assert p["cell_fused"][1] == 3 and p["cell_fused"][0] > 0.0
+    assert p["cell_fused"][2] == pytest.approx(3 * 6 * dom.n_cells)
     assert p["route_diag"][1] > 0 and p["route_diag"][0] > 0.0
-    # routing launches are sampled (one in 3): units follow the timed launches
+    # routing units are a call's node-steps, scaled to the timed launches
     assert p["route_tile"][1] > 0 and p["route_tile"][0] > 0.0
     assert 0.0 < p["route_tile"][2] <= 24 * dom.n_cells
     m1.close()
