@@ -207,6 +207,21 @@ int mhmhip_default_states(mhmhip_ctx *ctx, const double *horizon_depth);
  * (src/meteo/mo_meteo_handler.f90:1462-1605). */
 int mhmhip_set_forcing(mhmhip_ctx *ctx, int32_t var, int32_t day0, int32_t ndays,
                        const double *host, int64_t ld);
+/* Forcing columns day0..day0+ndays-1 of one variable for one domain, given
+ * on its meteo grid (L2) and mapped to the domain's L1 cells on the device
+ * as meteo_forcings_wrapper does after reading them
+ * (src/meteo/mo_meteo_helper.f90:119-143): L1 cells coarser than L2 average
+ * the valid L2 cells of their block (spatial_aggregation,
+ * src/meteo/mo_meteo_spatial_tools.f90:94-200), finer ones take their L2
+ * cell or nodata -9999 (spatial_disaggregation, 313-377), equal grids copy;
+ * then pack(L1, mask1).  data2: X(nrows2, ncols2, ndays) column-major;
+ * mask2, mask1: column-major grids, nonzero = valid; mask1 holds exactly the
+ * domain's cells in their order.  A valid cell that maps outside the other
+ * grid (where the reference indexes out of bounds) is an error. */
+int mhmhip_set_forcing_l2(mhmhip_ctx *ctx, int32_t var, int32_t domain, int32_t day0,
+                          int32_t ndays, const double *data2, int32_t nrows2, int32_t ncols2,
+                          double cellsize2, const uint8_t *mask2, int32_t nrows1,
+                          int32_t ncols1, double cellsize1, const uint8_t *mask1);
 
 /* ---- one step ----------------------------------------------------------------
  * get_corrected_pet + get_temp + get_prec + mHM() for every cell and set:

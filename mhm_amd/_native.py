@@ -81,6 +81,9 @@ SIGNATURES = {
     "mhmhip_get_cell_var": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int64, C.c_int32]),
     "mhmhip_default_states": (C.c_int, [C.c_void_p, _dp]),
     "mhmhip_set_forcing": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int64]),
+    "mhmhip_set_forcing_l2": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                        _dp, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
+                                        C.c_int32, C.c_int32, C.c_double, C.c_void_p]),
     "mhmhip_mhm_step": (C.c_int, [C.c_void_p, C.POINTER(Step)]),
     "mhmhip_routing_order": (C.c_int, [C.c_int64, C.c_int64, _ip, _ip, _ip, _ip]),
     "mhmhip_plan_network": (C.c_int, [C.c_int64, C.c_int32, _ip, _ip, _ip, C.c_void_p, _lp, _ip,

@@ -551,6 +551,133 @@ extern "C" int mhmhip_set_forcing(mhmhip_ctx *x, int32_t var, int32_t day0, int3
   return MHMHIP_OK;
 }
 
+namespace {
+// device buffers of one call, freed on every return path
+struct CallBufs {
+  double *data2 = nullptr;
+  uint8_t *mask2 = nullptr;
+  MeteoL2Cell *cells = nullptr;
+  ~CallBufs() {
+    (void)hipFree(data2);
+    (void)hipFree(mask2);
+    (void)hipFree(cells);
+  }
+};
+}  // namespace
+
+extern "C" int mhmhip_set_forcing_l2(mhmhip_ctx *x, int32_t var, int32_t domain, int32_t day0,
+                                     int32_t ndays, const double *data2, int32_t nr2,
+                                     int32_t nc2, double cs2, const uint8_t *mask2, int32_t nr1,
+                                     int32_t nc1, double cs1, const uint8_t *mask1) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!data2 || !mask2 || !mask1 || var < 0 || var >= MHMHIP_F_COUNT || domain < 0 ||
+      domain >= x->n_domains || day0 < 1 || ndays < 1 ||
+      (int64_t)day0 - 1 + ndays > x->cfg.n_forcing_days || nr2 < 1 || nc2 < 1 || nr1 < 1 ||
+      nc1 < 1 || (int64_t)nr2 * nc2 >= INT32_MAX || !(cs1 > 0.0) || !(cs2 > 0.0))
+    return x->fail(MHMHIP_EINVAL,
+                   "set_forcing_l2: variable, domain, day range, grid or cell size out of bounds");
+  const int64_t n1 = (int64_t)nr1 * nc1, n2 = (int64_t)nr2 * nc2;
+  const int64_t c_lo = x->dom_cell_start[domain], c_hi = x->dom_cell_start[domain + 1];
+  int64_t ncells = 0;
+  for (int64_t k = 0; k < n1; ++k) ncells += mask1[k] != 0;
+  if (ncells != c_hi - c_lo)
+    return x->fail(MHMHIP_EINVAL, "set_forcing_l2: mask1 must hold the domain's " +
+                                      std::to_string(c_hi - c_lo) + " cells, it holds " +
+                                      std::to_string(ncells));
+  // cellFactorHbyM (mo_meteo_helper.f90:120) picks the mapping
+  const double hbym = cs1 / cs2;
+  const bool aggregate = hbym > 1.0, equal = !(hbym > 1.0) && !(hbym < 1.0);
+  std::vector<MeteoL2Cell> cells((size_t)ncells);
+  if (aggregate) {
+    // ceiling(i / cellFactor) as in spatial_aggregation_3d: the L2 rows and
+    // columns of every L1 row and column (contiguous: the map is monotone)
+    const double f = cs1 / cs2;
+    auto ceil_div = [f](int32_t i) { return (int64_t)std::ceil((double)i / f); };
+    std::vector<int32_t> rlo(nr1, INT32_MAX), rhi(nr1, -1), clo(nc1, INT32_MAX), chi(nc1, -1);
+    for (int32_t i = 1; i <= nr2; ++i) {
+      const int64_t ic = ceil_div(i);
+      if (ic >= 1 && ic <= nr1) {
+        rlo[ic - 1] = std::min(rlo[ic - 1], i - 1);
+        rhi[ic - 1] = std::max(rhi[ic - 1], i - 1);
+      }
+    }
+    for (int32_t j = 1; j <= nc2; ++j) {
+      const int64_t jc = ceil_div(j);
+      if (jc >= 1 && jc <= nc1) {
+        clo[jc - 1] = std::min(clo[jc - 1], j - 1);
+        chi[jc - 1] = std::max(chi[jc - 1], j - 1);
+      }
+    }
+    // nTCells: the valid L2 cells of each L1 cell
+    std::vector<int32_t> ntc((size_t)n1, 0);
+    for (int32_t j = 1; j <= nc2; ++j) {
+      const int64_t jc = ceil_div(j);
+      for (int32_t i = 1; i <= nr2; ++i) {
+        if (!mask2[(int64_t)(j - 1) * nr2 + (i - 1)]) continue;
+        const int64_t ic = ceil_div(i);
+        if (ic > nr1 || jc > nc1)
+          return x->fail(MHMHIP_EINVAL, "set_forcing_l2: a valid L2 cell lies outside the L1 grid");
+        ntc[(size_t)((jc - 1) * nr1 + (ic - 1))] += 1;
+      }
+    }
+    int64_t c = 0;
+    for (int32_t j = 0; j < nc1; ++j)
+      for (int32_t i = 0; i < nr1; ++i) {
+        const int64_t k = (int64_t)j * nr1 + i;
+        if (!mask1[k]) continue;
+        MeteoL2Cell &m = cells[(size_t)c];
+        m.r0 = rlo[i];
+        m.r1 = rhi[i];
+        m.c0 = clo[j];
+        m.c1 = chi[j];
+        m.n = ntc[(size_t)k];
+        m.src = -1;
+        m.dst = x->pos[c_lo + c];
+        ++c;
+      }
+  } else {
+    if (equal && (nr1 != nr2 || nc1 != nc2))
+      return x->fail(MHMHIP_EINVAL, "set_forcing_l2: equal cell sizes need equal grids");
+    const double f = cs2 / cs1;
+    auto ceil_div = [f](int32_t i) { return (int64_t)std::ceil((double)i / f); };
+    int64_t c = 0;
+    for (int32_t j = 0; j < nc1; ++j)
+      for (int32_t i = 0; i < nr1; ++i) {
+        const int64_t k = (int64_t)j * nr1 + i;
+        if (!mask1[k]) continue;
+        int64_t src = k;
+        if (!equal) {
+          // spatial_disaggregation_3d: the L2 cell (ceiling(i / f), ceiling(j / f))
+          const int64_t ic = ceil_div(i + 1), jc = ceil_div(j + 1);
+          if (ic > nr2 || jc > nc2)
+            return x->fail(MHMHIP_EINVAL,
+                           "set_forcing_l2: a valid L1 cell lies outside the L2 grid");
+          const int64_t k2 = (jc - 1) * nr2 + (ic - 1);
+          src = mask2[k2] ? k2 : -1;
+        }
+        cells[(size_t)c] = MeteoL2Cell{0, -1, 0, -1, 0, (int32_t)src, x->pos[c_lo + c]};
+        ++c;
+      }
+  }
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_forcing_l2");
+  CallBufs b;
+  CHECK_HIP(x, hipMalloc((void **)&b.data2, sizeof(double) * n2 * ndays), "set_forcing_l2 alloc");
+  CHECK_HIP(x, hipMalloc((void **)&b.mask2, (size_t)n2), "set_forcing_l2 alloc");
+  CHECK_HIP(x, hipMalloc((void **)&b.cells, sizeof(MeteoL2Cell) * std::max<int64_t>(1, ncells)),
+            "set_forcing_l2 alloc");
+  CHECK_HIP(x, hipMemcpyAsync(b.data2, data2, sizeof(double) * n2 * ndays, hipMemcpyHostToDevice,
+                              x->stream), "set_forcing_l2 upload");
+  CHECK_HIP(x, hipMemcpyAsync(b.mask2, mask2, (size_t)n2, hipMemcpyHostToDevice, x->stream),
+            "set_forcing_l2 upload");
+  CHECK_HIP(x, hipMemcpyAsync(b.cells, cells.data(), sizeof(MeteoL2Cell) * ncells,
+                              hipMemcpyHostToDevice, x->stream), "set_forcing_l2 upload");
+  double *dst = x->F + ((int64_t)var * x->cfg.n_forcing_days + (day0 - 1)) * x->sstride;
+  CHECK_HIP(x, launch_meteo_l2(b.data2, b.mask2, nr2, n2, b.cells, ncells, ndays, dst, x->sstride,
+                               aggregate, x->stream), "set_forcing_l2 map");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_forcing_l2");
+  return MHMHIP_OK;
+}
+
 // ---------------------------------------------------------------------------
 // step scalars
 

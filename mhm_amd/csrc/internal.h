@@ -116,6 +116,16 @@ __host__ __device__ inline int nm_depth(int32_t y) { return y & 0xFFFFF; }
 __host__ __device__ inline int nm_nchild(int32_t y) { return (y >> 20) & 0xF; }
 constexpr int32_t NF_MASK = NF_LINK | NF_QOUT | NF_CUT | NF_GAUGE | NF_INFLOW;
 
+// One L1 cell of the meteo-grid mapping (meteo_l2_kernel): its L2 block
+// (0-based rows r0..r1, columns c0..c1; empty when r0 > r1) and the number n
+// of valid L2 cells in it when aggregating, else the source L2 cell (src, -1:
+// nodata); dst: the cell's index in a forcing row.
+struct MeteoL2Cell {
+  int32_t r0, r1, c0, c1;
+  int32_t n, src;
+  int64_t dst;
+};
+
 // Tiled routing.  Each domain's tree is cut into connected tiles of at most
 // kTileSlots slots and kTileHeight levels; a tile holds its nodes plus one
 // proxy slot per upstream node that roots another tile.  Tiles are packed

@@ -749,6 +749,49 @@ __global__ __launch_bounds__(kTileSlots) void route_tile_kernel(RouteArgs r, Til
   }
 }
 
+// Forcing on the meteo grid (L2) to the L1 cells of one domain, as
+// meteo_forcings_wrapper maps it (mo_meteo_helper.f90:119-143).  One thread
+// per (L1 cell, day).  Aggregation (L1 coarser, mo_meteo_spatial_tools.f90:
+// 94-200): the valid L2 cells of the cell's block added column by column (j
+// outer, i inner, the reference's order, from 0.0) and divided by their
+// count (0/0 gives NaN, as there).  Disaggregation (313-377) and equal grids
+// (mo_meteo_helper.f90:129-131): the source L2 cell, or nodata (-9999).
+__global__ __launch_bounds__(256) void meteo_l2_kernel(const double *data2, const uint8_t *mask2,
+                                                       int32_t nr2, int64_t n2,
+                                                       const MeteoL2Cell *cells, int64_t ncells,
+                                                       int32_t ndays, double *F,
+                                                       int64_t day_stride, int32_t aggregate) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const MeteoL2Cell m = cells[c];
+  for (int32_t t = blockIdx.y; t < ndays; t += gridDim.y) {
+    const double *d2 = data2 + (int64_t)t * n2;
+    double v;
+    if (aggregate) {
+      double sum = 0.0;
+      for (int32_t j = m.c0; j <= m.c1; ++j)
+        for (int32_t i = m.r0; i <= m.r1; ++i) {
+          const int64_t k = (int64_t)j * nr2 + i;
+          if (mask2[k]) sum = sum + d2[k];
+        }
+      v = sum / (double)m.n;
+    } else {
+      v = m.src >= 0 ? d2[m.src] : -9999.0;
+    }
+    F[(int64_t)t * day_stride + m.dst] = v;
+  }
+}
+
+hipError_t launch_meteo_l2(const double *data2, const uint8_t *mask2, int32_t nr2, int64_t n2,
+                           const MeteoL2Cell *cells, int64_t ncells, int32_t ndays, double *F,
+                           int64_t day_stride, bool aggregate, hipStream_t s) {
+  if (ncells <= 0 || ndays <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((ncells + 255) / 256), (unsigned)(ndays < 65535 ? ndays : 65535));
+  hipLaunchKernelGGL(meteo_l2_kernel, grid, dim3(256), 0, s, data2, mask2, nr2, n2, cells, ncells,
+                     ndays, F, day_stride, aggregate ? 1 : 0);
+  return hipGetLastError();
+}
+
 // Inflow gauges for the tiled routing (add_inflow, mo_mrm_pre_routing.f90:
 // 179-214, as route_node applies them): the qOUT of an inflow node, in the
 // ring rows of steps [t_lo, t_hi), is replaced by the gauge's discharge, or

@@ -18,6 +18,9 @@ hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_
                              int64_t n_bins, int32_t L, hipStream_t s);
 hipError_t launch_inflow_patch(const RouteArgs &r, double *ring, const uint8_t *lag, int64_t t_lo,
                                int64_t t_hi, hipStream_t s);
+hipError_t launch_meteo_l2(const double *data2, const uint8_t *mask2, int32_t nr2, int64_t n2,
+                           const MeteoL2Cell *cells, int64_t ncells, int32_t ndays, double *F,
+                           int64_t day_stride, bool aggregate, hipStream_t s);
 hipError_t launch_route_wg(const RouteArgs &r, const int64_t *dom_start, const int32_t *dom_dmax,
                            int n_domains, int64_t d_lo, int64_t d_hi, int drain, int block,
                            hipStream_t s);

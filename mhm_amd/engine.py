@@ -116,6 +116,22 @@ class Model:
         self._chk(N.lib().mhmhip_set_forcing(self._ctx, N.F[name], day0, data.shape[0],
                                              N.dptr(data), data.shape[1]))
 
+    def set_forcing_l2(self, name, domain, data2, mask2, cellsize2, mask1, cellsize1, day0=1):
+        """One domain's forcing on its meteo grid (L2), mapped to its L1 cells on
+        the device as meteo_forcings_wrapper does (mo_meteo_helper.f90:119-143).
+        data2: (n_days, nrows2, ncols2); mask2 (nrows2, ncols2), mask1 (nrows1,
+        ncols1) booleans, row i / column j as the reference's (i, j)."""
+        data2 = np.asarray(data2, dtype=np.float64)
+        nd, nr2, nc2 = data2.shape
+        d2 = np.ascontiguousarray(np.transpose(data2, (0, 2, 1)))  # X(nrows2, ncols2, ndays)
+        m2 = np.ascontiguousarray(np.asarray(mask2, dtype=bool).T.astype(np.uint8))
+        m1b = np.asarray(mask1, dtype=bool)
+        nr1, nc1 = m1b.shape
+        m1 = np.ascontiguousarray(m1b.T.astype(np.uint8))
+        self._chk(N.lib().mhmhip_set_forcing_l2(
+            self._ctx, N.F[name], int(domain), int(day0), nd, N.dptr(d2), nr2, nc2,
+            float(cellsize2), m2.ctypes.data, nr1, nc1, float(cellsize1), m1.ctypes.data))
+
     def set_domain_forcing(self, day0=1):
         """Upload every domain's own forcing (synth.make_forcing) for all packed cells."""
         for name in ("PRE", "TAVG", "TMIN", "TMAX"):

@@ -732,3 +732,101 @@ void or_routing_order(int64_t n_nodes, int64_t n_links, const int32_t *fromN,
   }
   for (ii = 0; ii < n_links; ++ii) netPerm[rOrder[ii] - 1] = (int32_t)(ii + 1);
 }
+
+/* ------------------------------------------------------------------------
+ * Forcing on the meteo grid (L2) to the L1 cells.
+ * meteo_forcings_wrapper, mo_meteo_helper.f90:119-143; the spatial tools
+ * mo_meteo_spatial_tools.f90:94-200 (aggregation) and 313-377
+ * (disaggregation).  Loops in the reference's order: the aggregation adds
+ * the valid L2 cells column by column (j outer, i inner) into their L1
+ * cell, then divides by the count; nodata_dp = -9999 (mo_common_constants).
+ * ------------------------------------------------------------------------ */
+int or_meteo_l2_to_l1(int32_t nr2, int32_t nc2, double cellsize2, const uint8_t *mask2,
+                      int32_t nr1, int32_t nc1, double cellsize1, const uint8_t *mask1,
+                      int32_t ndays, const double *data2, double *out) {
+  const double nodata = -9999.0;
+  const int64_t n1 = (int64_t)nr1 * nc1, n2 = (int64_t)nr2 * nc2;
+  int64_t ncells = 0;
+  for (int64_t k = 0; k < n1; ++k) ncells += mask1[k] != 0;
+  double *grid1 = (double *)malloc(sizeof(double) * (size_t)n1);
+  if (!grid1) return -1;
+  const double hbym = cellsize1 / cellsize2;  /* cellFactorHbyM (line 120) */
+  int rc = 0;
+  if (hbym > 1.0) {
+    /* spatial_aggregation_3d */
+    const double f = cellsize1 / cellsize2;
+    int32_t *ntc = (int32_t *)calloc((size_t)n1, sizeof(int32_t));
+    if (!ntc) {
+      free(grid1);
+      return -1;
+    }
+    for (int32_t j = 1; j <= nc2 && rc == 0; ++j) {
+      const int64_t jc = (int64_t)ceil((double)j / f);
+      for (int32_t i = 1; i <= nr2; ++i) {
+        const int64_t ic = (int64_t)ceil((double)i / f);
+        if (!mask2[(int64_t)(j - 1) * nr2 + (i - 1)]) continue;
+        if (ic > nr1 || jc > nc1) {
+          rc = -1;
+          break;
+        }
+        ntc[(jc - 1) * nr1 + (ic - 1)] += 1;
+      }
+    }
+    for (int32_t t = 0; t < ndays && rc == 0; ++t) {
+      for (int64_t k = 0; k < n1; ++k) grid1[k] = 0.0;
+      const double *d2 = data2 + (int64_t)t * n2;
+      for (int32_t j = 1; j <= nc2; ++j) {
+        const int64_t jc = (int64_t)ceil((double)j / f);
+        for (int32_t i = 1; i <= nr2; ++i) {
+          const int64_t ic = (int64_t)ceil((double)i / f);
+          const int64_t k2 = (int64_t)(j - 1) * nr2 + (i - 1);
+          if (!mask2[k2]) continue;
+          grid1[(jc - 1) * nr1 + (ic - 1)] = grid1[(jc - 1) * nr1 + (ic - 1)] + d2[k2];
+        }
+      }
+      for (int64_t k = 0; k < n1; ++k)
+        grid1[k] = mask1[k] ? grid1[k] / (double)ntc[k] : nodata;
+      int64_t c = 0;
+      for (int64_t k = 0; k < n1; ++k)
+        if (mask1[k]) out[(int64_t)t * ncells + c++] = grid1[k];
+    }
+    free(ntc);
+  } else if (hbym < 1.0) {
+    /* spatial_disaggregation_3d */
+    const double f = cellsize2 / cellsize1;
+    for (int32_t t = 0; t < ndays && rc == 0; ++t) {
+      const double *d2 = data2 + (int64_t)t * n2;
+      for (int32_t j = 1; j <= nc1 && rc == 0; ++j) {
+        const int64_t jc = (int64_t)ceil((double)j / f);
+        for (int32_t i = 1; i <= nr1; ++i) {
+          const int64_t ic = (int64_t)ceil((double)i / f);
+          const int64_t k1 = (int64_t)(j - 1) * nr1 + (i - 1);
+          grid1[k1] = nodata;
+          if (ic > nr2 || jc > nc2) {
+            if (mask1[k1]) {
+              rc = -1;
+              break;
+            }
+            continue;
+          }
+          const int64_t k2 = (jc - 1) * nr2 + (ic - 1);
+          if (!mask2[k2]) continue;
+          grid1[k1] = d2[k2];
+        }
+      }
+      int64_t c = 0;
+      for (int64_t k = 0; k < n1 && rc == 0; ++k)
+        if (mask1[k]) out[(int64_t)t * ncells + c++] = grid1[k];
+    }
+  } else {
+    /* equal resolution: L1_data = L2_data (mo_meteo_helper.f90:129-131) */
+    if (nr1 != nr2 || nc1 != nc2) rc = -1;
+    for (int32_t t = 0; t < ndays && rc == 0; ++t) {
+      int64_t c = 0;
+      for (int64_t k = 0; k < n1; ++k)
+        if (mask1[k]) out[(int64_t)t * ncells + c++] = data2[(int64_t)t * n2 + k];
+    }
+  }
+  free(grid1);
+  return rc;
+}

@@ -183,6 +183,20 @@ void or_mrm_routing(const or_route_cfg *rc, const double param[5], int64_t n_l1,
 void or_routing_order(int64_t n_nodes, int64_t n_links, const int32_t *fromN,
                       const int32_t *toN, int32_t *rOrder, int32_t *netPerm);
 
+/* ---- forcing on the meteo grid (L2) to the L1 cells ----------------------
+ * meteo_forcings_wrapper's mapping (mo_meteo_helper.f90:119-143): L1 cells
+ * coarser than L2 average the valid L2 cells of their block
+ * (spatial_aggregation_3d, mo_meteo_spatial_tools.f90:94-200), finer ones take
+ * their L2 cell's value or nodata (spatial_disaggregation_3d, 313-377), equal
+ * grids copy; then pack(L1, mask1) (column-major over the masked cells).
+ * data2: X(nrows2, ncols2, ndays); masks: column-major, nonzero = valid;
+ * out: X(n_cells1, ndays) with n_cells1 = count(mask1).  Returns 0, or -1 when
+ * a valid cell maps outside the other grid (where the reference indexes out
+ * of bounds). */
+int or_meteo_l2_to_l1(int32_t nrows2, int32_t ncols2, double cellsize2, const uint8_t *mask2,
+                      int32_t nrows1, int32_t ncols1, double cellsize1, const uint8_t *mask1,
+                      int32_t ndays, const double *data2, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,8 @@ def lib():
                                    _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip, _ip, _ip, _dp,
                                    _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
             "or_routing_order": (v, [C.c_int64, C.c_int64, _ip, _ip, _ip, _ip]),
+            "or_meteo_l2_to_l1": (i, [C.c_int32, C.c_int32, d, C.c_void_p, C.c_int32,
+                                      C.c_int32, d, C.c_void_p, C.c_int32, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -171,3 +173,21 @@ def runoff_sat_zone(k2, sat):
     s, b = ref(sat), ref()
     lib().or_runoff_sat_zone(k2, C.byref(s), C.byref(b))
     return s.value, b.value
+
+
+def meteo_l2_to_l1(data2, mask2, cellsize2, mask1, cellsize1):
+    """The oracle's meteo_forcings_wrapper mapping.  data2: (n_days, nrows2,
+    ncols2); masks (nrows, ncols) booleans.  Returns (n_days, count(mask1))
+    in pack order (column-major over mask1), or None where the oracle reports
+    a valid cell mapping outside the other grid."""
+    data2 = np.asarray(data2, dtype=np.float64)
+    nd, nr2, nc2 = data2.shape
+    d2 = np.ascontiguousarray(np.transpose(data2, (0, 2, 1)))
+    m2 = np.ascontiguousarray(np.asarray(mask2, dtype=bool).T.astype(np.uint8))
+    m1b = np.asarray(mask1, dtype=bool)
+    nr1, nc1 = m1b.shape
+    m1 = np.ascontiguousarray(m1b.T.astype(np.uint8))
+    out = np.zeros((nd, int(m1b.sum())))
+    rc = lib().or_meteo_l2_to_l1(nr2, nc2, float(cellsize2), m2.ctypes.data, nr1, nc1,
+                                 float(cellsize1), m1.ctypes.data, nd, dptr(d2), dptr(out))
+    return None if rc != 0 else out
