@@ -47,6 +47,12 @@ struct mhmhip_ctx {
   double *P = nullptr, *S = nullptr, *X = nullptr, *F = nullptr;
   int64_t n_prows = 0, n_srows = 0, n_xrows = 0;
   std::vector<int64_t> pos;  // reference cell -> device index (identity until a network is bound)
+  // Nodes of single-node domains: the reference skips L11_routing there
+  // (mo_mrm_routing.f90:285-289) and leaves their qTIN as set (0 at
+  // init); the device's qTIN holds Qmod for them, so their qTIN per set is
+  // kept here.
+  std::vector<int64_t> lone_nodes;
+  std::vector<double> lone_qtin;
   double *stage = nullptr;   // pinned staging row (sstride doubles)
 
   // routing
@@ -1281,6 +1287,10 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->h_inflow_hw = ihw;
   x->lev_start.clear();  // the tile-major layout does not keep depths contiguous
   x->lay = std::move(lay);
+  x->lone_nodes.clear();
+  for (int d = 0; d < x->n_domains; ++d)
+    if (ds[d + 1] - ds[d] == 1) x->lone_nodes.push_back(ds[d]);
+  x->lone_qtin.assign(x->lone_nodes.size() * (size_t)x->n_sets, 0.0);
   x->net_bound = true;
   x->qtr_slot = 1;
   drop_graph(x);
@@ -1380,6 +1390,9 @@ extern "C" int mhmhip_get_routing_state(mhmhip_ctx *x, int32_t set, double *qOUT
   };
   CHECK_RC(node_out(x->d_qOUT + off, qOUT));
   CHECK_RC(node_out(x->d_qTIN + off, qTIN));
+  if (qTIN)
+    for (size_t i = 0; i < x->lone_nodes.size(); ++i)
+      qTIN[x->lone_nodes[i]] = x->lone_qtin[(size_t)set * x->lone_nodes.size() + i];
   CHECK_RC(node_out(x->d_qTIN + off, Qmod));
   CHECK_RC(node_out(x->d_pub + x->qtr_slot * x->ld + off, qTR));
   CHECK_RC(link_out(x->d_C1 + off, C1));
@@ -1428,6 +1441,9 @@ extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double
   const int64_t off = (int64_t)set * x->sstride;
   auto id = [](double a) { return a; };
   if (qTIN) CHECK_RC(put_row(x, x->d_qTIN + off, qTIN, id));
+  if (qTIN)
+    for (size_t i = 0; i < x->lone_nodes.size(); ++i)
+      x->lone_qtin[(size_t)set * x->lone_nodes.size() + i] = qTIN[x->lone_nodes[i]];
   if (qTR) CHECK_RC(put_row(x, x->d_pub + x->qtr_slot * x->ld + off, qTR, id));
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_routing_state");
   return MHMHIP_OK;

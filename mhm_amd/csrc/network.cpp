@@ -154,8 +154,11 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
       }
     }
     // the last-ranked link's downstream node gets its own qOUT
-    // (mo_mrm_routing.f90:466-467); every link's source node does too (445)
-    const int64_t last_t = nL > 0 ? (int64_t)to[perm[nL - 1] - 1] - 1 : -1;
+    // (mo_mrm_routing.f90:466-467); every link's source node does too (445).
+    // A domain of one node takes the single-node shortcut instead, Qmod =
+    // qOUT (285-289): its node adding its own qOUT gives qTIN(IT) = 0 + qOUT,
+    // the same value.
+    const int64_t last_t = nL > 0 ? (int64_t)to[perm[nL - 1] - 1] - 1 : (nd == 1 ? 0 : -1);
     std::vector<int64_t> q;
     q.reserve(nd);
     std::vector<int32_t> depth(nd, -1);

@@ -198,6 +198,34 @@ def test_packed_domains_match_oracle(route_mode):
     model.close()
 
 
+@pytest.mark.parametrize("route_mode", [0, 1, 3])
+def test_single_node_and_tiny_domains_match_oracle(route_mode):
+    """Ragged pack with a one-cell basin (mRM_routing's single-node shortcut,
+    Qmod = qOUT with qTIN untouched, mo_mrm_routing.f90:285-289) and a
+    two-cell one (one link) beside a larger basin.  route_mode 0: the
+    per-step entry points."""
+    _log(f"case: tiny domains route_mode={route_mode}")
+    doms = [synth.make_domain(1, 1, seed=41), synth.make_domain(6, 5, seed=42),
+            synth.make_domain(1, 2, seed=43), synth.make_domain(1, 1, seed=44)]
+    model, sim, steps, recs = make_pair(doms, n_sets=2, n_steps=30, seed=12)
+    if route_mode == 0:
+        for k in range(len(recs)):
+            model.mhm_step(steps[k])
+            g = model.mrm_step(steps[k])
+            go = sim.step(recs[k])
+            ok, worst = rel_close(g, go, RTOL, ATOL)
+            assert ok, (k, worst, g, go)
+    else:
+        g = model.run(steps, mode=1, steps_per_launch=8, route_mode=route_mode)
+        go = _oracle_series(sim, recs)
+        assert rel_close(g, go, RTOL, ATOL)[0]
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    # the lone basins' gauges carry their qOUT, not zero
+    assert np.all(np.abs(g[..., 0]) > 0.0) or np.all(np.abs(go[..., 0]) == 0.0)
+    model.close()
+
+
 @pytest.mark.parametrize("soil,pet,timestep", [(2, 0, 1), (3, -1, 24), (4, 1, 24), (1, 0, 3)])
 def test_process_variants_match_oracle(soil, pet, timestep):
     _log(f"case: variant soil={soil} pet={pet} timestep={timestep}")
