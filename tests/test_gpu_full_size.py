@@ -1,5 +1,7 @@
-"""The headline basin at its full size: 1024 x 1024 cells, D8 network of
-depth 1848 (1,048,575 links), 3 horizons, hourly steps.
+"""The workloads at their full sizes.  The headline basin: 1024 x 1024
+cells, D8 network of depth 1848 (1,048,575 links), 3 horizons, hourly steps;
+the 256-set ensemble (64 x 64 cells, daily x 3650) and the 64-basin pack
+(hourly x 720) likewise.
 
 * Parity: 48 steps of the pipelined time loop (fused cells S = 24, tiled
   routing in windows of 64, hipGraph) against the oracle on the same
@@ -78,3 +80,58 @@ def test_full_size_schedules_agree_bit_for_bit(basin):
         assert np.array_equal(s_fused[k], s_plain[k]), k
     for k in ("qTIN", "qTR", "Qmod"):
         assert np.array_equal(r_fused[k], r_plain[k]), k
+
+
+def _parity_then_schedules(model, sim, steps, recs, n_check, S_check, S_full):
+    """The first n_check steps against the oracle, then the whole horizon
+    fused/tiled against plain/diagonal from the same states, bit for bit."""
+    snap = model.snapshot()
+    g = model.run(steps[:n_check], mode=N.RUN_FUSED, steps_per_launch=S_check, route_mode=3,
+                  use_graph=True)
+    go = np.stack([sim.step(r) for r in recs[:n_check]])
+    ok, worst = rel_close(g, go, RTOL, ATOL)
+    assert ok, worst
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    model.restore(snap)
+    g_fused = model.run(steps, mode=N.RUN_FUSED, steps_per_launch=S_full, route_mode=3,
+                        route_window=64, use_graph=True)
+    s_fused = [{k: model.get(k, s) for k in STATES} for s in range(model.n_sets)]
+    r_fused = [model.routing_state(s) for s in range(model.n_sets)]
+    model.restore(snap)
+    g_plain = model.run(steps, mode=N.RUN_PLAIN, route_mode=1)
+    assert np.isfinite(g_fused).all()
+    assert np.array_equal(g_fused, g_plain)
+    for s in range(model.n_sets):
+        for k in STATES:
+            assert np.array_equal(s_fused[s][k], model.get(k, s)), (s, k)
+        r = model.routing_state(s)
+        for k in ("qTIN", "qTR", "Qmod"):
+            assert np.array_equal(r_fused[s][k], r[k]), (s, k)
+
+
+def test_ensemble_full_size():
+    """256 parameter sets x 4096 cells (64 x 64), daily x 3650 steps."""
+    _log("case: full size ensemble 256 sets x 64x64, daily x 3650")
+    dom = synth.make_domain(64, 64, seed=3)
+    model, sim, steps, recs = make_pair([dom], n_sets=256, timestep=24, n_steps=3650, seed=8)
+    _parity_then_schedules(model, sim, steps, recs, n_check=30, S_check=10, S_full=365)
+    model.close()
+
+
+def test_multi_basin_full_size():
+    """64 basins, log-uniform 500..200k cells, each its own DEM, network,
+    forcing and parameters, packed in one context; hourly x 720 steps."""
+    import math
+    rng = np.random.default_rng(7)
+    sizes = np.exp(rng.uniform(math.log(500), math.log(200000), 64)).astype(int)
+    doms = []
+    for b, n in enumerate(sizes):
+        nr = max(2, int(round(math.sqrt(n) * rng.uniform(0.7, 1.3))))
+        nc = max(2, int(round(n / nr)))
+        doms.append(synth.make_domain(nr, nc, seed=7000 + b))
+    _log(f"case: full size multi-basin, {sum(d.n_cells for d in doms)} cells in 64 basins, "
+         f"hourly x 720")
+    model, sim, steps, recs = make_pair(doms, n_sets=1, n_steps=720, seed=9)
+    _parity_then_schedules(model, sim, steps, recs, n_check=24, S_check=12, S_full=24)
+    model.close()
