@@ -267,6 +267,20 @@ int mhmhip_bind_network(mhmhip_ctx *ctx, const mhmhip_network *net);
  * (src/mRM/mo_mrm_mpr.f90:61-119) over that domain's links. */
 int mhmhip_set_routing_params(mhmhip_ctx *ctx, int32_t set, int32_t domain,
                               const double gamma[5]);
+/* Adaptive-timestep routing with a constant celerity [m s-1]
+ * (processCase(8) = 2) for one parameter set and domain (domain < 0: every
+ * domain): mrm_update_param (src/mRM/mo_mrm_mpr.f90:268-329) -- K = link
+ * length / celerity, the routing step TSrout = given_TS(locate(given_TS,
+ * min K)) (src/mRM/mo_mrm_constants.F90:42-46), C1 = TSrout / (K + TSrout/2),
+ * C2 = 1 - C1 K / TSrout.  mhmhip_mrm_step then routes each step's qOUT
+ * rout_loop = max(1, nint(timestep / TSrout)) times and reports Qmod and the
+ * gauges as the average over them (src/mRM/mo_mrm_routing.f90:224-284);
+ * mhmhip_run takes the case rout_loop = 1 only.  Routing steps longer than
+ * the model step (runoff accumulated over several steps,
+ * src/mHM/mo_mhm_interface_run.f90:494-512) are rejected, as are sub-step
+ * counts that differ between the sets or domains of one context.
+ * mhmhip_set_routing_params returns a (set, domain) to processCase 1. */
+int mhmhip_set_routing_celerity(mhmhip_ctx *ctx, int32_t set, int32_t domain, double celerity);
 /* Inflow discharge [m3 s-1] of every inflow gauge for the next mrm step(s):
  * q[step][n_inflow] for nsteps steps (InflowDischarge). */
 int mhmhip_set_inflow(mhmhip_ctx *ctx, int32_t nsteps, const double *q);

@@ -90,6 +90,7 @@ SIGNATURES = {
                                       C.c_int64]),
     "mhmhip_bind_network": (C.c_int, [C.c_void_p, C.POINTER(Network)]),
     "mhmhip_set_routing_params": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
+    "mhmhip_set_routing_celerity": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "mhmhip_set_inflow": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mhmhip_get_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mhmhip_set_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,6 +54,12 @@ struct mhmhip_ctx {
   // kept here.
   std::vector<int64_t> lone_nodes;
   std::vector<double> lone_qtin;
+  // adaptive routing (processCase(8) = 2): the routing step [s] of each
+  // (set, domain), 0 for processCase 1; the Qmod sums of the last
+  // mrm_step's sub-steps and their count (> 1 when they apply)
+  std::vector<double> ts_rout;
+  double *d_qsub = nullptr;
+  int32_t qsub_loops = 0;
   double *stage = nullptr;   // pinned staging row (sstride doubles)
 
   // routing
@@ -333,6 +340,7 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_lag);
   (void)hipFree(x->d_slot_out);
   (void)hipFree(x->d_qacc);
+  (void)hipFree(x->d_qsub);
   (void)hipFree(x->d_inflow_node);
   (void)hipFree(x->d_inflow_hw);
   (void)hipFree(x->d_inflow_q);
@@ -1184,6 +1192,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->dfree(x->d_node_gauge, x->n_cells);
   x->dfree(x->d_lag, x->n_cells);
   x->dfree(x->d_qacc, x->ld);  // a new network restarts the output aggregation
+  x->dfree(x->d_qsub, x->ld);
   x->qacc_on = false;
   x->qacc_count = 0;
   x->dfree(x->d_inflow_node, std::max(1, x->n_inflow));
@@ -1291,6 +1300,8 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   for (int d = 0; d < x->n_domains; ++d)
     if (ds[d + 1] - ds[d] == 1) x->lone_nodes.push_back(ds[d]);
   x->lone_qtin.assign(x->lone_nodes.size() * (size_t)x->n_sets, 0.0);
+  x->ts_rout.assign((size_t)x->n_sets * x->n_domains, 0.0);
+  x->qsub_loops = 0;
   x->net_bound = true;
   x->qtr_slot = 1;
   drop_graph(x);
@@ -1322,6 +1333,7 @@ extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, int32_t dom
   std::vector<char> touched(std::max<int64_t>(nLtot, 1), 0);
   for (int d = 0; d < x->n_domains; ++d) {
     if (domain >= 0 && d != domain) continue;
+    x->ts_rout[(size_t)set * x->n_domains + d] = 0.0;  // processCase 1
     const int64_t l0 = x->dom_link_start[d], l1 = x->dom_link_start[d + 1];
     for (int64_t i = l0; i < l1; ++i) touched[i] = 1;
     if (l1 <= l0) continue;
@@ -1348,6 +1360,99 @@ extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, int32_t dom
   }
   CHECK_HIP(x, sync_copy(x, x->d_C1 + off, n1.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C1");
   CHECK_HIP(x, sync_copy(x, x->d_C2 + off, n2.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C2");
+  return MHMHIP_OK;
+}
+
+namespace {
+// the routing steps a domain may take [s]: given_TS (mo_mrm_constants.F90:42-46)
+constexpr double kGivenTS[19] = {60.0,    120.0,   180.0,   240.0,   300.0,   360.0,  600.0,
+                                 720.0,   900.0,   1200.0,  1800.0,  3600.0,  7200.0, 10800.0,
+                                 14400.0, 21600.0, 28800.0, 43200.0, 86400.0};
+// FORCES mo_utils locate over given_TS (the first minimum of |x - y|; c if
+// LE(x(c), y), else c - 1; 1-based) with LE = lesserequal (a <= b within
+// epsilon): restated from the copy in pre-proc/mtCLIM/mo_utils.f90:924-936,
+// 1329-1346
+bool le_eps(double a, double b) {
+  return !(((DBL_EPSILON * std::fabs(b) - std::fabs(a - b)) < 0.0) && (a > b));
+}
+int locate_ts(double y) {
+  int c = 0;
+  for (int i = 1; i < 19; ++i)
+    if (std::fabs(kGivenTS[i] - y) < std::fabs(kGivenTS[c] - y)) c = i;
+  return le_eps(kGivenTS[c], y) ? c + 1 : c;
+}
+
+// Routing sub-steps per mrm_step: every (set, domain) with links must agree.
+int rout_loops(mhmhip_ctx *x, int32_t *loops) {
+  int32_t common = 0;
+  for (int s = 0; s < x->n_sets; ++s)
+    for (int d = 0; d < x->n_domains; ++d) {
+      if (x->dom_link_start[d + 1] <= x->dom_link_start[d]) continue;
+      const double ts = x->ts_rout[(size_t)s * x->n_domains + d];
+      int32_t n = 1;
+      if (ts > 0.0) {
+        // tsRoutFactor (mo_mhm_interface_run.f90:482), rout_loop (mo_mrm_routing.f90:224)
+        const double f = ts / ((double)x->cfg.timestep * kHourSecs);
+        if (f > 1.0)
+          return x->fail(MHMHIP_EINVAL, "routing step longer than the model step (runoff "
+                                        "accumulated over several steps) is not supported");
+        n = std::max<int32_t>(1, (int32_t)std::floor(1.0 / f + 0.5));
+      }
+      if (common != 0 && n != common)
+        return x->fail(MHMHIP_EINVAL, "routing sub-steps differ between parameter sets or "
+                                      "domains of one context");
+      common = n;
+    }
+  *loops = common > 0 ? common : 1;
+  return MHMHIP_OK;
+}
+}  // namespace
+
+extern "C" int mhmhip_set_routing_celerity(mhmhip_ctx *x, int32_t set, int32_t domain,
+                                           double celerity) {
+  if (!x) return MHMHIP_EINVAL;
+  if (set < 0 || set >= x->n_sets || domain >= x->n_domains || !(celerity > 0.0))
+    return x->fail(MHMHIP_EINVAL, "set_routing_celerity: bad set, domain or celerity");
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_celerity: bind a network first");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_celerity");
+  const int64_t nLtot = x->dom_link_start[x->n_domains];
+  const int64_t off = (int64_t)set * x->sstride;
+  std::vector<double> n1(x->sstride, 0.0), n2(x->sstride, 0.0);
+  CHECK_HIP(x, sync_copy(x, n1.data(), x->d_C1 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C1");
+  CHECK_HIP(x, sync_copy(x, n2.data(), x->d_C2 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C2");
+  std::vector<double> c1(std::max<int64_t>(nLtot, 1), 0.0), c2(std::max<int64_t>(nLtot, 1), 0.0);
+  std::vector<char> touched(std::max<int64_t>(nLtot, 1), 0);
+  const double xi = std::fabs(0.0);  // rout_space_weight (mo_mrm_constants.F90:33)
+  for (int d = 0; d < x->n_domains; ++d) {
+    if (domain >= 0 && d != domain) continue;
+    const int64_t l0 = x->dom_link_start[d], l1 = x->dom_link_start[d + 1];
+    if (l1 <= l0) continue;
+    // mrm_update_param (mo_mrm_mpr.f90:302-323): K = length / celerity [s]
+    double kmin = 0.0;
+    for (int64_t i = l0; i < l1; ++i) {
+      const double K = x->h_length[i] / celerity;
+      if (i == l0 || K < kmin) kmin = K;
+    }
+    const int ind = std::max(1, locate_ts(kmin));
+    const double ts = kGivenTS[ind - 1];
+    for (int64_t i = l0; i < l1; ++i) {
+      const double K = x->h_length[i] / celerity;
+      c1[i] = ts / (K * (1.0 - xi) + 0.5 * ts);
+      c2[i] = 1.0 - c1[i] * K / ts;
+      touched[i] = 1;
+    }
+    x->ts_rout[(size_t)set * x->n_domains + d] = ts;
+  }
+  for (int64_t q = 0; q < x->n_cells; ++q) {
+    const int64_t l = x->lay.link_of_node[q];
+    if (l >= 0 && touched[l]) {
+      n1[q] = c1[l];
+      n2[q] = c2[l];
+    }
+  }
+  CHECK_HIP(x, sync_copy(x, x->d_C1 + off, n1.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C1");
+  CHECK_HIP(x, sync_copy(x, x->d_C2 + off, n2.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C2");
+  drop_graph(x);
   return MHMHIP_OK;
 }
 
@@ -1390,10 +1495,15 @@ extern "C" int mhmhip_get_routing_state(mhmhip_ctx *x, int32_t set, double *qOUT
   };
   CHECK_RC(node_out(x->d_qOUT + off, qOUT));
   CHECK_RC(node_out(x->d_qTIN + off, qTIN));
+  if (Qmod && x->qsub_loops > 1) {
+    // the average of the last step's routing sub-steps (qAcc / rout_loop)
+    CHECK_RC(node_out(x->d_qsub + off, Qmod));
+    for (int64_t k = 0; k < x->n_cells; ++k) Qmod[k] = Qmod[k] / (double)x->qsub_loops;
+  }
   if (qTIN)
     for (size_t i = 0; i < x->lone_nodes.size(); ++i)
       qTIN[x->lone_nodes[i]] = x->lone_qtin[(size_t)set * x->lone_nodes.size() + i];
-  CHECK_RC(node_out(x->d_qTIN + off, Qmod));
+  if (x->qsub_loops <= 1) CHECK_RC(node_out(x->d_qTIN + off, Qmod));
   CHECK_RC(node_out(x->d_pub + x->qtr_slot * x->ld + off, qTR));
   CHECK_RC(link_out(x->d_C1 + off, C1));
   CHECK_RC(link_out(x->d_C2 + off, C2));
@@ -1459,27 +1569,51 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
     return x->fail(MHMHIP_EINVAL, "mrm_step: read_states (C1/C2 from a restart) is not supported");
   if (x->n_inflow > 0 && x->inflow_q_rows < 1)
     return x->fail(MHMHIP_ESTATE, "mrm_step: inflow discharge not set");
+  int32_t loops = 1;
+  CHECK_RC(rout_loops(x, &loops));
+  if (loops > 1 && x->qacc_on)
+    return x->fail(MHMHIP_EINVAL, "mrm_step: Qmod output sums with routing sub-steps are not supported");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "mrm_step");
   CHECK_RC(ensure_gauge_buf(x, 1));
-  CHECK_RC(normalize_qtr(x));
-  // this step is step 0 of a one-step ring: mhm_step wrote its qOUT to ring
-  // row lag (the tiles' layout)
-  RouteArgs r = route_args(x, 0, 1, 0);
-  r.ring_len = 1;
   const int kind = MHMHIP_K_ROUTE_TILE;
   const TileArgs g = tile_args(x, 1, 1);
-  if (x->n_inflow > 0)
-    CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, 0, 1, x->stream), "inflow_patch");
-  CHECK_RC(route_tiles(x, r, g, 0, g.nwin + g.tau_max));
-  credit_route(x, kind, 1);
-  x->qtr_slot = 0;
+  const int64_t ng = (int64_t)x->n_sets * x->n_gauges;
+  std::vector<double> gsum(loops > 1 ? ng : 0, 0.0), gone(loops > 1 ? ng : 0);
+  if (loops > 1) {
+    if (!x->d_qsub) CHECK_RC(x->dalloc(&x->d_qsub, x->ld, "alloc Qmod sub-step sums"));
+    CHECK_HIP(x, hipMemsetAsync(x->d_qsub, 0, sizeof(double) * x->ld, x->stream), "mrm_step");
+  }
+  // loops routings of this step's qOUT (mRM_routing, mo_mrm_routing.f90:224-284):
+  // the states advance each time, Qmod is their average
+  for (int32_t it = 0; it < loops; ++it) {
+    CHECK_RC(normalize_qtr(x));
+    // this step is step 0 of a one-step ring: mhm_step wrote its qOUT to ring
+    // row lag (the tiles' layout)
+    RouteArgs r = route_args(x, 0, 1, 0);
+    r.ring_len = 1;
+    if (loops > 1) r.qacc = x->d_qsub;
+    if (x->n_inflow > 0 && it == 0)
+      CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, 0, 1, x->stream), "inflow_patch");
+    CHECK_RC(route_tiles(x, r, g, 0, g.nwin + g.tau_max));
+    credit_route(x, kind, 1);
+    x->qtr_slot = 0;
+    if (loops > 1 && gauge_out && ng > 0) {
+      CHECK_HIP(x, sync_copy(x, gone.data(), x->d_gauge_buf, sizeof(double) * ng,
+                             hipMemcpyDeviceToHost), "download gauges");
+      for (int64_t k = 0; k < ng; ++k) gsum[k] = gsum[k] + gone[k];  // qAcc = qAcc + Qmod
+    }
+  }
+  x->qsub_loops = loops;
   if (x->qacc_on) x->qacc_count += 1;
   x->t_route++;
-  if (gauge_out && x->n_gauges > 0) {
-    CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf,
-                                sizeof(double) * x->n_sets * x->n_gauges, hipMemcpyDeviceToHost,
-                                x->stream),
-              "download gauges");
+  if (gauge_out && ng > 0) {
+    if (loops > 1) {
+      for (int64_t k = 0; k < ng; ++k) gauge_out[k] = gsum[k] / (double)loops;
+    } else {
+      CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf, sizeof(double) * ng,
+                                  hipMemcpyDeviceToHost, x->stream),
+                "download gauges");
+    }
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "mrm_step");
   if (gauge_out) copy_dup_gauges(x, gauge_out, 1);
@@ -1501,6 +1635,14 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   if (o.route_window < 0 || o.route_window > kTileWindowMax)
     return x->fail(MHMHIP_EINVAL, "run: route_window must be 0.." + std::to_string(kTileWindowMax));
   const bool route = routing_on(x);
+  if (route) {
+    int32_t loops = 1;
+    CHECK_RC(rout_loops(x, &loops));
+    if (loops > 1)
+      return x->fail(MHMHIP_EINVAL, "run: routing sub-steps (adaptive routing step shorter "
+                                    "than the model step) run through mhmhip_mrm_step");
+    x->qsub_loops = 0;
+  }
   if (route && x->t_route != x->t_cell)
     return x->fail(MHMHIP_ESTATE, "run: previous step not routed (call mrm_step)");
   for (int32_t k = 0; k < nsteps; ++k)

@@ -184,6 +184,10 @@ class Model:
         g = np.ascontiguousarray(gamma5, dtype=np.float64)
         self._chk(N.lib().mhmhip_set_routing_params(self._ctx, set_, domain, N.dptr(g)))
 
+    def set_routing_celerity(self, celerity, set_=0, domain=-1):
+        """Adaptive-timestep routing, constant celerity [m/s] (processCase(8) = 2)."""
+        self._chk(N.lib().mhmhip_set_routing_celerity(self._ctx, set_, domain, float(celerity)))
+
     def set_inflow(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)
         self._chk(N.lib().mhmhip_set_inflow(self._ctx, q.shape[0], N.dptr(q)))

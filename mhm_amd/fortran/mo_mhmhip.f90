@@ -63,7 +63,8 @@ module mo_mhmhip
 
   public :: mhmhip_create, mhmhip_destroy, mhmhip_last_error, mhmhip_device_bytes, mhmhip_sync, &
             mhmhip_set_cell_var, mhmhip_get_cell_var, mhmhip_default_states, &
-            mhmhip_set_forcing, mhmhip_set_forcing_l2, mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, &
+            mhmhip_set_forcing, mhmhip_set_forcing_l2, mhmhip_set_routing_celerity, &
+            mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, &
             mhmhip_set_routing_params, mhmhip_set_inflow, mhmhip_get_routing_state, &
             mhmhip_set_routing_state, mhmhip_mrm_step, mhmhip_run, mhmhip_set_profiling, &
             mhmhip_get_profile, mhmhip_plan_network, mhmhip_qmod_accumulate, mhmhip_get_qmod_sum
@@ -124,6 +125,13 @@ module mo_mhmhip
       type(c_ptr), value :: ctx, data2, mask2, mask1
       integer(c_int32_t), value :: var, domain, day0, ndays, nrows2, ncols2, nrows1, ncols1
       real(c_double), value :: cellsize2, cellsize1
+    end function
+    integer(c_int) function mhmhip_set_routing_celerity(ctx, set, domain, celerity) &
+        bind(C, name="mhmhip_set_routing_celerity")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: set, domain
+      real(c_double), value :: celerity
     end function
     integer(c_int) function mhmhip_mhm_step(ctx, st) bind(C, name="mhmhip_mhm_step")
       import :: c_int, c_ptr, mhmhip_step

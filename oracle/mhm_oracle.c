@@ -683,10 +683,18 @@ void or_mrm_routing(const or_route_cfg *rc, const double param[5], int64_t n_l1,
                     rc->timestep, rc->map_flag, qOUT);
   or_add_inflow(rc->n_inflow, inflow_index, inflow_hw, inflow_node, inflow_q, qOUT);
   if (nN > 1) {
-    /* rout_loop = 1 for processCase 1 (tsRoutFactor = 1); qAcc = 0 + Qmod; Qmod = qAcc / 1 */
-    or_L11_routing(nN, nL, netPerm, fromN, toN, C1, C2, qOUT, rc->n_inflow, inflow_hw,
-                   inflow_node, qTIN, qTR, Qmod);
-    for (k = 0; k < nN; ++k) Qmod[k] = (0.0 + Qmod[k]) / 1.0;
+    /* rout_loop routings of the same qOUT, Qmod averaged (lines 224, 243-284):
+     * qAcc = 0, qAcc = qAcc + Qmod per loop, Qmod = qAcc / rout_loop */
+    const int32_t loops = rc->rout_loop > 1 ? rc->rout_loop : 1;
+    double *acc = (double *)calloc((size_t)nN, sizeof(double));
+    if (!acc) return;
+    for (int32_t it = 0; it < loops; ++it) {
+      or_L11_routing(nN, nL, netPerm, fromN, toN, C1, C2, qOUT, rc->n_inflow, inflow_hw,
+                     inflow_node, qTIN, qTR, Qmod);
+      for (k = 0; k < nN; ++k) acc[k] = acc[k] + Qmod[k];
+    }
+    for (k = 0; k < nN; ++k) Qmod[k] = acc[k] / (double)loops;
+    free(acc);
   } else {
     for (k = 0; k < nN; ++k) Qmod[k] = qOUT[k];
   }
@@ -829,4 +837,53 @@ int or_meteo_l2_to_l1(int32_t nr2, int32_t nc2, double cellsize2, const uint8_t 
   }
   free(grid1);
   return rc;
+}
+
+/* ------------------------------------------------------------------------
+ * Adaptive routing parameters, constant celerity (processCase(8) = 2):
+ * mrm_update_param, mo_mrm_mpr.f90:268-329.  given_TS: mo_mrm_constants.F90
+ * :42-46.  locate and LE as FORCES mo_utils, restated from the copy in
+ * pre-proc/mtCLIM/mo_utils.f90 (lesserequal_dp 924-936, locate_0d_dp 1329-1346).
+ * ------------------------------------------------------------------------ */
+static const double kGivenTS[19] = {60.0,   120.0,   180.0,   240.0,   300.0,   360.0,  600.0,
+                                    720.0,  900.0,   1200.0,  1800.0,  3600.0,  7200.0, 10800.0,
+                                    14400.0, 21600.0, 28800.0, 43200.0, 86400.0};
+
+/* lesserequal_dp, pre-proc/mtCLIM/mo_utils.f90:924-936 */
+static int or_le(double a, double b) {
+  return !(((DBL_EPSILON * fabs(b) - fabs(a - b)) < 0.0) && (a > b));
+}
+
+/* locate(x, y): c = minloc(|x - y|) (first minimum), c if LE(x(c), y) else c - 1 (1-based) */
+static int or_locate(const double *x, int n, double y) {
+  int c = 0;
+  for (int i = 1; i < n; ++i)
+    if (fabs(x[i] - y) < fabs(x[c] - y)) c = i;
+  return or_le(x[c], y) ? c + 1 : c;
+}
+
+void or_mrm_update_param_celerity(int64_t n_links, const double *length, double celerity,
+                                  int32_t timestep, double *C1, double *C2, double *ts_rout,
+                                  double *ts_rout_factor, int32_t *rout_loop) {
+  const double xi = fabs(0.0); /* rout_space_weight */
+  double kmin = 0.0;
+  for (int64_t i = 0; i < n_links; ++i) {
+    const double K = length[i] / celerity;
+    if (i == 0 || K < kmin) kmin = K;
+  }
+  int ind = or_locate(kGivenTS, 19, kmin);
+  if (ind < 1) ind = 1;
+  const double ts = kGivenTS[ind - 1];
+  for (int64_t i = 0; i < n_links; ++i) {
+    const double K = length[i] / celerity;
+    C1[i] = ts / (K * (1.0 - xi) + 0.5 * ts);
+    C2[i] = 1.0 - C1[i] * K / ts;
+  }
+  *ts_rout = ts;
+  /* tsRoutFactor = L11_tsRout / (timestep * HourSecs) */
+  *ts_rout_factor = ts / ((double)timestep * 3600.0);
+  /* rout_loop = max(1, nint(1 / tsRoutFactor)); Fortran nint rounds halves away from zero */
+  const double inv = 1.0 / *ts_rout_factor;
+  const int32_t nl = (int32_t)floor(inv + 0.5);
+  *rout_loop = nl > 1 ? nl : 1;
 }

@@ -158,13 +158,26 @@ typedef struct {
   int64_t n_nodes;
   int32_t n_outlets;
   int32_t timestep;       /* hours                                   */
-  int32_t process_case;   /* processMatrix(8,1); only 1 is restated  */
+  int32_t process_case;   /* processMatrix(8,1): 1, or 2 with C1/C2 given */
   int32_t read_states;
   int32_t map_flag;
   int32_t n_inflow;
   int32_t n_gauges;
-  int32_t pad;
+  int32_t rout_loop;      /* routing sub-steps per call (0 or 1: one)  */
 } or_route_cfg;
+
+/* Adaptive routing parameters for a constant celerity (processCase(8) = 2):
+ * mrm_update_param, src/mRM/mo_mrm_mpr.f90:268-329.  K = length / celerity
+ * [s] per link; the routing step ts_rout = given_TS(locate(given_TS, min K))
+ * (src/mRM/mo_mrm_constants.F90:42-46; locate and LE as in FORCES mo_utils,
+ * restated from pre-proc/mtCLIM/mo_utils.f90:924-936,1329-1346); C1 =
+ * ts_rout / (K (1 - xi) + ts_rout / 2), C2 = 1 - C1 K / ts_rout, xi = 0
+ * (rout_space_weight).  tsRoutFactor = ts_rout / (timestep * 3600) and
+ * rout_loop = max(1, nint(1 / tsRoutFactor)) as mo_mhm_interface_run.f90:
+ * 482-493 and mo_mrm_routing.f90:224 derive them. */
+void or_mrm_update_param_celerity(int64_t n_links, const double *length, double celerity,
+                                  int32_t timestep, double *C1, double *C2, double *ts_rout,
+                                  double *ts_rout_factor, int32_t *rout_loop);
 
 /* One mRM_routing() call.  Reference: mo_mrm_routing.f90:104-303.
  * n_l1 cells of runoff; gauge_out receives n_gauges values (may be NULL). */

@@ -49,7 +49,7 @@ class RouteCfg(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("n_outlets", C.c_int32), ("timestep", C.c_int32),
                 ("process_case", C.c_int32), ("read_states", C.c_int32),
                 ("map_flag", C.c_int32), ("n_inflow", C.c_int32), ("n_gauges", C.c_int32),
-                ("pad", C.c_int32)]
+                ("rout_loop", C.c_int32)]
 
 
 def build():
@@ -101,6 +101,8 @@ def lib():
                                    _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip, _ip, _ip, _dp,
                                    _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
             "or_routing_order": (v, [C.c_int64, C.c_int64, _ip, _ip, _ip, _ip]),
+            "or_mrm_update_param_celerity": (v, [C.c_int64, _dp, d, C.c_int32, _dp, _dp, _dp,
+                                                 _dp, _ip]),
             "or_meteo_l2_to_l1": (i, [C.c_int32, C.c_int32, d, C.c_void_p, C.c_int32,
                                       C.c_int32, d, C.c_void_p, C.c_int32, _dp, _dp]),
         }
@@ -191,3 +193,16 @@ def meteo_l2_to_l1(data2, mask2, cellsize2, mask1, cellsize1):
     rc = lib().or_meteo_l2_to_l1(nr2, nc2, float(cellsize2), m2.ctypes.data, nr1, nc1,
                                  float(cellsize1), m1.ctypes.data, nd, dptr(d2), dptr(out))
     return None if rc != 0 else out
+
+
+def mrm_update_param_celerity(length, celerity, timestep):
+    """The oracle's adaptive routing parameters: (C1, C2, ts_rout, factor, rout_loop)."""
+    length = np.ascontiguousarray(length, dtype=np.float64)
+    n = length.shape[0]
+    c1, c2 = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    ts, f = ref(), ref()
+    loops = C.c_int32()
+    lib().or_mrm_update_param_celerity(n, dptr(length), float(celerity), int(timestep),
+                                       dptr(c1), dptr(c2), C.byref(ts), C.byref(f),
+                                       C.byref(loops))
+    return c1[:n], c2[:n], ts.value, f.value, loops.value

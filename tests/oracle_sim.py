@@ -128,6 +128,18 @@ class OracleSim:
         for di in range(len(self.domains)):
             if domain < 0 or di == domain:
                 self.ds[di][set_].gamma[:] = gamma5
+                self.ds[di][set_].adaptive = None
+
+    def set_routing_celerity(self, celerity, set_=0, domain=-1):
+        """processCase(8) = 2: C1/C2 and sub-steps from a constant celerity."""
+        for di, d in enumerate(self.domains):
+            if domain < 0 or di == domain:
+                o = self.ds[di][set_]
+                c1, c2, ts, f, loops = O.mrm_update_param_celerity(
+                    d.length[:d.n_links], celerity, self.timestep)
+                o.C1[:d.n_links] = c1
+                o.C2[:d.n_links] = c2
+                o.adaptive = (ts, f, loops)
 
     # -- stepping ----------------------------------------------------------------------
     def step(self, st):
@@ -146,7 +158,9 @@ class OracleSim:
                 rc.n_nodes = d.n_cells
                 rc.n_outlets = d.n_outlets
                 rc.timestep = self.timestep
-                rc.process_case = 1
+                ad = getattr(o, "adaptive", None)
+                rc.process_case = 1 if ad is None else 2
+                rc.rout_loop = 1 if ad is None else ad[2]
                 rc.read_states = int(st["read_states"])
                 rc.map_flag = 1
                 rc.n_inflow = 0

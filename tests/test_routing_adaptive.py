@@ -1,0 +1,159 @@
+"""Adaptive-timestep routing with a constant celerity (processCase(8) = 2).
+
+mrm_update_param (src/mRM/mo_mrm_mpr.f90:268-329) sets K = length /
+celerity per link, picks the routing step TSrout from given_TS
+(mo_mrm_constants.F90:42-46) with FORCES locate, and C1/C2 with the space
+weight 0; mRM_routing then routes each step's qOUT rout_loop = max(1,
+nint(1 / tsRoutFactor)) times and averages Qmod (mo_mrm_routing.f90:
+224-284).
+
+CPU tests pin the oracle: C1/C2 against Chow's Muskingum coefficients for
+x = 0 (c0 = C2, c1 = C1 - C2, c2 = 1 - C1 with dt = TSrout), the routing
+step against the largest given_TS not above min K (equality included), the
+sub-step count from the factor, and the averaging against repeated
+one-loop calls.  GPU tests call mhmhip_set_routing_celerity and the per-step
+entry points through the C-ABI against the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GIVEN_TS = [60.0, 120.0, 180.0, 240.0, 300.0, 360.0, 600.0, 720.0, 900.0, 1200.0, 1800.0,
+            3600.0, 7200.0, 10800.0, 14400.0, 21600.0, 28800.0, 43200.0, 86400.0]
+
+
+@pytest.mark.parametrize("celerity", [0.3, 1.0, 2.5])
+def test_coefficients_are_chows_for_zero_space_weight(celerity):
+    length = np.array([1000.0, 1414.2135623730951, 2500.0, 1000.0])
+    c1, c2, ts, _, _ = O.mrm_update_param_celerity(length, celerity, 1)
+    K = length / celerity
+    dt = ts
+    chow_c0 = dt / (2 * K + dt)
+    chow_c1 = dt / (2 * K + dt)
+    chow_c2 = (2 * K - dt) / (2 * K + dt)
+    assert np.allclose(c2, chow_c0, rtol=1e-14, atol=0)
+    assert np.allclose(c1 - c2, chow_c1, rtol=1e-13, atol=1e-16)
+    assert np.allclose(1.0 - c1, chow_c2, rtol=1e-13, atol=1e-16)
+
+
+@pytest.mark.parametrize("kmin,want", [(1000.0, 900.0), (2000.0, 1800.0), (600.0, 600.0),
+                                       (599.9999, 360.0), (30.0, 60.0), (1e6, 86400.0),
+                                       (3600.0, 3600.0)])
+def test_routing_step_is_the_largest_given_step_not_above_min_k(kmin, want):
+    length = np.array([kmin * 2.0, kmin * 3.0, kmin])  # celerity 1: K = length
+    _, _, ts, _, _ = O.mrm_update_param_celerity(length, 1.0, 1)
+    assert ts == want
+    below = [t for t in GIVEN_TS if t <= kmin]
+    assert ts == (max(below) if below else GIVEN_TS[0])
+
+
+@pytest.mark.parametrize("ts_len,timestep,loops,factor", [(900.0, 1, 4, 0.25), (1800.0, 1, 2, 0.5),
+                                                          (3600.0, 1, 1, 1.0), (360.0, 1, 10, 0.1),
+                                                          (3600.0, 24, 24, 1.0 / 24)])
+def test_sub_steps_from_the_factor(ts_len, timestep, loops, factor):
+    _, _, ts, f, n = O.mrm_update_param_celerity(np.array([ts_len]), 1.0, timestep)
+    assert ts == ts_len and f == pytest.approx(factor, rel=1e-15) and n == loops
+
+
+def _chain(n_loops):
+    """Routing configuration of a 3-node chain 3 -> 2 -> 1 (outlet 1)."""
+    rc = O.RouteCfg()
+    rc.n_nodes, rc.n_outlets, rc.timestep, rc.process_case = 3, 1, 1, 2
+    rc.map_flag, rc.n_inflow, rc.n_gauges, rc.rout_loop = 1, 0, 1, n_loops
+    return rc
+
+
+def test_averaging_equals_repeated_single_loops():
+    """One call with rout_loop = 4 = the mean of four one-loop calls' Qmod on
+    the same qOUT, and the same final states."""
+    import ctypes as C
+    from_n = np.array([3, 2], np.int32)
+    to_n = np.array([2, 1], np.int32)
+    perm = np.array([1, 2], np.int32)
+    length = np.array([1000.0, 1414.2135623730951])
+    c1, c2, _, _, _ = O.mrm_update_param_celerity(length, 1.0, 1)
+    runoff = np.array([0.3, 0.7, 1.1])
+    area = np.array([1.0, 1.0, 1.0])
+    ids = np.array([1, 2, 3], np.int32)
+
+    def call(loops, qtin, qtr):
+        rc = _chain(loops)
+        qout, qmod, g = np.zeros(3), np.zeros(3), np.zeros(1)
+        O.lib().or_mrm_routing(C.byref(rc), O.dptr(np.zeros(5)), 3, O.dptr(runoff),
+                               O.dptr(area), O.iptr(ids), O.dptr(area), O.iptr(ids),
+                               O.iptr(perm), O.iptr(from_n), O.iptr(to_n), O.dptr(length),
+                               O.dptr(np.full(2, 0.01)), O.dptr(np.zeros(2)), None, None, None,
+                               None, O.iptr(np.array([1], np.int32)), O.dptr(c1.copy()),
+                               O.dptr(c2.copy()), O.dptr(qout), O.dptr(qtin), O.dptr(qtr),
+                               O.dptr(qmod), O.dptr(g))
+        return qmod, g[0]
+
+    qtin_a, qtr_a = np.zeros(6), np.zeros(6)
+    qmod_a, g_a = call(4, qtin_a, qtr_a)
+    qtin_b, qtr_b = np.zeros(6), np.zeros(6)
+    acc = np.zeros(3)
+    for _ in range(4):
+        q, _ = call(1, qtin_b, qtr_b)
+        acc = acc + q
+    assert np.array_equal(qmod_a, acc / 4.0)
+    assert g_a == qmod_a[0]
+    assert np.array_equal(qtin_a, qtin_b) and np.array_equal(qtr_a, qtr_b)
+
+
+# ---- GPU: the C-ABI against the oracle ---------------------------------------------
+
+def _pair(n_steps=24):
+    from mhm_amd import synth
+    from pairs import make_pair
+    dom = synth.make_domain(12, 10, seed=51)
+    return make_pair([dom], n_sets=2, n_steps=n_steps, seed=13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("celerity,loops", [(1.0, 4), (0.5, 2), (0.25, 1)])
+def test_per_step_adaptive_routing_matches_oracle(celerity, loops):
+    from oracle_sim import rel_close
+    from pairs import compare_cells, compare_routing
+    print(f"case: adaptive routing celerity={celerity}", flush=True)
+    model, sim, steps, recs = _pair()
+    for s in range(2):
+        model.set_routing_celerity(celerity, s)
+        sim.set_routing_celerity(celerity, s)
+    assert sim.ds[0][0].adaptive[2] == loops
+    for k in range(len(recs)):
+        model.mhm_step(steps[k])
+        g = model.mrm_step(steps[k])
+        go = sim.step(recs[k])
+        ok, worst = rel_close(g, go, 1e-10, 1e-12)
+        assert ok, (k, worst)
+    assert compare_cells(model, sim, 1e-10, 1e-12) == []
+    assert compare_routing(model, sim, 1e-10, 1e-12) == []
+    model.close()
+
+
+@pytest.mark.gpu
+def test_adaptive_routing_limits():
+    from mhm_amd import _native as N
+    print("case: adaptive routing limits", flush=True)
+    model, sim, steps, recs = _pair(n_steps=4)
+    # one sub-step (TSrout = the model step): the time loop takes it
+    model.set_routing_celerity(0.25, 0)
+    model.set_routing_celerity(0.25, 1)
+    model.run(steps[:2], mode=N.RUN_FUSED, steps_per_launch=2)
+    # sub-steps: per step only
+    model.set_routing_celerity(1.0, 0)
+    model.set_routing_celerity(1.0, 1)
+    with pytest.raises(N.MhmHipError):
+        model.run(steps[2:4], mode=N.RUN_FUSED, steps_per_launch=2)
+    # sets that disagree on the sub-steps
+    model.set_routing_celerity(0.5, 1)
+    model.mhm_step(steps[2])
+    with pytest.raises(N.MhmHipError):
+        model.mrm_step(steps[2])
+    # routing steps longer than the model step
+    model.set_routing_celerity(0.1, 0)
+    model.set_routing_celerity(0.1, 1)
+    with pytest.raises(N.MhmHipError):
+        model.mrm_step(steps[2])
+    model.close()
