@@ -230,8 +230,9 @@ def pmc_workload(args):
     model.close()
     plan, _, _ = N.plan_network(dom)
     n_fused = n_steps - 2
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "pmc_meta.json"), "w") as f:
+    meta_dir = os.path.dirname(os.path.abspath(args.pmc_meta))
+    os.makedirs(meta_dir, exist_ok=True)
+    with open(args.pmc_meta, "w") as f:
         json.dump({"cells": dom.n_cells, "steps_per_launch": args.steps_per_launch,
                    "fused_launches": -(-n_fused // args.steps_per_launch), "plain_launches": 2,
                    "route_window": args.route_window,
@@ -273,7 +274,7 @@ def _per_kernel(rows, counter):
     return out
 
 
-def summarize_pmc(fetch_dir, write_dir, out_path):
+def summarize_pmc(fetch_dir, write_dir, out_path, meta_path):
     fetch = _per_kernel(_pmc_rows(fetch_dir), "FETCH_SIZE")
     write = _per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
     if "copy" not in fetch or "copy" not in write:
@@ -291,7 +292,6 @@ def summarize_pmc(fetch_dir, write_dir, out_path):
         wv = sum(write[k]) / len(write[k]) * w_scale * 1024.0
         res[k] = {"bytes_per_launch": round(fv + wv), "fetch_bytes": round(fv),
                   "write_bytes": round(wv), "dispatches": len(fetch[k])}
-    meta_path = os.path.join(os.path.dirname(os.path.abspath(fetch_dir)), "pmc_meta.json")
     if os.path.exists(meta_path) and "route_tile" in fetch and "route_tile" in write:
         # the fused run's tile launches are the last ones: bytes per node-step
         meta = load_json(meta_path)
@@ -360,6 +360,9 @@ def main():
     ap.add_argument("--pmc-workload", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarize-flops", metavar="PMC_DIR")
+    ap.add_argument("--pmc-meta", default="pmc_meta.json",
+                    help="workload description written by --pmc-workload and read by the "
+                         "--summarize-* steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (every basin, "
@@ -367,10 +370,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.summarize_pmc:
-        summarize_pmc(*args.summarize_pmc, os.path.join(ROOT, "profiles", "traffic.json"))
+        summarize_pmc(*args.summarize_pmc, os.path.join(ROOT, "profiles", "traffic.json"),
+                      args.pmc_meta)
         return
     if args.summarize_flops:
-        summarize_flops(args.summarize_flops, os.path.join(ROOT, "gpurun_out", "pmc_meta.json"),
+        summarize_flops(args.summarize_flops, args.pmc_meta,
                         os.path.join(ROOT, "profiles", "fp64_flops.json"))
         return
     if args.pmc_workload:
