@@ -16,6 +16,7 @@ parameter-set counts arrive over time and are run with continuous batching.
 
 Latency = completion - arrival; p50/p99 are reported beside cell-steps/s.
 """
+from concurrent.futures import ThreadPoolExecutor
 import math
 import queue
 import threading
@@ -90,9 +91,16 @@ def prepare_model(req, device=0):
     m = engine.Model([dom], n_sets=req.gammas.shape[0], timestep=req.timestep, route=True,
                      n_forcing_days=n_days, device=device)
     m.bind_network()
-    for s, g in enumerate(req.gammas):
-        m.set_params(synth.cell_params(dom, g), s)
-        m.set_routing_params(PM.routing_gamma(g), s)
+    # the sets' parameter fields are independent host work (numpy releases the
+    # GIL in its array loops): build them side by side, upload them in order
+    # (calls on one context are serialised)
+    n = len(req.gammas)
+    with ThreadPoolExecutor(max_workers=min(4, n)) as pool:
+        for s0 in range(0, n, 4):  # four sets' fields held at a time
+            fields = list(pool.map(lambda g: synth.cell_params(dom, g), req.gammas[s0:s0 + 4]))
+            for i, f in enumerate(fields):
+                m.set_params(f, s0 + i)
+                m.set_routing_params(PM.routing_gamma(req.gammas[s0 + i]), s0 + i)
     m.default_states()
     m.set_domain_forcing()
     return m
