@@ -345,7 +345,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--days", type=int, default=90)
-    ap.add_argument("--steps-per-launch", type=int, default=24)
+    ap.add_argument("--steps-per-launch", type=int, default=144)
     ap.add_argument("--route-window", type=int, default=64,
                     help="routing window of the headline loop (ring memory grows with it)")
     ap.add_argument("--no-graph", action="store_true", help="headline loop without hipGraph")
