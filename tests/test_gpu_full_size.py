@@ -9,7 +9,8 @@ the 256-set ensemble (64 x 64 cells, daily x 3650) and the 64-basin pack
   suite's 1e-10 relative bar.  The oracle finishes them in a few seconds.
 * Schedule independence over the full 2160 steps (90 days), where the
   oracle would take minutes: from the same starting states, the fused loop
-  with tiled routing and the plain loop (one launch per step, every flux
+  (144 steps per launch, as the bench runs it) with tiled routing and the
+  plain loop (one launch per step, every flux
   written) with diagonal routing give the same gauge series and the same
   final states bit for bit -- the two share no kernel, so a fault in either
   schedule's indexing at full size (τmax 123 tile launches, 4008
@@ -65,7 +66,7 @@ def test_full_size_schedules_agree_bit_for_bit(basin):
                        doy0=START.timetuple().tm_yday)
     m2, _, steps, _ = make_pair([dom], n_sets=1, n_steps=n_steps, seed=5)
     snap = m2.snapshot()
-    g_fused = m2.run(steps, mode=N.RUN_FUSED, steps_per_launch=24, route_window=64,
+    g_fused = m2.run(steps, mode=N.RUN_FUSED, steps_per_launch=144, route_window=64,
                      use_graph=True)
     s_fused = {k: m2.get(k) for k in STATES}
     r_fused = m2.routing_state(0)
