@@ -224,13 +224,24 @@ STATE_NAMES = ["INTER", "SNOWPACK", "SEALSTW", "UNSATSTW", "SATSTW", "SOILMOIST"
 FLUX_NAMES = list(X_ROW) + ["INFILSOIL", "AETSOIL"]
 
 
-def rel_close(a, b, rtol=1e-10, atol=1e-12):
-    """|a - b| <= rtol * max(|a|, |b|) + atol, elementwise; returns (ok, worst)."""
+def rel_close(a, b, rtol=1e-10, atol=1e-12, allow_nan=False):
+    """|a - b| <= rtol * max(|a|, |b|) + atol, elementwise; returns (ok, worst).
+
+    A NaN on either side fails, unless allow_nan is set: then a NaN on both
+    sides at the same position passes (and a NaN on one side still fails)."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
-    err = np.abs(a - b)
-    lim = rtol * np.maximum(np.abs(a), np.abs(b)) + atol
-    both_nan = np.isnan(a) & np.isnan(b)
-    bad = (err > lim) & ~both_nan
-    worst = float(np.max(np.where(lim > 0, err / np.where(lim > 0, lim, 1), 0))) if a.size else 0.0
+    if a.shape != b.shape:
+        if a.squeeze().shape != b.squeeze().shape:  # unit axes aside, the shapes must agree
+            return False, float("inf")
+        a, b = a.squeeze(), b.squeeze()
+    with np.errstate(invalid="ignore"):
+        err = np.abs(a - b)
+        lim = rtol * np.maximum(np.abs(a), np.abs(b)) + atol
+        nan_a, nan_b = np.isnan(a), np.isnan(b)
+        bad = ~(err <= lim)  # NaN differences count as bad
+        if allow_nan:
+            bad &= ~(nan_a & nan_b)
+        ratio = np.where(lim > 0, err / np.where(lim > 0, lim, 1), 0)
+    worst = float(np.max(np.where(np.isnan(ratio), np.inf, ratio))) if a.size else 0.0
     return (not bad.any()), worst

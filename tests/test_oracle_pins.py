@@ -1,0 +1,335 @@
+"""Pins of the oracle's routing input, inflow gauges, meteo composition and
+initial states by closed forms and invariants (the reference's own tests
+do not cover these functions).
+
+* L11_runoff_acc (mo_mrm_pre_routing.f90:77-143): the unit conversion
+  mm km-2 per step -> m3 s-1 from first principles (1 mm on 1 km2 is
+  1e-3 m * 1e6 m2 = 1000 m3, released over TS * 3600 s), and the mass
+  identity of the map_flag scatter over an aggregating L1 -> L11 map.
+* add_inflow (mo_mrm_pre_routing.f90:179-214) and the inflow cut of
+  L11_routing (mo_mrm_routing.f90:452-457): a headwater gauge adds its
+  discharge to the node's runoff (the same as routing a node whose runoff
+  is larger by that discharge); a non-headwater gauge replaces everything
+  upstream of and at its node: the discharge downstream of the gauge is
+  the same whatever runs off upstream or at the node, the links into the
+  node carry nothing, and the node's Qmod is the gauge's discharge.
+* the meteo composition of one day of hourly steps inside the oracle's
+  step (get_corrected_pet / get_temp / get_prec, mo_meteo_handler.f90:
+  404-408, 1048-1285): with constant daily forcing the 24 hourly
+  precipitation and PET values add up to the daily values, the hourly
+  temperatures average to the daily mean, the day factor applies exactly
+  for hours 7..18, and each PET case corrects the daily value with its own
+  factor.
+* fluxes_states_default_init (mo_init_states.f90:258-345) with
+  mo_mpr_constants.f90:26-30 and mo_common_constants.f90:34: the initial
+  states worked out by hand for several horizon sets.
+Each check fails for a plausible mistake in the function it pins (a wrong
+conversion factor, add instead of replace, a missing cut, day and night
+swapped, a sign of the temperature correction, a wrong initial constant).
+"""
+import datetime as dt
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mhm_amd import calendar, parameters as PM, synth
+from oracle_sim import OracleSim, rel_close
+
+EPS = np.finfo(np.float64).eps
+
+
+# ---------------------------------------------------------------------------
+# L11_runoff_acc
+
+def _runoff_acc(q, area, l1_l11, n_l11, l11_area, l11_l1, ts, map_flag):
+    q = np.ascontiguousarray(q, np.float64)
+    area = np.ascontiguousarray(area, np.float64)
+    out = np.zeros(n_l11)
+    O.lib().or_L11_runoff_acc(q.shape[0], O.dptr(q), O.dptr(area),
+                              O.iptr(np.ascontiguousarray(l1_l11, np.int32)), n_l11,
+                              O.dptr(np.ascontiguousarray(l11_area, np.float64)),
+                              O.iptr(np.ascontiguousarray(l11_l1, np.int32)), ts,
+                              1 if map_flag else 0, O.dptr(out))
+    return out
+
+
+@pytest.mark.parametrize("ts", [1, 2, 3, 6, 8, 12, 24])
+def test_runoff_acc_unit_conversion(ts):
+    """1 mm of runoff on 1 km2 in one step of ts hours is 1000 m3 over ts*3600 s."""
+    mm, km2 = 1.0e-3, 1.0e6  # metres per mm, m2 per km2
+    want = 1.0 * mm * 1.0 * km2 / (ts * 3600.0)
+    one = np.ones(1, np.int32)
+    got = _runoff_acc([1.0], [1.0], one, 1, [1.0], one, ts, True)
+    assert got[0] == pytest.approx(want, rel=4 * EPS)
+    # the gathering branch (routing resolution finer than the hydrology) converts alike
+    got = _runoff_acc([1.0], [7.0], one, 1, [1.0], one, ts, False)
+    assert got[0] == pytest.approx(want, rel=4 * EPS)
+    # linear in depth and area
+    got = _runoff_acc([2.5], [4.0], one, 1, [1.0], one, ts, True)
+    assert got[0] == pytest.approx(10.0 * want, rel=4 * EPS)
+
+
+def test_runoff_acc_scatter_conserves_volume():
+    """map_flag: every L1 cell's volume lands on its own node, nothing else."""
+    rng = np.random.default_rng(5)
+    n1, n11, ts = 400, 37, 3
+    l1_l11 = rng.integers(1, n11 + 1, n1).astype(np.int32)
+    q = rng.gamma(0.7, 2.0, n1)            # mm per step
+    area = rng.uniform(0.2, 2.5, n1)       # km2
+    got = _runoff_acc(q, area, l1_l11, n11, np.zeros(n11), np.ones(n11, np.int32), ts, True)
+    vol = got * ts * 3600.0                # m3 per step per node
+    want = np.zeros(n11)
+    np.add.at(want, l1_l11 - 1, q * area * 1000.0)
+    assert np.allclose(vol, want, rtol=1e-13, atol=0.0)
+    assert vol.sum() == pytest.approx(float((q * area).sum() * 1000.0), rel=1e-13)
+    # a node without cells gets exactly zero
+    empty = np.setdiff1d(np.arange(1, n11 + 1), l1_l11)
+    assert np.all(got[empty - 1] == 0.0)
+
+
+def test_runoff_acc_gather_uses_node_area():
+    """Without map_flag every node takes the runoff depth of its L1 cell on its own area."""
+    rng = np.random.default_rng(6)
+    n1, n11, ts = 9, 36, 1
+    l11_l1 = rng.integers(1, n1 + 1, n11).astype(np.int32)
+    q = rng.uniform(0.0, 3.0, n1)
+    l11_area = rng.uniform(0.1, 0.3, n11)
+    got = _runoff_acc(q, np.full(n1, 9.9), np.ones(n1, np.int32), n11, l11_area, l11_l1, ts,
+                      False)
+    assert np.allclose(got * 3600.0, q[l11_l1 - 1] * l11_area * 1000.0, rtol=1e-14, atol=0.0)
+
+
+# ---------------------------------------------------------------------------
+# add_inflow and the inflow cut
+
+# 9 nodes: 1 -> 3, 2 -> 3, 3 -> 5, 4 -> 5, 5 -> 8, 6 -> 7, 7 -> 8, 8 -> 9 (outlet 9)
+FROM = np.array([1, 2, 3, 4, 5, 6, 7, 8], np.int32)
+TO = np.array([3, 3, 5, 5, 8, 7, 8, 9], np.int32)
+N_NODES = 9
+UPSTREAM_OF_5 = [1, 2, 3, 4]   # nodes draining into node 5
+
+
+def _net():
+    fr = np.zeros(N_NODES, np.int32)
+    to = np.zeros(N_NODES, np.int32)
+    fr[:8], to[:8] = FROM, TO
+    ro, pm = np.zeros(N_NODES, np.int32), np.zeros(N_NODES, np.int32)
+    O.lib().or_routing_order(N_NODES, 8, O.iptr(fr), O.iptr(to), O.iptr(ro), O.iptr(pm))
+    rng = np.random.default_rng(2)
+    C1 = rng.uniform(0.2, 0.8, 8)
+    C2 = rng.uniform(-0.1, 0.3, 8)
+    return fr, to, pm, C1, C2
+
+
+def _route(qout_series, gauges):
+    """Add the inflow gauges to each step's runoff, then route; returns Qmod and
+    qTR per step.  gauges: list of (node 1-based, headwater, series)."""
+    fr, to, pm, C1, C2 = _net()
+    qTIN = np.zeros(2 * N_NODES)
+    qTR = np.zeros(2 * N_NODES)
+    Qmod = np.zeros(N_NODES)
+    n_in = len(gauges)
+    idx = np.arange(1, n_in + 1, dtype=np.int32)
+    hw = np.array([1 if g[1] else 0 for g in gauges], np.int32)
+    node = np.array([g[0] for g in gauges], np.int32)
+    q_all, tr_all = [], []
+    for t, q in enumerate(qout_series):
+        qout = np.ascontiguousarray(q, np.float64).copy()
+        if n_in:
+            qin = np.array([g[2][t] for g in gauges])
+            O.lib().or_add_inflow(n_in, O.iptr(idx), O.iptr(hw), O.iptr(node), O.dptr(qin),
+                                  O.dptr(qout))
+        O.lib().or_L11_routing(N_NODES, 8, O.iptr(pm), O.iptr(fr), O.iptr(to), O.dptr(C1),
+                               O.dptr(C2), O.dptr(qout), n_in,
+                               O.iptr(hw) if n_in else None, O.iptr(node) if n_in else None,
+                               O.dptr(qTIN), O.dptr(qTR), O.dptr(Qmod))
+        q_all.append(Qmod.copy())
+        tr_all.append(qTR[:N_NODES].copy())
+    return np.array(q_all), np.array(tr_all)
+
+
+def _series(seed, steps=30):
+    return np.random.default_rng(seed).uniform(0.0, 4.0, (steps, N_NODES))
+
+
+def test_headwater_inflow_adds_to_the_node_runoff():
+    base = _series(11)
+    qin = np.random.default_rng(12).uniform(0.5, 9.0, base.shape[0])
+    with_gauge, _ = _route(base, [(1, True, qin)])
+    bumped = base.copy()
+    bumped[:, 0] = bumped[:, 0] + qin  # the same addition, done before routing
+    without, _ = _route(bumped, [])
+    assert np.array_equal(with_gauge, without)
+    # and it is an addition, not a replacement: the node's own runoff still counts
+    replaced = base.copy()
+    replaced[:, 0] = qin
+    other, _ = _route(replaced, [])
+    assert not np.allclose(with_gauge[:, 8], other[:, 8], rtol=1e-6)
+
+
+def test_non_headwater_inflow_replaces_everything_upstream():
+    base = _series(21)
+    qin = np.random.default_rng(22).uniform(0.5, 9.0, base.shape[0])
+    q1, tr1 = _route(base, [(5, False, qin)])
+    # the gauge's node carries exactly the gauge's discharge
+    assert np.array_equal(q1[:, 4], qin)
+    # the links into the node carry nothing downstream of them
+    for u in (3, 4):
+        assert np.all(tr1[:, u - 1] == 0.0)
+    # upstream of the gauge the routing goes on (the states are not frozen)
+    assert np.any(tr1[:, 0] != 0.0) and np.any(q1[:, 2] != 0.0)
+    # whatever runs off upstream of the gauge or at its node, the discharge
+    # downstream of it is the same, bit for bit
+    other = base.copy()
+    rng = np.random.default_rng(23)
+    for n in UPSTREAM_OF_5 + [5]:
+        other[:, n - 1] = rng.uniform(0.0, 50.0, base.shape[0])
+    q2, _ = _route(other, [(5, False, qin)])
+    for n in (5, 8, 9):
+        assert np.array_equal(q1[:, n - 1], q2[:, n - 1]), n
+    # the branch beside the gauge is untouched by it
+    q0, _ = _route(base, [])
+    for n in (6, 7):
+        assert np.array_equal(q1[:, n - 1], q0[:, n - 1]), n
+    # and the gauge does change the outlet (it is not ignored)
+    assert not np.allclose(q1[:, 8], q0[:, 8], rtol=1e-6)
+
+
+def test_non_headwater_inflow_equals_a_pruned_network():
+    """Routing with a non-headwater gauge at node 5 equals routing a network in
+    which node 5 is a headwater whose runoff is the gauge's discharge."""
+    base = _series(31)
+    qin = np.random.default_rng(32).uniform(0.5, 9.0, base.shape[0])
+    q1, _ = _route(base, [(5, False, qin)])
+    # pruned network: drop nodes 1..4, renumber 5..9 -> 1..5
+    fr = np.zeros(5, np.int32)
+    to = np.zeros(5, np.int32)
+    fr[:4] = [1, 2, 3, 4]         # 5 -> 8, 6 -> 7, 7 -> 8, 8 -> 9
+    to[:4] = [4, 3, 4, 5]
+    ro, pm = np.zeros(5, np.int32), np.zeros(5, np.int32)
+    O.lib().or_routing_order(5, 4, O.iptr(fr), O.iptr(to), O.iptr(ro), O.iptr(pm))
+    _, _, _, C1, C2 = _net()
+    C1p = np.ascontiguousarray(C1[4:])
+    C2p = np.ascontiguousarray(C2[4:])
+    qTIN, qTR, Qmod = np.zeros(10), np.zeros(10), np.zeros(5)
+    for t in range(base.shape[0]):
+        qout = np.ascontiguousarray(base[t, 4:]).copy()
+        qout[0] = qin[t]
+        O.lib().or_L11_routing(5, 4, O.iptr(pm), O.iptr(fr), O.iptr(to), O.dptr(C1p),
+                               O.dptr(C2p), O.dptr(qout), 0, None, None, O.dptr(qTIN),
+                               O.dptr(qTR), O.dptr(Qmod))
+        ok, worst = rel_close(q1[t, 4:], Qmod, 1e-13, 0.0)
+        assert ok, (t, worst)
+
+
+# ---------------------------------------------------------------------------
+# meteo composition
+
+def _one_day(pet_case=1, month=7, fnight=None, hourly=False):
+    dom = synth.make_domain(4, 5, seed=8)
+    synth.make_forcing(dom, 2, 9)
+    rng = np.random.default_rng(10)
+    n = dom.n_cells
+    # one constant daily value per cell on both days
+    for k in dom.forcing:
+        dom.forcing[k][1] = dom.forcing[k][0]
+    dom.forcing["PRE"][:] = rng.uniform(0.5, 20.0, n)
+    dom.forcing["PET"] = np.tile(rng.uniform(0.5, 6.0, n), (2, 1))
+    fn = fnight or {}
+    sim = OracleSim([dom], n_sets=1, timestep=1, process_pet=pet_case, n_forcing_days=2,
+                    hourly_forcing=hourly, route=False, **fn)
+    g = PM.draw(np.random.default_rng(3), 1)[0]
+    p = synth.cell_params(dom, g)
+    sim.set_params(p, 0)
+    sim.default_states()
+    sim.set_domain_forcing()
+    start = dt.datetime(1991, month, 10)
+    recs = calendar.step_records(start, 24, 1)
+    assert list(recs["hour"]) == list(range(24))
+    X = []
+    for r in recs:
+        sim.step(r)
+        X.append(sim.ds[0][0].X[:3].copy())
+    return dom, p, recs, np.array(X)  # (24, 3 [pet, temp, prec], n)
+
+
+def test_hourly_precipitation_and_pet_add_up_to_the_day():
+    dom, p, recs, X = _one_day(pet_case=0)
+    pre = dom.forcing["PRE"][0]
+    pet_day = p["FASP"] * dom.forcing["PET"][0]  # PET case 0: fAsp * input PET
+    assert np.allclose(X[:, 2].sum(0), pre, rtol=1e-14, atol=0.0)
+    assert np.allclose(X[:, 0].sum(0), pet_day, rtol=1e-14, atol=0.0)
+
+
+def test_hourly_temperature_averages_to_the_daily_mean():
+    dom, p, recs, X = _one_day()
+    tavg = dom.forcing["TAVG"][0]
+    assert np.allclose(X[:, 1].mean(0), tavg, rtol=0.0, atol=1e-13)
+    # the correction is added, with the night ratio at night and its negative by day
+    m = recs[0]["month"] - 1
+    for h in range(24):
+        f = -synth.FNIGHT_TEMP[m] if 6 < h <= 18 else synth.FNIGHT_TEMP[m]
+        assert np.allclose(X[h, 1], tavg + f, rtol=0.0, atol=1e-13), h
+
+
+def test_day_factor_applies_for_hours_7_to_18():
+    # a night ratio below one half: daytime precipitation per hour is the larger
+    fn = dict(fnight_prec=np.full(12, 0.3), fnight_pet=np.full(12, 0.2))
+    dom, p, recs, X = _one_day(pet_case=0, fnight=fn)
+    pre = dom.forcing["PRE"][0]
+    wet_hours = {h for h in range(24) if np.all(X[h, 2] > pre / 24.0)}
+    assert wet_hours == set(range(7, 19))
+    # per hour: 2 v f / 24, f = 0.7 by day and 0.3 at night
+    for h in range(24):
+        f = 0.7 if 7 <= h <= 18 else 0.3
+        assert np.allclose(X[h, 2], pre * f / 12.0, rtol=1e-15, atol=0.0), h
+
+
+@pytest.mark.parametrize("pet_case", [-1, 0, 1])
+def test_each_pet_case_corrects_the_daily_value(pet_case):
+    dom, p, recs, X = _one_day(pet_case=pet_case)
+    m = recs[0]["month"] - 1
+    if pet_case == -1:
+        want = p["PETLAICOR"][:, m] * dom.forcing["PET"][0]
+    elif pet_case == 0:
+        want = p["FASP"] * dom.forcing["PET"][0]
+    else:
+        f = dom.forcing
+        hs = np.array([O.lib().or_pet_hargreaves(p["HARSAMCOEFF"][k], 17.8, f["TAVG"][0][k],
+                                                 f["TMAX"][0][k], f["TMIN"][0][k], dom.lat[k],
+                                                 int(recs[0]["doy"]))
+                       for k in range(dom.n_cells)])
+        want = p["FASP"] * hs
+    assert np.all(want > 0.0)
+    assert np.allclose(X[:, 0].sum(0), want, rtol=1e-14, atol=0.0)
+
+
+def test_hourly_forcing_passes_through():
+    dom, p, recs, X = _one_day(pet_case=0, hourly=True)
+    for h in range(24):
+        assert np.array_equal(X[h, 2], dom.forcing["PRE"][0])
+        assert np.array_equal(X[h, 1], dom.forcing["TAVG"][0])
+
+
+# ---------------------------------------------------------------------------
+# initial states
+
+@pytest.mark.parametrize("depths,sm", [
+    ([200.0, 500.0, 1000.0], [50.0, 75.0, 250.0]),
+    ([300.0, 900.0], [75.0, 300.0]),
+    ([100.0, 250.0, 600.0, 1200.0], [25.0, 37.5, 87.5, 225.0]),
+    ([50.0, 100.0, 150.0, 200.0, 400.0, 2000.0], [12.5, 12.5, 12.5, 12.5, 50.0, 275.0]),
+])
+def test_default_states_closed_form(depths, sm):
+    n, nh = 7, len(depths)
+    hd = np.array(depths)
+    S = np.full((O.lib().or_n_state_rows(nh), n), np.nan)
+    X = np.full((O.lib().or_n_flux_rows(nh), n), np.nan)
+    O.lib().or_default_states(n, n, nh, O.dptr(hd), O.dptr(S), O.dptr(X))
+    # interception 0, snow pack 15, sealed storage 0, unsaturated 10, saturated 75 [mm]
+    for row, v in enumerate([0.0, 15.0, 0.0, 10.0, 75.0]):
+        assert np.all(S[row] == v), row
+    for h in range(nh):
+        assert np.all(S[5 + h] == sm[h]), h
+    assert np.all(X == 0.0)
