@@ -43,16 +43,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_per_unit(kernel, steps_per_launch, steps_per_day, window=16):
-    """Algorithmic HBM bytes per work unit (cell-step or node-step)."""
+def bytes_per_unit(kernel, steps_per_launch, steps_per_forcing, window=16, sets_sharing=1):
+    """Algorithmic HBM bytes per work unit (cell-step or node-step).
+    steps_per_forcing: model steps per forcing column (iMeteoTS changes every
+    24 steps for daily forcing, hourly and daily model steps alike);
+    sets_sharing: parameter sets reading one forcing column."""
     if kernel == "cell_step":
         # forcing 32 + PET params 24 + mHM params 232 + states 64 r/64 w + 23 fluxes 184 w
         return 600.0
     if kernel == "cell_fused":
-        # forcing once per forcing day (32/F), qOUT to the ring every step (8);
-        # once per launch: params 256 + states 64 + ring lag 1 read, states 64 +
-        # 23 fluxes 184 written (569/S)
-        return 32.0 / steps_per_day + 8.0 + 569.0 / steps_per_launch
+        # forcing once per forcing column, shared by the sets (32/(F n_sets)), qOUT
+        # to the ring every step (8); once per launch: params 256 + states 64 +
+        # ring lag 1 read, states 64 + 23 fluxes 184 written (569/S)
+        return 32.0 / (steps_per_forcing * sets_sharing) + 8.0 + 569.0 / steps_per_launch
     if kernel == "route_tile":
         # node-step: qOUT from the ring 8; once per window of B steps: slot metadata
         # 16, C1 + C2 16, qTIN read + write 16, qTR read + write 16
@@ -70,11 +73,11 @@ def load_json(path):
     return {}
 
 
-def kernel_table(prof, steps_per_launch, steps_per_day, window=8):
+def kernel_table(prof, steps_per_launch, steps_per_day, window=8, sets_sharing=1):
     detail = {}
     for k, (kms, kn, ku) in prof.items():
         if kn:
-            b = bytes_per_unit(k, steps_per_launch, steps_per_day, window)
+            b = bytes_per_unit(k, steps_per_launch, steps_per_day, window, sets_sharing)
             # the timed launches only (one in PROF_STRIDE of each kernel)
             detail[k] = {"ms": round(kms, 3), "timed_launches": kn, "units": ku,
                          "avg_us": round(kms * 1e3 / kn, 3),
@@ -121,9 +124,55 @@ def fp64_roofline(name, stats, flops_per_unit, valu_per_cell_step=None):
     return out
 
 
-def cpu_baseline(dom, gam, n_days, sample_steps, threads):
+def host_cores():
+    """The host cores this process may use, measured: logical CPUs in the
+    affinity mask, the physical cores among them (lscpu), the cgroup CPU
+    quota, and OMP_NUM_THREADS when the environment sets one; the oracle runs
+    one thread per usable physical core (the smallest of these limits)."""
+    import math
+    import subprocess
+    aff = sorted(os.sched_getaffinity(0))
+    info = {"nproc": len(aff)}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                info["lscpu_model"] = line.split(":", 1)[1].strip()
+            if line.startswith("Thread(s) per core:"):
+                info["threads_per_core"] = int(line.split(":", 1)[1])
+        par = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        cores = set()
+        for line in par.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(v) if v else -1 for v in line.split(",")[:3])
+            if cpu in aff:
+                cores.add((sock, core))
+        info["physical_cores"] = len(cores) if cores else len(aff)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        info["physical_cores"] = len(aff)
+    limits = {"physical_cores": info["physical_cores"]}
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_cpus"] = int(quota) / int(period)
+            limits["cgroup_cpus"] = max(1, math.floor(info["cgroup_cpus"]))
+    except (OSError, ValueError):
+        pass
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        info["OMP_NUM_THREADS"] = int(os.environ["OMP_NUM_THREADS"])
+        limits["OMP_NUM_THREADS"] = info["OMP_NUM_THREADS"]
+    info["threads"] = min(limits.values())
+    info["bound_by"] = min(limits, key=limits.get)
+    info["OMP_PROC_BIND"] = os.environ.get("OMP_PROC_BIND")
+    info["OMP_PLACES"] = os.environ.get("OMP_PLACES")
+    return info
+
+
+def cpu_baseline(dom, gam, n_days, sample_steps, threads, hosts=None, label="same"):
     """The oracle (C restatement, OpenMP over cells, serial routing) on the
-    first `sample_steps` hourly steps of the same basin."""
+    first `sample_steps` hourly steps of a basin (step 1 untimed)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_sim import OracleSim
     from mhm_amd import calendar, parameters as PM, synth, workloads
@@ -139,10 +188,13 @@ def cpu_baseline(dom, gam, n_days, sample_steps, threads):
         sim.step(r)
     dt = time.perf_counter() - t0
     n = dom.n_cells * (sample_steps - 1)
-    return {"value": n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle, OpenMP {threads} threads: hourly steps 2..{sample_steps} of the "
-                      f"same {dom.nrows}x{dom.ncols} basin, meteo + mHM + serial mRM routing, "
-                      f"{dt:.1f} s"}
+    out = {"value": n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+           "sample": f"oracle, OpenMP {threads} threads: hourly steps 2..{sample_steps} of the "
+                     f"{label} {dom.nrows}x{dom.ncols} basin, meteo + mHM + serial mRM routing, "
+                     f"{dt:.1f} s"}
+    if hosts:
+        out["host"] = hosts
+    return out
 
 
 def timed_runs(model, steps, k, **kw):
@@ -172,10 +224,58 @@ def bench_ensemble(N, workloads, autotune, traffic):
     em.close()
     units = 256 * edom.n_cells * 3650
     S = best["steps_per_launch"]
+    # daily steps on daily forcing: iMeteoTS = ceiling(tt / 24) moves every 24 steps
     return {"value": units / te, "unit": "cell-steps/s",
             "workload": "256 parameter sets x 4096 cells (64x64), daily x 3650, shared forcing",
             "settings": best, "autotune_trials": len(timings), "setup_s": round(setup, 1),
-            "kernels": kernel_table(eprof, S, 1)}
+            "kernels": kernel_table(eprof, S, 24, best.get("route_window") or 8, 256)}
+
+
+def bench_year_basin(N, workloads, autotune, threads, hosts):
+    """64 x 64 cells (4096), 3 horizons, hourly x 8760 (one year): the
+    fused time loop with autotuned settings, the plain path, and the oracle
+    over the whole year on the host (all usable cores, then one thread)."""
+    n_steps = 8760
+    t0 = time.perf_counter()
+    model, steps, dom, gam = workloads.plain_domain(64, 64, n_steps, seed=2)
+    setup = time.perf_counter() - t0
+    cands = [dict(c) for c in autotune.candidates(model, S_values=(144, 730, 2190),
+                                                  blocks=(64, 128), graphs=(True,),
+                                                  windows=(64,))]
+    best, timings = autotune.autotune(model, steps, trial_steps=2190, cands=cands, repeats=2)
+    snap = model.snapshot()
+    model.run(steps, want_gauges=True, **best)  # warm-up (graph capture)
+    model.restore(snap)
+    tf = timed_runs(model, steps, 3, want_gauges=True, **best) / 3
+    model.restore(snap)
+    model.set_profiling(True, 1)
+    model.run(steps, want_gauges=True, **best)
+    prof = model.profile()
+    model.set_profiling(False)
+    model.restore(snap)
+    pkw = dict(mode=N.RUN_PLAIN, want_gauges=True, route_mode=best["route_mode"],
+               route_window=best.get("route_window", 0))
+    model.run(steps[:48], **pkw)
+    model.restore(snap)
+    tp = timed_runs(model, steps, 1, **pkw)
+    model.close()
+    units = dom.n_cells * n_steps
+    out = {"value": units / tf, "ms_per_run": round(tf * 1e3, 3), "unit": "cell-steps/s",
+           "workload": "one basin 64x64 L1 cells (4096), 3 horizons, hourly x 8760 (one year), "
+                       f"D8 routing (max depth {dom.dmax}), fused time loop",
+           "settings": best, "autotune_trials": len(timings), "setup_s": round(setup, 1),
+           "kernels": kernel_table(prof, best["steps_per_launch"], 24,
+                                   best.get("route_window") or 8),
+           "plain": {"value": units / tp, "ms_per_run": round(tp * 1e3, 3),
+                     "unit": "cell-steps/s",
+                     "workload": "same basin and steps, one launch per step, every flux written "
+                                 "every step"}}
+    if threads:
+        out["cpu_baseline"] = cpu_baseline(dom, gam, dom.n_days, n_steps, threads, hosts,
+                                           label="year")
+        out["cpu_baseline_1thread"] = cpu_baseline(dom, gam, dom.n_days, n_steps, 1,
+                                                   label="year")
+    return out
 
 
 def bench_multi_basin(N, workloads, S):
@@ -350,6 +450,7 @@ def main():
                     help="routing window of the headline loop (ring memory grows with it)")
     ap.add_argument("--no-graph", action="store_true", help="headline loop without hipGraph")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-year", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -381,6 +482,11 @@ def main():
         pmc_workload(args)
         return
 
+    # the oracle's OpenMP threads: one per core, packed (read when its
+    # runtime starts, so set before anything loads it)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    hosts = host_cores()
     from mhm_amd import _native as N, autotune, serve, workloads
 
     traffic = load_json(os.path.join(ROOT, "profiles", "traffic.json"))
@@ -435,6 +541,11 @@ def main():
         log(f"plain: {extra['plain']['value']:.4g} cell-steps/s")
     model.close()
 
+    if not args.no_year:
+        extra["year_basin"] = bench_year_basin(N, workloads, autotune,
+                                               None if args.no_cpu else hosts["threads"], hosts)
+        log(f"year basin: {extra['year_basin']['value']:.4g} "
+            f"plain {extra['year_basin']['plain']['value']:.4g} {extra['year_basin']['settings']}")
     if not args.no_ensemble:
         extra["ensemble"] = bench_ensemble(N, workloads, autotune, traffic)
         log(f"ensemble: {extra['ensemble']['value']:.4g} {extra['ensemble']['settings']}")
@@ -447,10 +558,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
-        cpu = cpu_baseline(dom, gam, dom.n_days, args.cpu_steps, threads)
-        log(f"cpu: {cpu['value']:.4g} cell-steps/s")
-        one = cpu_baseline(dom, gam, dom.n_days, 4, 1)
+        cpu = cpu_baseline(dom, gam, dom.n_days, args.cpu_steps, hosts["threads"], hosts)
+        log(f"cpu: {cpu['value']:.4g} cell-steps/s ({json.dumps(hosts)})")
+        one = cpu_baseline(dom, gam, dom.n_days, 25, 1)  # one simulated day after step 1
         extra["cpu_baseline_1thread"] = one
         log(f"cpu 1 thread: {one['value']:.4g} cell-steps/s")
 

@@ -298,19 +298,29 @@ int mhmhip_qmod_accumulate(mhmhip_ctx *ctx, int32_t on);
 /* The sums of one set per node in reference order and the number of steps
  * added; sum / count is the average writeVariableTimestep writes. */
 int mhmhip_get_qmod_sum(mhmhip_ctx *ctx, int32_t set, double *sum, int64_t *count);
+/* Routing states of one set from a restart (mrm_read_restart_states,
+ * src/mRM/mo_mrm_restart.f90:459-555): qTIN(:,IT1), qTR(:,IT1) per node in
+ * reference order, and C1, C2 per link (as mhmhip_get_routing_state gives
+ * them); any pointer may be NULL, C1 and C2 go together.  Loaded C1/C2 stay
+ * while steps run with read_states = 1; the first routed step with
+ * read_states = 0 replaces them by reg_rout of the set's routing parameters
+ * (src/mRM/mo_mrm_routing.f90:211-218), which must have been given. */
 int mhmhip_set_routing_state(mhmhip_ctx *ctx, int32_t set, const double *qTIN,
-                             const double *qTR);
+                             const double *qTR, const double *C1, const double *C2);
 
 /* One mRM_routing() call for every domain and set after mhmhip_mhm_step:
  * src/mRM/mo_mrm_routing.f90:104-303, called at
  * src/mHM/mo_mhm_interface_run.f90:552-590.  gauge_out (host, may be NULL)
- * receives GaugeDischarge as [set][gauge]. */
+ * receives GaugeDischarge as [set][gauge].  st->read_states is read_restart:
+ * 0 evaluates reg_rout (C1/C2 from the routing parameters), 1 keeps C1/C2 as
+ * they are (set_routing_params' or a restart's). */
 int mhmhip_mrm_step(mhmhip_ctx *ctx, const mhmhip_step *st, double *gauge_out);
 
 /* ---- the time loop ------------------------------------------------------------
  * nsteps consecutive steps of do_time_step's hot part (mHM + mRM_routing,
  * src/mHM/mo_mhm_interface_run.f90:341-638) with the routing pipelined
- * across time.  gauge_out (host, may be NULL) receives [step][set][gauge]. */
+ * across time.  gauge_out (host, may be NULL) receives [step][set][gauge].
+ * read_states (read_restart) must be the same for every step of a call. */
 int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
                const mhmhip_run_opts *opts, double *gauge_out);
 

@@ -93,7 +93,7 @@ SIGNATURES = {
     "mhmhip_set_routing_celerity": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "mhmhip_set_inflow": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mhmhip_get_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]),
-    "mhmhip_set_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    "mhmhip_set_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]),
     "mhmhip_qmod_accumulate": (C.c_int, [C.c_void_p, C.c_int32]),
     "mhmhip_get_qmod_sum": (C.c_int, [C.c_void_p, C.c_int32, _dp, _lp]),
     "mhmhip_mrm_step": (C.c_int, [C.c_void_p, C.POINTER(Step), _dp]),

@@ -2,7 +2,8 @@
 
   mhm_amd/libmhmhip.so  HIP kernels + C-ABI (include/mhmhip.h), gfx950
   mhm_amd/libmhmsyn.so  host-only synthetic-domain tools (priority-flood D8)
-  mhm_amd/libmhmcheck.so  host + device evaluation of the chain's log/exp/pow (tests)
+  tests/native/libmhmcheck.so  host + device evaluation of the chain's log/exp/pow
+                               (test-only, built from tests/native/mathcheck.hip)
   mhm_amd/fortran/mhmhip_driver  Fortran host on the iso_c_binding module
 
 Run ``python -m mhm_amd.build`` (or ``__graft_entry__.build()``).
@@ -74,7 +75,8 @@ def build_hip(verbose=False, out=None, defines=(), obj_dir=None):
     return target
 
 
-CHECK_LIB = os.path.join(PKG, "libmhmcheck.so")
+TEST_NATIVE = os.path.join(ROOT, "tests", "native")
+CHECK_LIB = os.path.join(TEST_NATIVE, "libmhmcheck.so")
 
 
 def build_check():
@@ -82,7 +84,8 @@ def build_check():
     obj = os.path.join(OBJ, "mathcheck.hip.o")
     os.makedirs(OBJ, exist_ok=True)
     hipcc = _hipcc()
-    _run([hipcc] + hip_flags() + ["-c", "-x", "hip", os.path.join(CSRC, "mathcheck.hip"), "-o", obj])
+    _run([hipcc] + hip_flags() + ["-c", "-x", "hip", os.path.join(TEST_NATIVE, "mathcheck.hip"),
+                                  "-o", obj])
     tmp = CHECK_LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj])
     os.replace(tmp, CHECK_LIB)

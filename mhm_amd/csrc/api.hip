@@ -58,6 +58,13 @@ struct mhmhip_ctx {
   // (set, domain), 0 for processCase 1; the Qmod sums of the last
   // mrm_step's sub-steps and their count (> 1 when they apply)
   std::vector<double> ts_rout;
+  // processCase(8) = 1 coefficients per (set, domain): the routing1
+  // parameters last given to set_routing_params (reg_rout recomputes C1/C2
+  // from them at every routed step with read_states = 0,
+  // mo_mrm_routing.f90:211-218) and whether C1/C2 were last loaded from a
+  // restart (mo_mrm_restart.f90:548-555), which only read_states = 1 keeps
+  std::vector<double> gam5;        // [set][domain][5]
+  std::vector<char> gam_set, coef_restart;
   double *d_qsub = nullptr;
   int32_t qsub_loops = 0;
   double *stage = nullptr;   // pinned staging row (sstride doubles)
@@ -216,6 +223,14 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
       c.process_soil > 4 || c.process_pet < -1 || c.process_pet > 1 || c.process_route < 0 ||
       c.process_route > 1 || c.n_forcing_days < 1 || c.n_domains < 1) {
     g_create_error = "create: configuration out of range";
+    return MHMHIP_EINVAL;
+  }
+  // The kernels address a (cell, set) of a row by a 32-bit byte offset
+  // ((set * sstride + cell) * 8) and step between ring rows by ld * 8 bytes:
+  // every row of all sets must span less than 4 GiB.
+  if ((uint64_t)round_up(c.n_cells, 64) * (uint64_t)c.n_sets * 8u > 0xFFFFFFFFull) {
+    g_create_error = "create: n_sets x cells too large (a row of all sets must stay below 4 GiB: "
+                     "at most 536870912 (cell, set) slots, cells rounded up to 64 per set)";
     return MHMHIP_EINVAL;
   }
   for (int m = 0; m < 12; ++m)
@@ -1172,7 +1187,17 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->h_slope.assign(net->slope, net->slope + nLtot);
   x->h_fFPimp.assign(net->fFPimp, net->fFPimp + nLtot);
 
-  // free old routing buffers and allocate new ones
+  // free old routing buffers and allocate new ones; until the new network is
+  // complete the context has none (a failure below leaves routing unbound)
+  x->net_bound = false;
+  drop_graph(x);
+  // the gauge series and inflow discharge buffers are sized by the old
+  // network's gauge and inflow counts: a rebound network starts without them
+  // (set_inflow must be called again)
+  x->dfree(x->d_gauge_buf, x->gauge_buf_rows * (int64_t)x->n_sets * std::max(1, x->n_gauges));
+  x->gauge_buf_rows = 0;
+  x->dfree(x->d_inflow_q, x->inflow_q_rows * x->n_inflow);
+  x->inflow_q_rows = 0;
   x->dfree(x->d_meta, x->n_cells);
   x->dfree(x->d_area, x->n_cells);
   x->dfree(x->d_C1, x->ld);
@@ -1301,6 +1326,9 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
     if (ds[d + 1] - ds[d] == 1) x->lone_nodes.push_back(ds[d]);
   x->lone_qtin.assign(x->lone_nodes.size() * (size_t)x->n_sets, 0.0);
   x->ts_rout.assign((size_t)x->n_sets * x->n_domains, 0.0);
+  x->gam5.assign((size_t)x->n_sets * x->n_domains * 5, 0.0);
+  x->gam_set.assign((size_t)x->n_sets * x->n_domains, 0);
+  x->coef_restart.assign((size_t)x->n_sets * x->n_domains, 0);
   x->qsub_loops = 0;
   x->net_bound = true;
   x->qtr_slot = 1;
@@ -1312,44 +1340,25 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   return MHMHIP_OK;
 }
 
-extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, int32_t domain,
-                                         const double *p) {
-  if (!x) return MHMHIP_EINVAL;
-  if (!p || set < 0 || set >= x->n_sets || domain >= x->n_domains)
-    return x->fail(MHMHIP_EINVAL, "set_routing_params: bad set or domain");
-  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_params: bind a network first");
-  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_params");
-  // reg_rout (mo_mrm_mpr.f90:61-119) per domain, TS = real(timestep) [h].  The
-  // reference re-evaluates it every step with unchanged inputs (mo_mrm_routing.f90:211-218);
-  // evaluating it once gives the same C1, C2.
-  const double TS = (double)x->cfg.timestep;
+namespace {
+
+// C1/C2 of the links of the (set, domain) pairs selected by `pick`, uploaded
+// into the set's rows (other domains keep theirs).  coef(d, lo, hi, c1, c2)
+// fills the links [lo, hi) of domain d (indices into the packed link arrays)
+// and returns false to leave the domain untouched.
+template <typename Fn>
+int upload_coefs(mhmhip_ctx *x, int32_t set, Fn coef) {
   const int64_t nLtot = x->dom_link_start[x->n_domains];
   const int64_t off = (int64_t)set * x->sstride;
-  // start from the set's current C1/C2 so other domains keep theirs
   std::vector<double> n1(x->sstride, 0.0), n2(x->sstride, 0.0);
   CHECK_HIP(x, sync_copy(x, n1.data(), x->d_C1 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C1");
   CHECK_HIP(x, sync_copy(x, n2.data(), x->d_C2 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C2");
   std::vector<double> c1(std::max<int64_t>(nLtot, 1), 0.0), c2(std::max<int64_t>(nLtot, 1), 0.0);
   std::vector<char> touched(std::max<int64_t>(nLtot, 1), 0);
   for (int d = 0; d < x->n_domains; ++d) {
-    if (domain >= 0 && d != domain) continue;
-    x->ts_rout[(size_t)set * x->n_domains + d] = 0.0;  // processCase 1
     const int64_t l0 = x->dom_link_start[d], l1 = x->dom_link_start[d + 1];
+    if (!coef(d, l0, l1, c1.data(), c2.data())) continue;
     for (int64_t i = l0; i < l1; ++i) touched[i] = 1;
-    if (l1 <= l0) continue;
-    double ssMax = x->h_slope[l0];
-    for (int64_t i = l0 + 1; i < l1; ++i)
-      if (x->h_slope[i] > ssMax) ssMax = x->h_slope[i];
-    for (int64_t i = l0; i < l1; ++i) {
-      double K = p[0] + p[1] * (x->h_length[i] * 0.001) + p[2] * x->h_slope[i] + p[3] * x->h_fFPimp[i];
-      double xi = p[4] * (1.0 + x->h_slope[i] / ssMax);
-      if (xi > 0.5) xi = 0.5;
-      if (xi < 0.005) xi = 0.005;
-      if (K > 0.5 * TS / xi) K = 0.5 * TS / xi;
-      if (K < 0.5 * TS / (1.0 - xi)) K = 0.5 * TS / (1.0 - xi);
-      c1[i] = TS / (K * (1.0 - xi) + 0.5 * TS);
-      c2[i] = 1.0 - c1[i] * K / TS;
-    }
   }
   for (int64_t q = 0; q < x->n_cells; ++q) {
     const int64_t l = x->lay.link_of_node[q];
@@ -1360,6 +1369,81 @@ extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, int32_t dom
   }
   CHECK_HIP(x, sync_copy(x, x->d_C1 + off, n1.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C1");
   CHECK_HIP(x, sync_copy(x, x->d_C2 + off, n2.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C2");
+  return MHMHIP_OK;
+}
+
+// reg_rout (mo_mrm_mpr.f90:61-119) over the links [l0, l1) of one domain,
+// TS = real(timestep) [h]
+void reg_rout(const mhmhip_ctx *x, const double *p, int64_t l0, int64_t l1, double *c1,
+              double *c2) {
+  if (l1 <= l0) return;
+  const double TS = (double)x->cfg.timestep;
+  double ssMax = x->h_slope[l0];
+  for (int64_t i = l0 + 1; i < l1; ++i)
+    if (x->h_slope[i] > ssMax) ssMax = x->h_slope[i];
+  for (int64_t i = l0; i < l1; ++i) {
+    double K = p[0] + p[1] * (x->h_length[i] * 0.001) + p[2] * x->h_slope[i] + p[3] * x->h_fFPimp[i];
+    double xi = p[4] * (1.0 + x->h_slope[i] / ssMax);
+    if (xi > 0.5) xi = 0.5;
+    if (xi < 0.005) xi = 0.005;
+    if (K > 0.5 * TS / xi) K = 0.5 * TS / xi;
+    if (K < 0.5 * TS / (1.0 - xi)) K = 0.5 * TS / (1.0 - xi);
+    c1[i] = TS / (K * (1.0 - xi) + 0.5 * TS);
+    c2[i] = 1.0 - c1[i] * K / TS;
+  }
+}
+
+// A routed step with read_states = 0 evaluates reg_rout from the routing1
+// parameters (mo_mrm_routing.f90:211-218): C1/C2 loaded from a restart are
+// replaced by that.  The parameters must be known; other (set, domain)
+// pairs already hold reg_rout's (or mrm_update_param's) values.
+int refresh_reg_rout(mhmhip_ctx *x) {
+  for (int s = 0; s < x->n_sets; ++s) {
+    bool any = false;
+    for (int d = 0; d < x->n_domains; ++d) {
+      const size_t k = (size_t)s * x->n_domains + d;
+      if (!x->coef_restart[k] || x->ts_rout[k] > 0.0) continue;
+      if (x->dom_link_start[d + 1] <= x->dom_link_start[d]) continue;
+      if (!x->gam_set[k])
+        return x->fail(MHMHIP_EINVAL,
+                       "read_states = 0 recomputes C1/C2 by reg_rout from the routing "
+                       "parameters (mo_mrm_routing.f90:211-218): set them first, or route "
+                       "with read_states = 1 to keep the restart's C1/C2");
+      any = true;
+    }
+    if (!any) continue;
+    CHECK_RC(upload_coefs(x, s, [&](int d, int64_t l0, int64_t l1, double *c1, double *c2) {
+      const size_t k = (size_t)s * x->n_domains + d;
+      if (!x->coef_restart[k] || x->ts_rout[k] > 0.0) return false;
+      reg_rout(x, &x->gam5[k * 5], l0, l1, c1, c2);
+      x->coef_restart[k] = 0;
+      return true;
+    }));
+  }
+  return MHMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int mhmhip_set_routing_params(mhmhip_ctx *x, int32_t set, int32_t domain,
+                                         const double *p) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!p || set < 0 || set >= x->n_sets || domain >= x->n_domains)
+    return x->fail(MHMHIP_EINVAL, "set_routing_params: bad set or domain");
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_params: bind a network first");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_params");
+  // The reference re-evaluates reg_rout every step with unchanged inputs
+  // (mo_mrm_routing.f90:211-218); evaluating it once gives the same C1, C2.
+  CHECK_RC(upload_coefs(x, set, [&](int d, int64_t l0, int64_t l1, double *c1, double *c2) {
+    if (domain >= 0 && d != domain) return false;
+    const size_t k = (size_t)set * x->n_domains + d;
+    x->ts_rout[k] = 0.0;  // processCase 1
+    for (int i = 0; i < 5; ++i) x->gam5[k * 5 + i] = p[i];
+    x->gam_set[k] = 1;
+    x->coef_restart[k] = 0;
+    reg_rout(x, p, l0, l1, c1, c2);
+    return true;
+  }));
   return MHMHIP_OK;
 }
 
@@ -1415,18 +1499,9 @@ extern "C" int mhmhip_set_routing_celerity(mhmhip_ctx *x, int32_t set, int32_t d
     return x->fail(MHMHIP_EINVAL, "set_routing_celerity: bad set, domain or celerity");
   if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_celerity: bind a network first");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_celerity");
-  const int64_t nLtot = x->dom_link_start[x->n_domains];
-  const int64_t off = (int64_t)set * x->sstride;
-  std::vector<double> n1(x->sstride, 0.0), n2(x->sstride, 0.0);
-  CHECK_HIP(x, sync_copy(x, n1.data(), x->d_C1 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C1");
-  CHECK_HIP(x, sync_copy(x, n2.data(), x->d_C2 + off, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download C2");
-  std::vector<double> c1(std::max<int64_t>(nLtot, 1), 0.0), c2(std::max<int64_t>(nLtot, 1), 0.0);
-  std::vector<char> touched(std::max<int64_t>(nLtot, 1), 0);
   const double xi = std::fabs(0.0);  // rout_space_weight (mo_mrm_constants.F90:33)
-  for (int d = 0; d < x->n_domains; ++d) {
-    if (domain >= 0 && d != domain) continue;
-    const int64_t l0 = x->dom_link_start[d], l1 = x->dom_link_start[d + 1];
-    if (l1 <= l0) continue;
+  CHECK_RC(upload_coefs(x, set, [&](int d, int64_t l0, int64_t l1, double *c1, double *c2) {
+    if ((domain >= 0 && d != domain) || l1 <= l0) return false;
     // mrm_update_param (mo_mrm_mpr.f90:302-323): K = length / celerity [s]
     double kmin = 0.0;
     for (int64_t i = l0; i < l1; ++i) {
@@ -1439,19 +1514,12 @@ extern "C" int mhmhip_set_routing_celerity(mhmhip_ctx *x, int32_t set, int32_t d
       const double K = x->h_length[i] / celerity;
       c1[i] = ts / (K * (1.0 - xi) + 0.5 * ts);
       c2[i] = 1.0 - c1[i] * K / ts;
-      touched[i] = 1;
     }
-    x->ts_rout[(size_t)set * x->n_domains + d] = ts;
-  }
-  for (int64_t q = 0; q < x->n_cells; ++q) {
-    const int64_t l = x->lay.link_of_node[q];
-    if (l >= 0 && touched[l]) {
-      n1[q] = c1[l];
-      n2[q] = c2[l];
-    }
-  }
-  CHECK_HIP(x, sync_copy(x, x->d_C1 + off, n1.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C1");
-  CHECK_HIP(x, sync_copy(x, x->d_C2 + off, n2.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice), "upload C2");
+    const size_t k = (size_t)set * x->n_domains + d;
+    x->ts_rout[k] = ts;
+    x->coef_restart[k] = 0;
+    return true;
+  }));
   drop_graph(x);
   return MHMHIP_OK;
 }
@@ -1543,10 +1611,12 @@ extern "C" int mhmhip_get_qmod_sum(mhmhip_ctx *x, int32_t set, double *sum, int6
 }
 
 extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double *qTIN,
-                                        const double *qTR) {
+                                        const double *qTR, const double *C1, const double *C2) {
   if (!x) return MHMHIP_EINVAL;
   if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_state: no network bound");
   if (set < 0 || set >= x->n_sets) return x->fail(MHMHIP_EINVAL, "set_routing_state: bad set");
+  if ((C1 == nullptr) != (C2 == nullptr))
+    return x->fail(MHMHIP_EINVAL, "set_routing_state: give C1 and C2 together");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_state");
   const int64_t off = (int64_t)set * x->sstride;
   auto id = [](double a) { return a; };
@@ -1556,6 +1626,17 @@ extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double
       x->lone_qtin[(size_t)set * x->lone_nodes.size() + i] = qTIN[x->lone_nodes[i]];
   if (qTR) CHECK_RC(put_row(x, x->d_pub + x->qtr_slot * x->ld + off, qTR, id));
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_routing_state");
+  if (C1) {
+    // the restart's L11_C1/L11_C2 (mo_mrm_restart.f90:548-555), per link
+    CHECK_RC(upload_coefs(x, set, [&](int d, int64_t l0, int64_t l1, double *c1, double *c2) {
+      for (int64_t i = l0; i < l1; ++i) {
+        c1[i] = C1[i];
+        c2[i] = C2[i];
+      }
+      x->coef_restart[(size_t)set * x->n_domains + d] = 1;
+      return true;
+    }));
+  }
   return MHMHIP_OK;
 }
 
@@ -1565,8 +1646,6 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   if (!routing_on(x)) return x->fail(MHMHIP_ESTATE, "mrm_step: routing is off or no network bound");
   if (x->t_route != x->t_cell - 1)
     return x->fail(MHMHIP_ESTATE, "mrm_step: call mhm_step for this step first");
-  if (st->read_states)
-    return x->fail(MHMHIP_EINVAL, "mrm_step: read_states (C1/C2 from a restart) is not supported");
   if (x->n_inflow > 0 && x->inflow_q_rows < 1)
     return x->fail(MHMHIP_ESTATE, "mrm_step: inflow discharge not set");
   int32_t loops = 1;
@@ -1574,6 +1653,7 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   if (loops > 1 && x->qacc_on)
     return x->fail(MHMHIP_EINVAL, "mrm_step: Qmod output sums with routing sub-steps are not supported");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "mrm_step");
+  if (!st->read_states) CHECK_RC(refresh_reg_rout(x));
   CHECK_RC(ensure_gauge_buf(x, 1));
   const int kind = MHMHIP_K_ROUTE_TILE;
   const TileArgs g = tile_args(x, 1, 1);
@@ -1645,12 +1725,14 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   }
   if (route && x->t_route != x->t_cell)
     return x->fail(MHMHIP_ESTATE, "run: previous step not routed (call mrm_step)");
-  for (int32_t k = 0; k < nsteps; ++k)
-    if (steps[k].read_states && route)
-      return x->fail(MHMHIP_EINVAL, "run: read_states with routing is not supported");
+  // read_restart is a constant of a run in the reference: one value per call
+  for (int32_t k = 1; k < nsteps; ++k)
+    if ((steps[k].read_states != 0) != (steps[0].read_states != 0))
+      return x->fail(MHMHIP_EINVAL, "run: read_states must be the same for every step of a call");
   if (route && x->n_inflow > 0 && x->inflow_q_rows < nsteps)
     return x->fail(MHMHIP_ESTATE, "run: inflow discharge series shorter than the run");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "run");
+  if (route && !steps[0].read_states) CHECK_RC(refresh_reg_rout(x));
   std::vector<StepDev> hs(nsteps);
   for (int32_t k = 0; k < nsteps; ++k) CHECK_RC(make_step(x, steps[k], hs[k]));
   const int S = o.mode == MHMHIP_RUN_PLAIN ? 1 : (o.steps_per_launch > 0 ? o.steps_per_launch : 24);

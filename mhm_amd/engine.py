@@ -202,10 +202,13 @@ class Model:
                                                    ("qOUT", "qTIN", "qTR", "Qmod", "C1", "C2")]))
         return out
 
-    def set_routing_state(self, qTIN, qTR, set_=0):
-        a = np.ascontiguousarray(qTIN, dtype=np.float64)
-        b = np.ascontiguousarray(qTR, dtype=np.float64)
-        self._chk(N.lib().mhmhip_set_routing_state(self._ctx, set_, N.dptr(a), N.dptr(b)))
+    def set_routing_state(self, qTIN, qTR, set_=0, C1=None, C2=None):
+        """Routing states of one set as a restart holds them: qTIN, qTR per node,
+        C1, C2 per link (optional; kept while steps run with read_states = 1)."""
+        arr = lambda v: None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        a, b, c1, c2 = arr(qTIN), arr(qTR), arr(C1), arr(C2)
+        self._chk(N.lib().mhmhip_set_routing_state(self._ctx, set_, N.dptr(a), N.dptr(b),
+                                                   N.dptr(c1), N.dptr(c2)))
 
     def qmod_accumulate(self, on=True):
         """Start (sums zeroed) or stop summing every node's Qmod over the routed
@@ -255,12 +258,17 @@ class Model:
             snap["routing"] = [self.routing_state(s) for s in range(self.n_sets)]
         return snap
 
-    def restore(self, snap):
+    def restore(self, snap, coefficients=False):
+        """States back from a snapshot.  coefficients: also the routing
+        coefficients C1/C2, as a restart file gives them (kept while steps
+        run with read_states = 1, replaced by reg_rout at the first step
+        with read_states = 0)."""
         for s, cells in enumerate(snap["cells"]):
             for k, v in cells.items():
                 self.set_var(k, v, s)
         for s, rs in enumerate(snap.get("routing", [])):
-            self.set_routing_state(rs["qTIN"], rs["qTR"], s)
+            c1, c2 = (rs.get("C1"), rs.get("C2")) if coefficients else (None, None)
+            self.set_routing_state(rs["qTIN"], rs["qTR"], s, c1, c2)
 
     # -- measurement ---------------------------------------------------------------
     def set_profiling(self, on=True, stride=1):

@@ -168,10 +168,10 @@ module mo_mhmhip
       type(c_ptr), value :: ctx, qOUT, qTIN, qTR, Qmod, C1, C2
       integer(c_int32_t), value :: set
     end function
-    integer(c_int) function mhmhip_set_routing_state(ctx, set, qTIN, qTR) &
+    integer(c_int) function mhmhip_set_routing_state(ctx, set, qTIN, qTR, C1, C2) &
         bind(C, name="mhmhip_set_routing_state")
       import :: c_int, c_ptr, c_int32_t
-      type(c_ptr), value :: ctx, qTIN, qTR
+      type(c_ptr), value :: ctx, qTIN, qTR, C1, C2
       integer(c_int32_t), value :: set
     end function
     integer(c_int) function mhmhip_mrm_step(ctx, st, gauge_out) bind(C, name="mhmhip_mrm_step")

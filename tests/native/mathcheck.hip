@@ -1,6 +1,6 @@
 // mathcheck.hip -- evaluates the chain's own log / exp / pow (chain.h) on the
 // host and on the device for accuracy tests.  Not part of the model library:
-// built as libmhmcheck.so next to it.
+// built as libmhmcheck.so next to this file (mhm_amd/build.py).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

@@ -58,3 +58,22 @@ def test_routing_order_rejects_cycles_and_double_outlinks():
     fr = np.array([1, 1, 0], np.int32)
     to = np.array([2, 3, 0], np.int32)
     assert N.lib().mhmhip_routing_order(3, 2, N.iptr(fr), N.iptr(to), N.iptr(ro[:3]), N.iptr(pm[:3])) == N.EINVAL
+
+
+def test_create_rejects_rows_beyond_32bit_offsets():
+    """The kernels address (cell, set) slots by 32-bit byte offsets: a row of
+    all sets must stay below 4 GiB, and create says so before any device call."""
+    L = N.lib()
+    cfg = N.Config()
+    cfg.n_horizons, cfg.timestep, cfg.process_soil, cfg.process_pet = 3, 1, 1, 1
+    cfg.n_forcing_days, cfg.n_domains = 1, 1
+    for m in range(12):
+        cfg.evap_coeff[m] = 1.0
+    ctx = C.c_void_p()
+    cfg.n_cells, cfg.n_sets = 1 << 26, 8  # 2^29 slots x 8 B = 4 GiB
+    assert L.mhmhip_create(C.byref(cfg), C.byref(ctx)) == N.EINVAL
+    assert b"4 GiB" in L.mhmhip_last_error(None)
+    cfg.n_cells, cfg.n_sets = (1 << 26) - 64, 8  # one padded set below: passes the bound
+    cfg.evap_coeff[4] = 0.0  # and fails at the next check, before any device call
+    assert L.mhmhip_create(C.byref(cfg), C.byref(ctx)) == N.EINVAL
+    assert b"evap_coeff" in L.mhmhip_last_error(None)

@@ -19,21 +19,12 @@ ORDER = ["FSEALED", "DEGDAY_INC", "DEGDAY_MAX", "DEGDAY_NOPRE", "TEMP_THRESH", "
          "SOILMOIST_FC", "WILTINGPOINT", "SOILMOIST_EXP", "MAXINTER", "PETLAICOR"]
 
 
-def test_fortran_host_matches_oracle(tmp_path):
-    print("case: fortran driver", flush=True)
-    assert os.path.exists(build.FORTRAN_DRIVER), "build the Fortran driver first"
-    dom = synth.make_domain(13, 11, seed=71)
-    n_steps = 36
-    n_days = calendar.n_forcing_days_needed(n_steps)
-    synth.make_forcing(dom, n_days, 72)
-    gam = PM.draw(np.random.default_rng(73), 1)[0]
-    p = synth.cell_params(dom, gam)
-    recs = calendar.step_records(START, n_steps, 1)
+def _write_bundle(fin, dom, n_days, p, gam, recs, recs2=None):
     n, nh = dom.n_cells, dom.horizon_depth.shape[0]
-    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
         np.array([n], np.int64).tofile(f)
-        np.array([nh, n_days, n_steps, dom.n_links, dom.gauge_nodes.shape[0]], np.int32).tofile(f)
+        np.array([nh, n_days, len(recs), dom.n_links, dom.gauge_nodes.shape[0]],
+                 np.int32).tofile(f)
         dom.horizon_depth.astype(np.float64).tofile(f)
         for t in (synth.EVAP_COEFF, synth.FNIGHT_PREC, synth.FNIGHT_PET, synth.FNIGHT_TEMP):
             np.asarray(t, np.float64).tofile(f)
@@ -51,11 +42,16 @@ def test_fortran_host_matches_oracle(tmp_path):
             np.asfortranarray(np.asarray(v, np.float64).reshape(n, -1)).ravel(order="F").tofile(f)
         for k in ("PRE", "TAVG", "TMIN", "TMAX"):
             dom.forcing[k].astype(np.float64).tofile(f)  # (day, cell) = Fortran F(cell, day)
-        np.array([[r["tt"], r["month"], r["hour"], r["doy"], r["iMeteoTS"], r["read_states"]]
-                  for r in recs], np.int32).tofile(f)
-    r = subprocess.run([build.FORTRAN_DRIVER, str(fin), str(fout)], capture_output=True, text=True,
-                       timeout=120)
-    assert r.returncode == 0, r.stdout + r.stderr
+        rows = lambda rr: np.array([[r["tt"], r["month"], r["hour"], r["doy"], r["iMeteoTS"],
+                                     r["read_states"]] for r in rr], np.int32)
+        rows(recs).tofile(f)
+        if recs2 is not None:
+            np.array([len(recs2)], np.int32).tofile(f)
+            rows(recs2).tofile(f)
+
+
+def _read_output(fout, dom, n_steps):
+    n, nh = dom.n_cells, dom.horizon_depth.shape[0]
     out = np.fromfile(fout, np.float64)
     ng = dom.gauge_nodes.shape[0]
     gauges = out[:ng * n_steps].reshape(n_steps, ng)
@@ -63,15 +59,68 @@ def test_fortran_host_matches_oracle(tmp_path):
     sm = out[off:off + n * nh].reshape(nh, n).T
     off += n * nh
     scal = out[off:off + 6 * n].reshape(6, n)
+    return gauges, sm, scal
 
+
+def _check_against_oracle(dom, n_days, p, gam, recs, gauges, sm, scal):
     sim = OracleSim([dom], n_sets=1, n_forcing_days=n_days)
     sim.set_params(p, 0)
     sim.set_routing_params(PM.routing_gamma(gam))
     sim.default_states()
     sim.set_domain_forcing()
     go = np.stack([sim.step(rr)[0] for rr in recs])
-    assert rel_close(gauges, go, 1e-10, 1e-12)[0]
+    ok, worst = rel_close(gauges, go, 1e-10, 1e-12)
+    assert ok, worst
     assert rel_close(sm, sim.get("SOILMOIST"), 1e-10, 1e-12)[0]
     for i, name in enumerate(["INTER", "SNOWPACK", "SEALSTW", "UNSATSTW", "SATSTW",
                               "TOTAL_RUNOFF"]):
         assert rel_close(scal[i], sim.get(name), 1e-10, 1e-12)[0], name
+    return go
+
+
+def test_fortran_host_matches_oracle(tmp_path):
+    print("case: fortran driver", flush=True)
+    assert os.path.exists(build.FORTRAN_DRIVER), "build the Fortran driver first"
+    dom = synth.make_domain(13, 11, seed=71)
+    n_steps = 36
+    n_days = calendar.n_forcing_days_needed(n_steps)
+    synth.make_forcing(dom, n_days, 72)
+    gam = PM.draw(np.random.default_rng(73), 1)[0]
+    p = synth.cell_params(dom, gam)
+    recs = calendar.step_records(START, n_steps, 1)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_bundle(fin, dom, n_days, p, gam, recs)
+    r = subprocess.run([build.FORTRAN_DRIVER, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _check_against_oracle(dom, n_days, p, gam, recs, *_read_output(fout, dom, n_steps))
+
+
+def test_fortran_restart_matches_oracle(tmp_path):
+    """A run stopped after 30 hourly steps writes its restart variable set
+    (cell states, qTIN, qTR, C1, C2) to a file; a fresh context reads it and
+    continues 30 steps with read_restart (tt from 1 again, no soil-moisture
+    reset, C1/C2 from the file: the restarted context never gets the routing
+    parameters).  Gauges and final states match the oracle running straight
+    through with the same step records."""
+    print("case: fortran driver restart", flush=True)
+    assert os.path.exists(build.FORTRAN_DRIVER), "build the Fortran driver first"
+    dom = synth.make_domain(13, 11, seed=81)
+    n1, n2 = 30, 30
+    n_days = calendar.n_forcing_days_needed(n1 + n2)
+    synth.make_forcing(dom, n_days, 82)
+    gam = PM.draw(np.random.default_rng(83), 1)[0]
+    p = synth.cell_params(dom, gam)
+    recs = calendar.step_records(START, n1 + n2, 1)
+    recs1 = recs[:n1]
+    recs2 = recs[n1:].copy()
+    recs2["tt"] = np.arange(1, n2 + 1)  # the restarted run counts from 1 (read_restart)
+    recs2["read_states"] = 1
+    fin, fout, frst = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "restart.bin"
+    _write_bundle(fin, dom, n_days, p, gam, recs1, recs2)
+    r = subprocess.run([build.FORTRAN_DRIVER, str(fin), str(fout), str(frst)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    gauges, sm, scal = _read_output(fout, dom, n1 + n2)
+    _check_against_oracle(dom, n_days, p, gam, np.concatenate([recs1, recs2]), gauges, sm,
+                          scal)

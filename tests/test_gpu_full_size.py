@@ -136,3 +136,37 @@ def test_multi_basin_full_size():
     model, sim, steps, recs = make_pair(doms, n_sets=1, n_steps=720, seed=9)
     _parity_then_schedules(model, sim, steps, recs, n_check=24, S_check=12, S_full=24)
     model.close()
+
+
+def test_year_basin_whole_horizon_matches_oracle():
+    """The 64 x 64 basin (4096 cells, 3 horizons) hourly x 8760 (one year),
+    as the bench runs it (fused S = 144, tiled routing in windows of 64,
+    hipGraph; the ring wraps many times): every gauge value of the whole
+    year, the final cell states and fluxes and the routing states against
+    the oracle at the suite's 1e-10 bar -- the chain's own exp/log/pow and
+    the device acos/sin against libm over 8760 steps -- and the plain loop
+    with diagonal routing from the same states bit for bit."""
+    n_steps = 8760
+    _log(f"case: year basin 64x64, hourly x {n_steps}, whole horizon against the oracle")
+    dom = synth.make_domain(64, 64, seed=2)
+    model, sim, steps, recs = make_pair([dom], n_sets=1, n_steps=n_steps, seed=6)
+    snap = model.snapshot()
+    g = model.run(steps, mode=N.RUN_FUSED, steps_per_launch=144, route_window=64,
+                  use_graph=True)
+    go = np.stack([sim.step(r) for r in recs])
+    assert np.isfinite(g).all() and np.abs(g).max() > 0.0
+    ok, worst = rel_close(g, go, RTOL, ATOL)
+    assert ok, worst
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    s_fused = {k: model.get(k) for k in STATES}
+    r_fused = model.routing_state(0)
+    model.restore(snap)
+    g_plain = model.run(steps, mode=N.RUN_PLAIN, route_mode=1)
+    assert np.array_equal(g, g_plain)
+    for k in STATES:
+        assert np.array_equal(s_fused[k], model.get(k)), k
+    r = model.routing_state(0)
+    for k in ("qTIN", "qTR", "Qmod"):
+        assert np.array_equal(r_fused[k], r[k]), k
+    model.close()

@@ -38,3 +38,42 @@ def test_inflow_gauges_match_oracle(route_mode):
     assert ok, worst
     assert compare_routing(model, sim, 1e-10, 1e-12) == []
     model.close()
+
+
+def test_rebind_with_more_gauges_and_inflow_points():
+    """A network bound again with more gauges and inflow gauges: the gauge
+    series and inflow buffers of the first network are dropped (the inflow
+    discharge must be given again) and the new lists route as the oracle."""
+    print("case: rebind 20x24, 1 -> 3 gauges, 0 -> 2 inflow gauges", flush=True)
+    from mhm_amd import parameters as PM
+    dom = synth.make_domain(20, 24, seed=17)
+    n1, n2 = 10, 30
+    model, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=n1 + n2, seed=5)
+    g = model.run(steps[:n1], mode=1, steps_per_launch=4, route_mode=3)
+    go = np.stack([sim.step(r) for r in recs[:n1]])
+    assert rel_close(g, go, 1e-10, 1e-12)[0]
+    a, b = _inflow_nodes(dom)
+    order = np.argsort(-dom.depth, kind="stable")
+    dom.gauge_nodes = np.array([dom.gauge_nodes[0], a, int(order[5]) + 1], np.int32)
+    rng = np.random.default_rng(4)
+    q = 1.0 + rng.gamma(2.0, 1.5, size=(n2, 2))
+    inflow = {"domain": [0, 0], "node": [a, b], "headwater": [0, 1], "q": q}
+    model.bind_network(inflow=inflow)
+    with pytest.raises(RuntimeError, match="inflow"):
+        model.run(steps[n1:], mode=1, steps_per_launch=4, route_mode=3)
+    model.set_inflow(q)
+    gam = PM.draw(np.random.default_rng(5), 2).reshape(2, 1, -1)
+    for s in range(2):
+        model.set_routing_params(PM.routing_gamma(gam[s, 0]), s, 0)
+        o = sim.ds[0][s]
+        for arr in (o.qOUT, o.qTIN, o.qTR, o.Qmod):
+            arr[:] = 0.0  # binding a network resets the routing states (mo_mrm_init.f90:453-494)
+    dom._oracle_inflow = {"node": np.array([a, b]), "headwater": np.array([0, 1]), "q": q,
+                          "tt0": n1}
+    g = model.run(steps[n1:], mode=1, steps_per_launch=4, route_mode=3)
+    go = np.stack([sim.step(r) for r in recs[n1:]])
+    assert g.shape == (n2, 2, 3)
+    ok, worst = rel_close(g, go, 1e-10, 1e-12)
+    assert ok, worst
+    assert compare_routing(model, sim, 1e-10, 1e-12) == []
+    model.close()

@@ -303,3 +303,62 @@ def test_qmod_sums_match_oracle(route_mode, graph):
     got, n = model.qmod_sum(0)
     assert n == 0 and not got.any()
     model.close()
+
+
+def _with_read_states(steps, recs, lo, hi, flag):
+    """Copies of steps[lo:hi] and recs[lo:hi] with read_states = flag."""
+    from mhm_amd import _native as N
+    st = (N.Step * (hi - lo))()
+    for i in range(hi - lo):
+        C_src = steps[lo + i]
+        for f, _ in N.Step._fields_:
+            setattr(st[i], f, getattr(C_src, f))
+        st[i].read_states = flag
+    rr = recs[lo:hi].copy()
+    rr["read_states"] = flag
+    return st, rr
+
+
+def test_restart_coefficients_follow_read_states():
+    """C1/C2 loaded as a restart holds them (mrm_read_restart_states) are
+    routed with while steps run with read_states = 1 (the reference skips
+    reg_rout, mo_mrm_routing.f90:211) and replaced by reg_rout of the routing
+    parameters at the first step with read_states = 0; both the time loop and
+    the per-step path, against the oracle given the same coefficients."""
+    _log("case: restart coefficients and read_states")
+    dom = synth.make_domain(16, 18, seed=31)
+    model, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=40, seed=13)
+    kw = dict(mode=1, steps_per_launch=8, route_mode=3)
+    g = model.run(steps[:20], **kw)
+    go = np.stack([sim.step(r) for r in recs[:20]])
+    assert rel_close(g, go, RTOL, ATOL)[0]
+    nl = dom.n_links
+    rng = np.random.default_rng(2)
+    for s in range(2):
+        # coefficients of another parameter draw (still C1 + C2 <= 1)
+        c1 = rng.uniform(0.2, 0.6, nl)
+        c2 = rng.uniform(0.0, 1.0 - c1)
+        model.set_routing_state(None, None, s, c1, c2)
+        sim.ds[0][s].C1[:nl] = c1
+        sim.ds[0][s].C2[:nl] = c2
+    st, rr = _with_read_states(steps, recs, 20, 30, 1)
+    g = model.run(st, **kw)
+    go = np.stack([sim.step(r) for r in rr])
+    ok, worst = rel_close(g, go, RTOL, ATOL)
+    assert ok, worst
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    # a call may not mix read_states values
+    mixed, _ = _with_read_states(steps, recs, 30, 32, 0)
+    mixed[1].read_states = 1
+    with pytest.raises(RuntimeError, match="read_states"):
+        model.run(mixed, **kw)
+    # read_states = 0: reg_rout again, on the per-step path
+    st, rr = _with_read_states(steps, recs, 30, 40, 0)
+    for i in range(10):
+        model.mhm_step(st[i])
+        gi = model.mrm_step(st[i])
+        ok, worst = rel_close(gi, sim.step(rr[i]), RTOL, ATOL)
+        assert ok, (i, worst)
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    model.close()

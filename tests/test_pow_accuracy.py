@@ -29,7 +29,7 @@ import pytest
 mp = pytest.importorskip("mpmath")
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "mhm_amd", "libmhmcheck.so")
+LIB = os.path.join(ROOT, "tests", "native", "libmhmcheck.so")
 
 
 @pytest.fixture(scope="module")
