@@ -231,7 +231,35 @@ def bench_ensemble(N, workloads, autotune, traffic):
             "kernels": kernel_table(eprof, S, 24, best.get("route_window") or 8, 256)}
 
 
-def bench_year_basin(N, workloads, autotune, threads, hosts):
+def bench_per_step(model, steps, dom, n):
+    """The drop-in as the reference's time loop calls it: per step one
+    mhmhip_mhm_step (meteo + mHM(), every flux written) and one
+    mhmhip_mrm_step (mRM_routing) handing the gauge discharge back to the
+    host, on the headline basin; n steps timed after a warm-up step."""
+    model.mhm_step(steps[0])
+    model.mrm_step(steps[0])
+    model.sync()
+    t0 = time.perf_counter()
+    for k in range(1, n + 1):
+        model.mhm_step(steps[k])
+        model.mrm_step(steps[k])
+    dt = time.perf_counter() - t0
+    model.set_profiling(True, 1)
+    model.mhm_step(steps[n + 1])
+    model.mrm_step(steps[n + 1])
+    prof = model.profile()
+    model.set_profiling(False)
+    plan_launches = prof["route_tile"][1]
+    return {"value": dom.n_cells * n / dt, "unit": "cell-steps/s", "ms_per_step": dt / n * 1e3,
+            "workload": f"same basin, {n} hourly steps, each one mhm_step + one mrm_step "
+                        "(gauges to the host every step), as mhm_interface_run_do_time_step "
+                        "would call them",
+            "one_step": {"cell_step_us": round(prof["cell_step"][0] * 1e3, 1),
+                         "route_tile_launches": plan_launches,
+                         "route_tile_us": round(prof["route_tile"][0] * 1e3, 1)}}
+
+
+def bench_year_basin(N, workloads, autotune):
     """64 x 64 cells (4096), 3 horizons, hourly x 8760 (one year): the
     fused time loop with autotuned settings, the plain path, and the oracle
     over the whole year on the host (all usable cores, then one thread)."""
@@ -270,12 +298,7 @@ def bench_year_basin(N, workloads, autotune, threads, hosts):
                      "unit": "cell-steps/s",
                      "workload": "same basin and steps, one launch per step, every flux written "
                                  "every step"}}
-    if threads:
-        out["cpu_baseline"] = cpu_baseline(dom, gam, dom.n_days, n_steps, threads, hosts,
-                                           label="year")
-        out["cpu_baseline_1thread"] = cpu_baseline(dom, gam, dom.n_days, n_steps, 1,
-                                                   label="year")
-    return out
+    return out, dom, gam
 
 
 def bench_multi_basin(N, workloads, S):
@@ -451,6 +474,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="headline loop without hipGraph")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-year", action="store_true")
+    ap.add_argument("--no-per-step", action="store_true")
+    ap.add_argument("--per-step-steps", type=int, default=48)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -539,11 +564,16 @@ def main():
                                                              24, traffic),
                           "kernels": kernel_table(pprof, 1, 24, args.route_window)}
         log(f"plain: {extra['plain']['value']:.4g} cell-steps/s")
+    if not args.no_per_step:
+        extra["per_step"] = bench_per_step(model, steps, dom, args.per_step_steps)
+        log(f"per step: {extra['per_step']['value']:.4g} cell-steps/s, "
+            f"{extra['per_step']['ms_per_step']:.3f} ms per step")
     model.close()
 
+    year = None
     if not args.no_year:
-        extra["year_basin"] = bench_year_basin(N, workloads, autotune,
-                                               None if args.no_cpu else hosts["threads"], hosts)
+        extra["year_basin"], ydom, ygam = bench_year_basin(N, workloads, autotune)
+        year = (ydom, ygam)
         log(f"year basin: {extra['year_basin']['value']:.4g} "
             f"plain {extra['year_basin']['plain']['value']:.4g} {extra['year_basin']['settings']}")
     if not args.no_ensemble:
@@ -556,13 +586,26 @@ def main():
         extra["serving"] = bench_serving(serve, args.serve_requests, args.serve_qps)
         log(f"serving: {json.dumps(extra['serving'])}")
 
+    # The oracle last: its OpenMP runtime, started with OMP_PROC_BIND, pins
+    # the calling thread (and every thread started from it later) to one core.
     cpu = None
     if not args.no_cpu:
+        aff = os.sched_getaffinity(0)
         cpu = cpu_baseline(dom, gam, dom.n_days, args.cpu_steps, hosts["threads"], hosts)
         log(f"cpu: {cpu['value']:.4g} cell-steps/s ({json.dumps(hosts)})")
         one = cpu_baseline(dom, gam, dom.n_days, 25, 1)  # one simulated day after step 1
         extra["cpu_baseline_1thread"] = one
         log(f"cpu 1 thread: {one['value']:.4g} cell-steps/s")
+        if year is not None:
+            ydom, ygam = year
+            yb = extra["year_basin"]
+            yb["cpu_baseline"] = cpu_baseline(ydom, ygam, ydom.n_days, 8760, hosts["threads"],
+                                              hosts, label="year")
+            yb["cpu_baseline_1thread"] = cpu_baseline(ydom, ygam, ydom.n_days, 8760, 1,
+                                                      label="year")
+            log(f"cpu year basin: {yb['cpu_baseline']['value']:.4g}, 1 thread "
+                f"{yb['cpu_baseline_1thread']['value']:.4g} cell-steps/s")
+        os.sched_setaffinity(0, aff)
 
     out = {
         "metric": "cell-steps/s FP64 plain vs batched; p50/p99 request latency at fixed QPS",

@@ -31,6 +31,7 @@ extern "C" {
 #define MHMHIP_EHIP (-2)   /* HIP runtime error (no device, launch failure) */
 #define MHMHIP_ESTATE (-3) /* call out of order (e.g. step before bind)     */
 #define MHMHIP_ENOMEM (-4) /* device memory exhausted                       */
+#define MHMHIP_PENDING 1   /* mhmhip_mrm_step: runoff accumulated, not routed at this step */
 
 #define MHMHIP_MAX_HORIZONS 6
 
@@ -131,8 +132,11 @@ typedef struct {
   int32_t month;       /* 1..12                                         */
   int32_t hour;        /* 0..23                                         */
   int32_t doy;         /* 1..366                                        */
-  int32_t iMeteoTS;    /* 1-based forcing column (relative to day0 = 1) */
+  int32_t iMeteoTS;    /* 1-based forcing day (iMeteoTS of the run)     */
   int32_t read_states; /* read_restart                                  */
+  int32_t n_steps;     /* run_cfg%domainDateTime%nTimeSteps (0: unknown);
+                          ends the last routing period early when routing
+                          steps are longer than the model step          */
 } mhmhip_step;
 
 /* The L11 network of every packed domain, concatenated like the
@@ -207,6 +211,23 @@ int mhmhip_default_states(mhmhip_ctx *ctx, const double *horizon_depth);
  * (src/meteo/mo_meteo_handler.f90:1462-1605). */
 int mhmhip_set_forcing(mhmhip_ctx *ctx, int32_t var, int32_t day0, int32_t ndays,
                        const double *host, int64_t ld);
+/* Streamed forcing: days day0..day0+ndays-1 (1-based iMeteoTS, any day of the
+ * run) of the variables host[MHMHIP_F_COUNT] points at (NULL: not given; its
+ * columns keep their old values), each X(ld, ndays) of all cells, uploaded
+ * asynchronously on the context's copy stream into the device's
+ * n_forcing_days forcing columns, day d in column (d - 1) mod
+ * n_forcing_days -- the chunked read of the meteo handler
+ * (src/meteo/mo_meteo_helper.f90:283-327 chunk_config, iMeteoTS relative to
+ * the chunk at src/meteo/mo_meteo_handler.f90:609-617) as a ring of device
+ * columns.  Returns once the host arrays are copied (to pinned staging, two
+ * slots: double buffering); the upload overlaps the compute work queued on
+ * the context.  A chunk overwrites its columns only after the work queued
+ * before it; a step reading a day waits for the chunk that brought it, and a
+ * step whose day is not in its column is refused.  A chunk fits the columns
+ * (ndays <= n_forcing_days, n_forcing_days >= 2); stream the next chunk
+ * while the current one runs with n_forcing_days >= 2 chunks. */
+int mhmhip_stream_forcing(mhmhip_ctx *ctx, int32_t day0, int32_t ndays,
+                          const double *const *host, int64_t ld);
 /* Forcing columns day0..day0+ndays-1 of one variable for one domain, given
  * on its meteo grid (L2) and mapped to the domain's L1 cells on the device
  * as meteo_forcings_wrapper does after reading them
@@ -275,12 +296,22 @@ int mhmhip_set_routing_params(mhmhip_ctx *ctx, int32_t set, int32_t domain,
  * C2 = 1 - C1 K / TSrout.  mhmhip_mrm_step then routes each step's qOUT
  * rout_loop = max(1, nint(timestep / TSrout)) times and reports Qmod and the
  * gauges as the average over them (src/mRM/mo_mrm_routing.f90:224-284);
- * mhmhip_run takes the case rout_loop = 1 only.  Routing steps longer than
- * the model step (runoff accumulated over several steps,
- * src/mHM/mo_mhm_interface_run.f90:494-512) are rejected, as are sub-step
- * counts that differ between the sets or domains of one context.
- * mhmhip_set_routing_params returns a (set, domain) to processCase 1. */
+ * mhmhip_run takes rout_loop >= 1 (routing sub-steps inside the time loop);
+ * routing steps longer than the model step (runoff accumulated over several
+ * steps, src/mHM/mo_mhm_interface_run.f90:494-512) run through
+ * mhmhip_mrm_step.  Routing steps that differ between the sets or domains of
+ * one context are rejected.  mhmhip_set_routing_params returns a (set,
+ * domain) to processCase 1. */
 int mhmhip_set_routing_celerity(mhmhip_ctx *ctx, int32_t set, int32_t domain, double celerity);
+/* Adaptive-timestep routing with a celerity per link [m s-1] (processCase(8)
+ * = 3): mrm_update_param with K = L11_length / L11_celerity
+ * (src/mRM/mo_mrm_mpr.f90:306-311), the rest as mhmhip_set_routing_celerity.
+ * celerity holds the domain's links (domain >= 0) or every packed link
+ * (domain < 0) in link order: L11_celerity as L11_calc_celerity
+ * (src/mRM/mo_mrm_net_startup.f90:2212-2423) derives it from the L0 river
+ * slopes -- MPR-time work of the caller. */
+int mhmhip_set_routing_celerity_links(mhmhip_ctx *ctx, int32_t set, int32_t domain,
+                                      const double *celerity);
 /* Inflow discharge [m3 s-1] of every inflow gauge for the next mrm step(s):
  * q[step][n_inflow] for nsteps steps (InflowDischarge). */
 int mhmhip_set_inflow(mhmhip_ctx *ctx, int32_t nsteps, const double *q);
@@ -308,12 +339,35 @@ int mhmhip_get_qmod_sum(mhmhip_ctx *ctx, int32_t set, double *sum, int64_t *coun
 int mhmhip_set_routing_state(mhmhip_ctx *ctx, int32_t set, const double *qTIN,
                              const double *qTR, const double *C1, const double *C2);
 
+/* Output aggregation of the L1 states and fluxes on the device
+ * (mHM_updateDataset, src/mHM/mo_write_fluxes_states.f90:283-438, with
+ * updateVariable, src/common/mo_nc_output.f90:140-148, called every step
+ * from src/mHM/mo_mhm_interface_run.f90:702-727): while on, every step adds
+ * each selected output's value (states after the step, the step's fluxes, in
+ * the reference's expressions) to a per-cell sum, in step order.
+ * flags[k - 1] is outputFlxState(k), k = 1..21; k = 18 (neutrons) is not on
+ * this path and must be 0.  Starting zeroes the sums and the step count;
+ * flags NULL stops (the sums stay readable). */
+int mhmhip_output_accumulate(mhmhip_ctx *ctx, const int32_t *flags);
+/* The sum of output k (outputFlxState index; horizon 1..nH for k = 3, 4,
+ * 17, 19, else ignored) of one set per cell in reference order, and the
+ * number of steps added; sum / count is writeVariableTimestep's average. */
+int mhmhip_get_output_sum(mhmhip_ctx *ctx, int32_t k, int32_t horizon, int32_t set,
+                          double *sum, int64_t *count);
+
 /* One mRM_routing() call for every domain and set after mhmhip_mhm_step:
  * src/mRM/mo_mrm_routing.f90:104-303, called at
  * src/mHM/mo_mhm_interface_run.f90:552-590.  gauge_out (host, may be NULL)
  * receives GaugeDischarge as [set][gauge].  st->read_states is read_restart:
  * 0 evaluates reg_rout (C1/C2 from the routing parameters), 1 keeps C1/C2 as
- * they are (set_routing_params' or a restart's). */
+ * they are (set_routing_params' or a restart's).
+ * With an adaptive routing step longer than the model step (tsRoutFactor
+ * > 1) the step's L1_total_runoff and inflow add up (RunToRout,
+ * InflowDischarge) and routing runs every nint(tsRoutFactor) steps of tt and
+ * at tt == st->n_steps (src/mHM/mo_mhm_interface_run.f90:494-512): a step
+ * that only accumulates returns MHMHIP_PENDING and leaves gauge_out alone;
+ * a routing step returns MHMHIP_OK and its discharge, which the reference
+ * then copies to the period's earlier steps (lines 614-618, the caller's). */
 int mhmhip_mrm_step(mhmhip_ctx *ctx, const mhmhip_step *st, double *gauge_out);
 
 /* ---- the time loop ------------------------------------------------------------

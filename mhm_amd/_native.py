@@ -13,6 +13,7 @@ HIP_LIB = os.environ.get("MHMHIP_LIB", os.path.join(PKG, "libmhmhip.so"))
 SYN_LIB = os.path.join(PKG, "libmhmsyn.so")
 
 OK, EINVAL, EHIP, ESTATE, ENOMEM = 0, -1, -2, -3, -4
+PENDING = 1  # mhmhip_mrm_step: runoff accumulated, no routing at this step
 MAX_HORIZONS = 6
 
 # enum mhmhip_var
@@ -52,7 +53,7 @@ class Config(C.Structure):
 
 class Step(C.Structure):
     _fields_ = [("tt", C.c_int32), ("month", C.c_int32), ("hour", C.c_int32), ("doy", C.c_int32),
-                ("iMeteoTS", C.c_int32), ("read_states", C.c_int32)]
+                ("iMeteoTS", C.c_int32), ("read_states", C.c_int32), ("n_steps", C.c_int32)]
 
 
 class Network(C.Structure):
@@ -81,6 +82,8 @@ SIGNATURES = {
     "mhmhip_get_cell_var": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int64, C.c_int32]),
     "mhmhip_default_states": (C.c_int, [C.c_void_p, _dp]),
     "mhmhip_set_forcing": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp, C.c_int64]),
+    "mhmhip_stream_forcing": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32,
+                                        C.POINTER(_dp), C.c_int64]),
     "mhmhip_set_forcing_l2": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                         _dp, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
                                         C.c_int32, C.c_int32, C.c_double, C.c_void_p]),
@@ -91,12 +94,16 @@ SIGNATURES = {
     "mhmhip_bind_network": (C.c_int, [C.c_void_p, C.POINTER(Network)]),
     "mhmhip_set_routing_params": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
     "mhmhip_set_routing_celerity": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
+    "mhmhip_set_routing_celerity_links": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
     "mhmhip_set_inflow": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mhmhip_get_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mhmhip_set_routing_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]),
     "mhmhip_qmod_accumulate": (C.c_int, [C.c_void_p, C.c_int32]),
     "mhmhip_get_qmod_sum": (C.c_int, [C.c_void_p, C.c_int32, _dp, _lp]),
     "mhmhip_mrm_step": (C.c_int, [C.c_void_p, C.POINTER(Step), _dp]),
+    "mhmhip_output_accumulate": (C.c_int, [C.c_void_p, _ip]),
+    "mhmhip_get_output_sum": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp,
+                                        C.POINTER(C.c_int64)]),
     "mhmhip_run": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts), _dp]),
     "mhmhip_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "mhmhip_get_profile": (C.c_int, [C.c_void_p, C.c_int32, _dp, _lp, _dp]),

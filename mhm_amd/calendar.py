@@ -30,15 +30,17 @@ def step_table(start, n_steps, timestep, n_tstep_forcing_day=1, read_states=Fals
         s.doy = t.timetuple().tm_yday
         s.iMeteoTS = int(math.ceil(float(tt) / meteo_div))
         s.read_states = 1 if read_states else 0
+        s.n_steps = tt0 + n_steps - 1  # the table's last step ends the run
     return arr
 
 
 def step_records(start, n_steps, timestep, n_tstep_forcing_day=1, read_states=False, tt0=1):
     """The same table as a numpy structured array (for tests and the oracle)."""
     tab = step_table(start, n_steps, timestep, n_tstep_forcing_day, read_states, tt0)
-    return np.array([(s.tt, s.month, s.hour, s.doy, s.iMeteoTS, s.read_states) for s in tab],
+    return np.array([(s.tt, s.month, s.hour, s.doy, s.iMeteoTS, s.read_states, s.n_steps)
+                     for s in tab],
                     dtype=[("tt", "i4"), ("month", "i4"), ("hour", "i4"), ("doy", "i4"),
-                           ("iMeteoTS", "i4"), ("read_states", "i4")])
+                           ("iMeteoTS", "i4"), ("read_states", "i4"), ("n_steps", "i4")])
 
 
 def n_forcing_days_needed(n_steps, n_tstep_forcing_day=1, tt0=1):

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -63,11 +64,42 @@ struct mhmhip_ctx {
   // from them at every routed step with read_states = 0,
   // mo_mrm_routing.f90:211-218) and whether C1/C2 were last loaded from a
   // restart (mo_mrm_restart.f90:548-555), which only read_states = 1 keeps
+  // routing steps longer than the model step: RunToRout [mm] per (set, cell)
+  // on the device, InflowDischarge summed on the host, the inflow row of
+  // the next mrm_step kept on the host, and the averaged inflow of a
+  // routing step on the device
+  double *d_runacc = nullptr;
+  std::vector<double> inflow_acc, h_inflow_row;
+  double *d_inflow_avg = nullptr;
   std::vector<double> gam5;        // [set][domain][5]
   std::vector<char> gam_set, coef_restart;
   double *d_qsub = nullptr;
   int32_t qsub_loops = 0;
   double *stage = nullptr;   // pinned staging row (sstride doubles)
+
+  // Forcing columns.  Day d (1-based, iMeteoTS) lives in column
+  // (d - 1) mod n_forcing_days; col_day[c] is the day column c holds (c + 1
+  // until a streamed chunk replaces it) and col_seq[c] the streamed chunk
+  // that wrote it (-1: written synchronously).  Streamed chunks go up on the
+  // copy stream (cstream) in the order they are given; chunk_ev[k] marks the
+  // end of chunk k's upload, and a launch reading a column first waits for
+  // the chunk that wrote it (waited_seq: the newest chunk the compute stream
+  // has waited for).  A chunk overwrites its columns only after the compute
+  // work queued before it (which may still read them) is done.
+  std::vector<int32_t> col_day;
+  std::vector<int64_t> col_seq;
+  std::vector<hipEvent_t> chunk_ev;
+  int64_t waited_seq = -1;
+  hipStream_t cstream = nullptr;
+  // double-buffered staging of streamed chunks: pinned host copies of the
+  // caller's rows and their device copies before the gather into routing
+  // order; fstage_ev[i] marks the end of the upload out of slot i
+  double *fstage_h[2] = {nullptr, nullptr}, *fstage_d[2] = {nullptr, nullptr};
+  int64_t fstage_cap = 0;  // doubles per slot
+  hipEvent_t fstage_ev[2] = {nullptr, nullptr};
+  int fstage_next = 0;
+  int32_t *d_pos = nullptr;       // device index of every reference cell (gather)
+  bool d_pos_stale = true;
 
   // routing
   bool net_bound = false;
@@ -92,6 +124,13 @@ struct mhmhip_ctx {
   int32_t *d_node_gauge = nullptr;  // [n_cells] first gauge at the node, -1: none
   uint8_t *d_lag = nullptr;         // [n_cells] ring lag of each node (its tile slot's lag)
   int32_t *d_slot_out = nullptr;    // [n_slots] LDS output rows of the tiled kernel
+  // L1 output sums (mhmhip_output_accumulate): rows in the reference's
+  // output order, (kind, horizon) each
+  double *d_out = nullptr;
+  int64_t out_rows_cap = 0;
+  OutAcc out_cfg{};
+  bool out_on = false;
+  int64_t out_count = 0;
   double *d_qacc = nullptr;         // [ld] Qmod summed over routed steps (output aggregation)
   bool qacc_on = false;
   int64_t qacc_count = 0;
@@ -119,14 +158,16 @@ struct mhmhip_ctx {
 
   // time loop captured as a hipGraph, replayed while its key matches
   struct GraphKey {
-    int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1, prof = -1, B = -1;
+    int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1, prof = -1, B = -1, out_n = -1,
+            rep = -1;
     const void *ring = nullptr, *gauge = nullptr, *steps = nullptr, *inflow = nullptr,
-               *qacc = nullptr;
+               *qacc = nullptr, *out = nullptr;
     int64_t ring_len = 0;
     bool operator==(const GraphKey &o) const {
       return nsteps == o.nsteps && mode == o.mode && S == o.S && block == o.block && wg == o.wg &&
              prof == o.prof && B == o.B && ring == o.ring && gauge == o.gauge && steps == o.steps &&
-             inflow == o.inflow && qacc == o.qacc && ring_len == o.ring_len;
+             inflow == o.inflow && qacc == o.qacc && ring_len == o.ring_len && out == o.out &&
+             out_n == o.out_n && rep == o.rep;
     }
   };
   GraphKey gkey;
@@ -296,6 +337,9 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
   }
   x->pos.resize(c.n_cells);
   for (int64_t i = 0; i < c.n_cells; ++i) x->pos[i] = i;
+  x->col_day.resize(c.n_forcing_days);
+  for (int32_t k = 0; k < c.n_forcing_days; ++k) x->col_day[k] = k + 1;
+  x->col_seq.assign(c.n_forcing_days, -1);
   int rc = MHMHIP_OK;
   if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
     g_create_error = "create: stream creation failed";
@@ -355,7 +399,10 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_lag);
   (void)hipFree(x->d_slot_out);
   (void)hipFree(x->d_qacc);
+  (void)hipFree(x->d_out);
   (void)hipFree(x->d_qsub);
+  (void)hipFree(x->d_runacc);
+  (void)hipFree(x->d_inflow_avg);
   (void)hipFree(x->d_inflow_node);
   (void)hipFree(x->d_inflow_hw);
   (void)hipFree(x->d_inflow_q);
@@ -363,6 +410,15 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_gauge_buf);
   (void)hipFree(x->d_steps);
   if (x->stage) (void)hipHostFree(x->stage);
+  if (x->cstream) (void)hipStreamSynchronize(x->cstream);
+  for (int i = 0; i < 2; ++i) {
+    if (x->fstage_h[i]) (void)hipHostFree(x->fstage_h[i]);
+    (void)hipFree(x->fstage_d[i]);
+    if (x->fstage_ev[i]) (void)hipEventDestroy(x->fstage_ev[i]);
+  }
+  (void)hipFree(x->d_pos);
+  for (hipEvent_t e : x->chunk_ev) (void)hipEventDestroy(e);
+  if (x->cstream) (void)hipStreamDestroy(x->cstream);
   for (auto &p : x->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -564,6 +620,126 @@ extern "C" int mhmhip_default_states(mhmhip_ctx *x, const double *hd) {
   return MHMHIP_OK;
 }
 
+namespace {
+
+// Columns day0 - 1 .. day0 - 1 + ndays - 1 written synchronously (resident
+// forcing): streamed uploads still in flight finish first, and the columns
+// hold those days from now on.
+int forcing_columns_direct(mhmhip_ctx *x, int32_t day0, int32_t ndays) {
+  if (x->cstream) CHECK_HIP(x, hipStreamSynchronize(x->cstream), "forcing");
+  for (int32_t j = 0; j < ndays; ++j) {
+    x->col_day[day0 - 1 + j] = day0 + j;
+    x->col_seq[day0 - 1 + j] = -1;
+  }
+  return MHMHIP_OK;
+}
+
+// The compute stream waits for the streamed chunks that wrote the columns
+// the steps read (chunks complete in order on the copy stream: waiting for
+// the newest is enough).
+int forcing_wait(mhmhip_ctx *x, const StepDev *st, int64_t n) {
+  int64_t newest = -1;
+  for (int64_t k = 0; k < n; ++k) newest = std::max(newest, x->col_seq[st[k].day]);
+  if (newest > x->waited_seq) {
+    CHECK_HIP(x, hipStreamWaitEvent(x->stream, x->chunk_ev[newest], 0), "forcing wait");
+    x->waited_seq = newest;
+  }
+  return MHMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int mhmhip_stream_forcing(mhmhip_ctx *x, int32_t day0, int32_t ndays,
+                                     const double *const *host, int64_t ld) {
+  if (!x) return MHMHIP_EINVAL;
+  const int32_t nfd = x->cfg.n_forcing_days;
+  if (!host || day0 < 1 || ndays < 1 || ld < x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "stream_forcing: bad day range, pointers or leading dimension");
+  if (nfd < 2 || ndays > nfd || ndays > 65535)
+    return x->fail(MHMHIP_EINVAL, "stream_forcing: a chunk must fit the " + std::to_string(nfd) +
+                                      " forcing columns (at least 2; two chunks for double "
+                                      "buffering)");
+  int32_t mask = 0, nv = 0;
+  for (int v = 0; v < MHMHIP_F_COUNT; ++v)
+    if (host[v]) {
+      mask |= 1 << v;
+      ++nv;
+    }
+  if (!mask) return x->fail(MHMHIP_EINVAL, "stream_forcing: no variable given");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "stream_forcing");
+  if (!x->cstream) {
+    CHECK_HIP(x, hipStreamCreateWithFlags(&x->cstream, hipStreamNonBlocking), "copy stream");
+    for (int i = 0; i < 2; ++i)
+      CHECK_HIP(x, hipEventCreateWithFlags(&x->fstage_ev[i], hipEventDisableTiming),
+                "stream_forcing event");
+  }
+  if (x->d_pos_stale) {
+    CHECK_HIP(x, hipStreamSynchronize(x->cstream), "stream_forcing");
+    x->dfree(x->d_pos, x->n_cells);
+    CHECK_RC(x->dalloc(&x->d_pos, x->n_cells, "alloc cell positions"));
+    std::vector<int32_t> p32(x->n_cells);
+    for (int64_t c = 0; c < x->n_cells; ++c) p32[c] = (int32_t)x->pos[c];
+    CHECK_HIP(x, sync_copy(x, x->d_pos, p32.data(), sizeof(int32_t) * x->n_cells,
+                           hipMemcpyHostToDevice), "upload cell positions");
+    x->d_pos_stale = false;
+  }
+  const int64_t need = (int64_t)nv * ndays * x->n_cells;
+  const int i = x->fstage_next;
+  x->fstage_next ^= 1;
+  if (x->fstage_cap < need) {
+    // both slots grow together, once nothing is in flight from them
+    CHECK_HIP(x, hipStreamSynchronize(x->cstream), "stream_forcing");
+    for (int k = 0; k < 2; ++k) {
+      if (x->fstage_h[k]) (void)hipHostFree(x->fstage_h[k]);
+      x->fstage_h[k] = nullptr;
+      x->dfree(x->fstage_d[k], x->fstage_cap);
+    }
+    x->fstage_cap = 0;
+    for (int k = 0; k < 2; ++k) {
+      if (hipHostMalloc((void **)&x->fstage_h[k], sizeof(double) * need, hipHostMallocDefault) !=
+          hipSuccess)
+        return x->fail(MHMHIP_ENOMEM, "stream_forcing: pinned staging allocation failed");
+      CHECK_RC(x->dalloc(&x->fstage_d[k], need, "alloc forcing staging"));
+    }
+    x->fstage_cap = need;
+  }
+  // the slot's previous upload has left its pinned buffer
+  CHECK_HIP(x, hipEventSynchronize(x->fstage_ev[i]), "stream_forcing");
+  double *h = x->fstage_h[i];
+  int32_t k = 0;
+  for (int v = 0; v < MHMHIP_F_COUNT; ++v) {
+    if (!host[v]) continue;
+    for (int32_t j = 0; j < ndays; ++j)
+      std::memcpy(h + ((int64_t)k * ndays + j) * x->n_cells, host[v] + (int64_t)j * ld,
+                  sizeof(double) * x->n_cells);
+    ++k;
+  }
+  // the columns are overwritten after the compute work queued so far, which
+  // may still read them
+  hipEvent_t after = nullptr;
+  CHECK_HIP(x, hipEventCreateWithFlags(&after, hipEventDisableTiming), "stream_forcing event");
+  hipError_t e = hipEventRecord(after, x->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(x->cstream, after, 0);
+  (void)hipEventDestroy(after);
+  CHECK_HIP(x, e, "stream_forcing order");
+  CHECK_HIP(x, hipMemcpyAsync(x->fstage_d[i], h, sizeof(double) * need, hipMemcpyHostToDevice,
+                              x->cstream), "stream_forcing upload");
+  CHECK_HIP(x, hipEventRecord(x->fstage_ev[i], x->cstream), "stream_forcing event");
+  CHECK_HIP(x, launch_forcing_gather(x->fstage_d[i], mask, ndays, x->n_cells, x->d_pos, x->F, nfd,
+                                     day0 - 1, x->sstride, x->cstream), "stream_forcing gather");
+  hipEvent_t done = nullptr;
+  CHECK_HIP(x, hipEventCreateWithFlags(&done, hipEventDisableTiming), "stream_forcing event");
+  x->chunk_ev.push_back(done);
+  CHECK_HIP(x, hipEventRecord(done, x->cstream), "stream_forcing event");
+  const int64_t seq = (int64_t)x->chunk_ev.size() - 1;
+  for (int32_t j = 0; j < ndays; ++j) {
+    const int32_t col = (day0 - 1 + j) % nfd;
+    x->col_day[col] = day0 + j;
+    x->col_seq[col] = seq;
+  }
+  return MHMHIP_OK;
+}
+
 extern "C" int mhmhip_set_forcing(mhmhip_ctx *x, int32_t var, int32_t day0, int32_t ndays,
                                   const double *host, int64_t ld) {
   if (!x) return MHMHIP_EINVAL;
@@ -571,6 +747,7 @@ extern "C" int mhmhip_set_forcing(mhmhip_ctx *x, int32_t var, int32_t day0, int3
       (int64_t)day0 - 1 + ndays > x->cfg.n_forcing_days || ld < x->n_cells)
     return x->fail(MHMHIP_EINVAL, "set_forcing: variable or day range out of bounds");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_forcing");
+  CHECK_RC(forcing_columns_direct(x, day0, ndays));
   auto id = [](double a) { return a; };
   for (int32_t j = 0; j < ndays; ++j) {
     double *dst = x->F + ((int64_t)var * x->cfg.n_forcing_days + (day0 - 1 + j)) * x->sstride;
@@ -689,6 +866,7 @@ extern "C" int mhmhip_set_forcing_l2(mhmhip_ctx *x, int32_t var, int32_t domain,
       }
   }
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_forcing_l2");
+  CHECK_RC(forcing_columns_direct(x, day0, ndays));
   CallBufs b;
   CHECK_HIP(x, hipMalloc((void **)&b.data2, sizeof(double) * n2 * ndays), "set_forcing_l2 alloc");
   CHECK_HIP(x, hipMalloc((void **)&b.mask2, (size_t)n2), "set_forcing_l2 alloc");
@@ -714,10 +892,15 @@ namespace {
 
 int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
   if (s.tt < 1 || s.month < 1 || s.month > 12 || s.hour < 0 || s.hour > 23 || s.doy < 1 ||
-      s.doy > 366 || s.iMeteoTS < 1 || s.iMeteoTS > x->cfg.n_forcing_days)
-    return x->fail(MHMHIP_EINVAL, "step: tt/month/hour/doy/iMeteoTS out of range (iMeteoTS " +
-                                      std::to_string(s.iMeteoTS) + ", forcing days " +
-                                      std::to_string(x->cfg.n_forcing_days) + ")");
+      s.doy > 366 || s.iMeteoTS < 1)
+    return x->fail(MHMHIP_EINVAL, "step: tt/month/hour/doy/iMeteoTS out of range");
+  const int32_t col = (s.iMeteoTS - 1) % x->cfg.n_forcing_days;
+  if (x->col_day[col] != s.iMeteoTS)
+    return x->fail(MHMHIP_EINVAL, "step: forcing day " + std::to_string(s.iMeteoTS) +
+                                      " (iMeteoTS) is not on the device: its column holds day " +
+                                      std::to_string(x->col_day[col]) + " (" +
+                                      std::to_string(x->cfg.n_forcing_days) +
+                                      " columns; stream the day first)");
   // mo_pet.f90:339-340, evaluated with the same libm as the reference
   const double doy = (double)s.doy;
   d.dr = 1.0 + 0.0330 * std::cos(kTwoPiD * doy / kYearDays);
@@ -725,7 +908,7 @@ int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
   d.tand = std::tan(delta);
   d.sind = std::sin(delta);
   d.cosd = std::cos(delta);
-  d.day = s.iMeteoTS - 1;
+  d.day = col;
   d.month0 = s.month - 1;
   d.isday = (s.hour > 6) && (s.hour <= 18);  // mo_meteo_handler.f90:1048
   const int m = s.month - 1;
@@ -736,7 +919,7 @@ int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
   d.revap = 1.0 / d.evap;
   d.init_sm = (s.tt == 1) && !s.read_states;  // mo_mhm.f90:448
   d.doy = s.doy;
-  d.pad = 0;
+  d.meteo_same = 0;
   return MHMHIP_OK;
 }
 
@@ -761,6 +944,9 @@ CellArgs cell_args(mhmhip_ctx *x, bool ring) {
   a.hourly = x->cfg.hourly_forcing;
   a.nT = x->nT;
   a.rnT = 1.0 / x->nT;
+  a.ring_rep = 1;
+  if (x->out_on) a.out = x->out_cfg;
+  else a.out.sum = nullptr;
   return a;
 }
 
@@ -824,6 +1010,8 @@ RouteArgs route_args(mhmhip_ctx *x, int64_t t_lo, int64_t t_hi, int64_t t_base) 
   r.n_inflow = x->n_inflow;
   r.dmax = x->lay.dmax;
   r.tst = kHourSecs * (double)x->cfg.timestep;  // mo_mrm_pre_routing.f90:110
+  r.rep = 1;
+  r.acc_from = 0;
   return r;
 }
 
@@ -1070,6 +1258,7 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "mhm_step");
   StepDev d;
   CHECK_RC(make_step(x, *st, d));
+  CHECK_RC(forcing_wait(x, &d, 1));
   const bool route = routing_on(x);
   if (route) {
     if (x->t_route != x->t_cell)
@@ -1085,6 +1274,7 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
             }),
             "cell_step");
   x->t_cell++;
+  if (x->out_on) x->out_count += 1;
   if (!route) x->t_route = x->t_cell;
   return MHMHIP_OK;
 }
@@ -1152,6 +1342,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   }
 
   // re-lay any cell data already on the device into routing order
+  if (x->cstream) CHECK_HIP(x, hipStreamSynchronize(x->cstream), "bind_network");
   std::vector<int64_t> old_pos = x->pos;
   bool moved = false;
   for (int64_t c = 0; c < x->n_cells && !moved; ++c) moved = old_pos[c] != lay.pos[c];
@@ -1176,6 +1367,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
     CHECK_RC(relayout(x->F, (int64_t)MHMHIP_F_COUNT * x->cfg.n_forcing_days, x->sstride, 1));
   }
   x->pos = lay.pos;
+  x->d_pos_stale = true;
 
   // per-link host copies for reg_rout, packed by domain
   x->h_nout.assign(net->n_outlets, net->n_outlets + x->n_domains);
@@ -1218,6 +1410,9 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->dfree(x->d_lag, x->n_cells);
   x->dfree(x->d_qacc, x->ld);  // a new network restarts the output aggregation
   x->dfree(x->d_qsub, x->ld);
+  x->dfree(x->d_runacc, x->ld);
+  x->dfree(x->d_inflow_avg, std::max(1, x->n_inflow));
+  x->inflow_acc.clear();
   x->qacc_on = false;
   x->qacc_count = 0;
   x->dfree(x->d_inflow_node, std::max(1, x->n_inflow));
@@ -1467,51 +1662,69 @@ int locate_ts(double y) {
 }
 
 // Routing sub-steps per mrm_step: every (set, domain) with links must agree.
-int rout_loops(mhmhip_ctx *x, int32_t *loops) {
+// The routing cadence of the context (every (set, domain) with links must
+// agree): tsRoutFactor = TSrout / (timestep HourSecs)
+// (mo_mhm_interface_run.f90:482); below 1 each step routes
+// rout_loop = max(1, nint(1 / tsRoutFactor)) sub-steps
+// (mo_mrm_routing.f90:224); from 1 up, runoff is accumulated and routed every
+// nint(tsRoutFactor) steps (mo_mhm_interface_run.f90:494-512).  *loops: the
+// sub-steps (1 when accumulating); *factor: tsRoutFactor (1 for
+// processCase 1).
+int rout_cadence(mhmhip_ctx *x, int32_t *loops, double *factor) {
   int32_t common = 0;
+  double cf = -1.0;
   for (int s = 0; s < x->n_sets; ++s)
     for (int d = 0; d < x->n_domains; ++d) {
       if (x->dom_link_start[d + 1] <= x->dom_link_start[d]) continue;
       const double ts = x->ts_rout[(size_t)s * x->n_domains + d];
       int32_t n = 1;
+      double f = 1.0;
       if (ts > 0.0) {
-        // tsRoutFactor (mo_mhm_interface_run.f90:482), rout_loop (mo_mrm_routing.f90:224)
-        const double f = ts / ((double)x->cfg.timestep * kHourSecs);
-        if (f > 1.0)
-          return x->fail(MHMHIP_EINVAL, "routing step longer than the model step (runoff "
-                                        "accumulated over several steps) is not supported");
-        n = std::max<int32_t>(1, (int32_t)std::floor(1.0 / f + 0.5));
+        f = ts / ((double)x->cfg.timestep * kHourSecs);
+        if (f < 1.0) n = std::max<int32_t>(1, (int32_t)std::floor(1.0 / f + 0.5));
       }
-      if (common != 0 && n != common)
-        return x->fail(MHMHIP_EINVAL, "routing sub-steps differ between parameter sets or "
+      if ((common != 0 && n != common) || (cf >= 0.0 && f != cf))
+        return x->fail(MHMHIP_EINVAL, "routing steps differ between parameter sets or "
                                       "domains of one context");
       common = n;
+      cf = f;
     }
   *loops = common > 0 ? common : 1;
+  *factor = cf >= 0.0 ? cf : 1.0;
+  return MHMHIP_OK;
+}
+
+int rout_loops(mhmhip_ctx *x, int32_t *loops) {
+  double f = 1.0;
+  CHECK_RC(rout_cadence(x, loops, &f));
+  if (f > 1.0)
+    return x->fail(MHMHIP_EINVAL, "routing step longer than the model step (runoff accumulated "
+                                  "over several steps): route step by step with mhmhip_mrm_step");
   return MHMHIP_OK;
 }
 }  // namespace
 
-extern "C" int mhmhip_set_routing_celerity(mhmhip_ctx *x, int32_t set, int32_t domain,
-                                           double celerity) {
-  if (!x) return MHMHIP_EINVAL;
-  if (set < 0 || set >= x->n_sets || domain >= x->n_domains || !(celerity > 0.0))
-    return x->fail(MHMHIP_EINVAL, "set_routing_celerity: bad set, domain or celerity");
-  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_celerity: bind a network first");
-  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_celerity");
+namespace {
+
+// mrm_update_param (mo_mrm_mpr.f90:241-329) for the links of the selected
+// domains of one set: K = length / celerity [s] (302-311; cel(i) the
+// celerity of packed link i), the routing step TSrout = given_TS(locate(
+// given_TS, min K)) (312-317), C1 = TSrout / (K (1 - xi) + TSrout / 2),
+// C2 = 1 - C1 K / TSrout with xi = |rout_space_weight| = 0 (318-323).
+template <typename Cel>
+int update_param(mhmhip_ctx *x, int32_t set, int32_t domain, Cel cel) {
   const double xi = std::fabs(0.0);  // rout_space_weight (mo_mrm_constants.F90:33)
   CHECK_RC(upload_coefs(x, set, [&](int d, int64_t l0, int64_t l1, double *c1, double *c2) {
     if ((domain >= 0 && d != domain) || l1 <= l0) return false;
-    // mrm_update_param (mo_mrm_mpr.f90:302-323): K = length / celerity [s]
     double kmin = 0.0;
     for (int64_t i = l0; i < l1; ++i) {
-      const double K = x->h_length[i] / celerity;
+      const double K = x->h_length[i] / cel(i);
       if (i == l0 || K < kmin) kmin = K;
     }
     const int ind = std::max(1, locate_ts(kmin));
     const double ts = kGivenTS[ind - 1];
     for (int64_t i = l0; i < l1; ++i) {
-      const double K = x->h_length[i] / celerity;
+      const double K = x->h_length[i] / cel(i);
       c1[i] = ts / (K * (1.0 - xi) + 0.5 * ts);
       c2[i] = 1.0 - c1[i] * K / ts;
     }
@@ -1522,6 +1735,39 @@ extern "C" int mhmhip_set_routing_celerity(mhmhip_ctx *x, int32_t set, int32_t d
   }));
   drop_graph(x);
   return MHMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int mhmhip_set_routing_celerity(mhmhip_ctx *x, int32_t set, int32_t domain,
+                                           double celerity) {
+  if (!x) return MHMHIP_EINVAL;
+  if (set < 0 || set >= x->n_sets || domain >= x->n_domains || !(celerity > 0.0))
+    return x->fail(MHMHIP_EINVAL, "set_routing_celerity: bad set, domain or celerity");
+  if (!x->net_bound) return x->fail(MHMHIP_ESTATE, "set_routing_celerity: bind a network first");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_celerity");
+  // processCase(8) = 2: K = L11_length / param(1) (mo_mrm_mpr.f90:302-304)
+  return update_param(x, set, domain, [celerity](int64_t) { return celerity; });
+}
+
+extern "C" int mhmhip_set_routing_celerity_links(mhmhip_ctx *x, int32_t set, int32_t domain,
+                                                 const double *celerity) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!celerity || set < 0 || set >= x->n_sets || domain >= x->n_domains)
+    return x->fail(MHMHIP_EINVAL, "set_routing_celerity_links: bad set, domain or pointer");
+  if (!x->net_bound)
+    return x->fail(MHMHIP_ESTATE, "set_routing_celerity_links: bind a network first");
+  // celerity is given for the domain's links (domain >= 0) or all packed links
+  const int64_t base = domain >= 0 ? x->dom_link_start[domain] : 0;
+  const int64_t l_lo = domain >= 0 ? x->dom_link_start[domain] : 0;
+  const int64_t l_hi = domain >= 0 ? x->dom_link_start[domain + 1] : x->dom_link_start[x->n_domains];
+  for (int64_t i = l_lo; i < l_hi; ++i)
+    if (!(celerity[i - base] > 0.0) || !std::isfinite(celerity[i - base]))
+      return x->fail(MHMHIP_EINVAL, "set_routing_celerity_links: celerity of link " +
+                                        std::to_string(i - base + 1) + " is not positive");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_celerity_links");
+  // processCase(8) = 3: K = L11_length / L11_celerity (mo_mrm_mpr.f90:306-311)
+  return update_param(x, set, domain, [celerity, base](int64_t i) { return celerity[i - base]; });
 }
 
 extern "C" int mhmhip_set_inflow(mhmhip_ctx *x, int32_t nsteps, const double *q) {
@@ -1536,6 +1782,7 @@ extern "C" int mhmhip_set_inflow(mhmhip_ctx *x, int32_t nsteps, const double *q)
   CHECK_RC(x->dalloc(&x->d_inflow_q, (int64_t)nsteps * x->n_inflow, "alloc inflow series"));
   CHECK_HIP(x, sync_copy(x, x->d_inflow_q, q, sizeof(double) * nsteps * x->n_inflow, hipMemcpyHostToDevice), "upload inflow");
   x->inflow_q_rows = nsteps;
+  x->h_inflow_row.assign(q, q + x->n_inflow);  // the row an mrm_step reads
   return MHMHIP_OK;
 }
 
@@ -1610,6 +1857,78 @@ extern "C" int mhmhip_get_qmod_sum(mhmhip_ctx *x, int32_t set, double *sum, int6
   return MHMHIP_OK;
 }
 
+namespace {
+// outputFlxState indices with one variable per soil horizon
+bool out_per_horizon(int k) { return k == 3 || k == 4 || k == 17 || k == 19; }
+}  // namespace
+
+extern "C" int mhmhip_output_accumulate(mhmhip_ctx *x, const int32_t *flags) {
+  if (!x) return MHMHIP_EINVAL;
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "output_accumulate");
+  if (!flags) {
+    x->out_on = false;
+    return MHMHIP_OK;
+  }
+  if (flags[17])
+    return x->fail(MHMHIP_EINVAL, "output_accumulate: outputFlxState(18) (neutrons) is not on "
+                                  "this path");
+  // the reference's output order (mHM_updateDataset, mo_write_fluxes_states.f90:318-436)
+  static const int order[] = {1, 2, 3, 4, 5, 6, 7, 8, 18, 9, 10, 11, 12, 13, 14, 15, 16, 17,
+                              19, 20, 21};
+  OutAcc o{};
+  int n = 0;
+  for (int k : order) {
+    if (!flags[k - 1]) continue;
+    const int nh = out_per_horizon(k) ? x->nh : 1;
+    for (int h = 0; h < nh; ++h) {
+      if (n >= kOutRowsMax) return x->fail(MHMHIP_EINVAL, "output_accumulate: too many rows");
+      o.kind[n] = (int8_t)k;
+      o.hor[n] = (int8_t)h;
+      ++n;
+    }
+  }
+  if (n == 0) {
+    x->out_on = false;
+    return MHMHIP_OK;
+  }
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "output_accumulate");
+  if (x->out_rows_cap < n) {
+    x->dfree(x->d_out, x->out_rows_cap * x->ld);
+    x->out_rows_cap = 0;
+    CHECK_RC(x->dalloc(&x->d_out, (int64_t)n * x->ld, "alloc output sums"));  // zeroed
+    x->out_rows_cap = n;
+  } else {
+    CHECK_HIP(x, hipMemsetAsync(x->d_out, 0, sizeof(double) * n * x->ld, x->stream),
+              "output_accumulate");
+  }
+  o.sum = x->d_out;
+  o.n = n;
+  x->out_cfg = o;
+  x->out_on = true;
+  x->out_count = 0;
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_get_output_sum(mhmhip_ctx *x, int32_t k, int32_t horizon, int32_t set,
+                                     double *sum, int64_t *count) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!x->out_cfg.sum) return x->fail(MHMHIP_ESTATE, "get_output_sum: accumulation never started");
+  if (set < 0 || set >= x->n_sets) return x->fail(MHMHIP_EINVAL, "get_output_sum: bad set");
+  const int h = out_per_horizon(k) ? horizon - 1 : 0;
+  int row = -1;
+  for (int r = 0; r < x->out_cfg.n; ++r)
+    if (x->out_cfg.kind[r] == k && x->out_cfg.hor[r] == h) row = r;
+  if (row < 0 || (out_per_horizon(k) && (horizon < 1 || horizon > x->nh)))
+    return x->fail(MHMHIP_EINVAL, "get_output_sum: output " + std::to_string(k) +
+                                      " (horizon " + std::to_string(horizon) +
+                                      ") is not accumulated");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "get_output_sum");
+  if (sum) CHECK_RC(get_row(x, x->out_cfg.sum + (int64_t)row * x->ld + (int64_t)set * x->sstride,
+                            sum));
+  if (count) *count = x->out_count;
+  return MHMHIP_OK;
+}
+
 extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double *qTIN,
                                         const double *qTR, const double *C1, const double *C2) {
   if (!x) return MHMHIP_EINVAL;
@@ -1649,12 +1968,56 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   if (x->n_inflow > 0 && x->inflow_q_rows < 1)
     return x->fail(MHMHIP_ESTATE, "mrm_step: inflow discharge not set");
   int32_t loops = 1;
-  CHECK_RC(rout_loops(x, &loops));
+  double factor = 1.0;
+  CHECK_RC(rout_cadence(x, &loops, &factor));
   if (loops > 1 && x->qacc_on)
     return x->fail(MHMHIP_EINVAL, "mrm_step: Qmod output sums with routing sub-steps are not supported");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "mrm_step");
   if (!st->read_states) CHECK_RC(refresh_reg_rout(x));
   CHECK_RC(ensure_gauge_buf(x, 1));
+  const double *inflow_row = x->d_inflow_q;
+  if (factor > 1.0) {
+    // routing steps longer than the model step (mo_mhm_interface_run.f90:494-512):
+    // RunToRout and InflowDischarge add up; routing happens every
+    // nint(tsRoutFactorIn) steps of tt and at the run's last step (tt ==
+    // nTimeSteps, a shorter last period: tsRoutFactorIn = mod(tt, nint(...)))
+    if (x->qacc_on)
+      return x->fail(MHMHIP_EINVAL, "mrm_step: Qmod output sums with routing steps longer than "
+                                    "the model step are not supported");
+    double fin = factor;
+    const int32_t tt = st->tt;
+    const bool last = st->n_steps > 0 && tt == st->n_steps;
+    const int32_t nn = (int32_t)std::floor(fin + 0.5);
+    if (last && tt % nn != 0) fin = (double)(tt % nn);
+    const bool route = tt % (int32_t)std::floor(fin + 0.5) == 0 || last;
+    if (!x->d_runacc) CHECK_RC(x->dalloc(&x->d_runacc, x->ld, "alloc RunToRout"));  // zeroed
+    if (x->n_inflow > 0) {
+      if (x->inflow_acc.empty()) x->inflow_acc.assign(x->n_inflow, 0.0);
+      for (int32_t g = 0; g < x->n_inflow; ++g)
+        x->inflow_acc[g] = x->inflow_acc[g] + x->h_inflow_row[g];
+    }
+    const double tst = kHourSecs * (double)(x->cfg.timestep * (int32_t)std::floor(fin + 0.5));
+    CHECK_HIP(x, launch_runoff_sum(x->X + XR_TOTAL * x->ld, x->d_runacc, x->d_ring, x->d_area,
+                                   x->d_lag, x->n_cells, x->sstride, x->ld, x->n_sets, tst, route,
+                                   x->stream),
+              "runoff_sum");
+    if (!route) {
+      x->t_route++;
+      CHECK_HIP(x, hipStreamSynchronize(x->stream), "mrm_step");
+      return MHMHIP_PENDING;
+    }
+    if (x->n_inflow > 0) {
+      // InflowDischarge / tsRoutFactorIn, then the sums start again
+      std::vector<double> avg(x->n_inflow);
+      for (int32_t g = 0; g < x->n_inflow; ++g) avg[g] = x->inflow_acc[g] / fin;
+      std::fill(x->inflow_acc.begin(), x->inflow_acc.end(), 0.0);
+      if (!x->d_inflow_avg)
+        CHECK_RC(x->dalloc(&x->d_inflow_avg, x->n_inflow, "alloc inflow average"));
+      CHECK_HIP(x, sync_copy(x, x->d_inflow_avg, avg.data(), sizeof(double) * x->n_inflow,
+                             hipMemcpyHostToDevice), "upload inflow average");
+      inflow_row = x->d_inflow_avg;
+    }
+  }
   const int kind = MHMHIP_K_ROUTE_TILE;
   const TileArgs g = tile_args(x, 1, 1);
   const int64_t ng = (int64_t)x->n_sets * x->n_gauges;
@@ -1671,6 +2034,7 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
     // row lag (the tiles' layout)
     RouteArgs r = route_args(x, 0, 1, 0);
     r.ring_len = 1;
+    r.inflow_q = inflow_row;
     if (loops > 1) r.qacc = x->d_qsub;
     if (x->n_inflow > 0 && it == 0)
       CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, 0, 1, x->stream), "inflow_patch");
@@ -1700,8 +2064,10 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
   return prof_collect(x);
 }
 
-extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
-                          const mhmhip_run_opts *opts, double *gauge_out) {
+namespace {
+
+int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
+             const mhmhip_run_opts *opts, double *gauge_out) {
   if (!x) return MHMHIP_EINVAL;
   if (!steps || nsteps < 1) return x->fail(MHMHIP_EINVAL, "run: no steps");
   mhmhip_run_opts o{};
@@ -1715,14 +2081,19 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   if (o.route_window < 0 || o.route_window > kTileWindowMax)
     return x->fail(MHMHIP_EINVAL, "run: route_window must be 0.." + std::to_string(kTileWindowMax));
   const bool route = routing_on(x);
+  // rep: routing sub-steps per model step (adaptive routing step shorter than
+  // the model step, mo_mrm_routing.f90:224-284): the routing runs rep steps on
+  // each model step's qOUT, and a model step's Qmod and gauges are the
+  // average of its sub-steps'
+  int32_t rep = 1;
   if (route) {
-    int32_t loops = 1;
-    CHECK_RC(rout_loops(x, &loops));
-    if (loops > 1)
-      return x->fail(MHMHIP_EINVAL, "run: routing sub-steps (adaptive routing step shorter "
-                                    "than the model step) run through mhmhip_mrm_step");
+    CHECK_RC(rout_loops(x, &rep));
+    if (rep > 1 && x->qacc_on)
+      return x->fail(MHMHIP_EINVAL, "run: Qmod output sums with routing sub-steps are not "
+                                    "supported");
     x->qsub_loops = 0;
   }
+  const int64_t nr = (int64_t)nsteps * rep;  // routing steps of the call
   if (route && x->t_route != x->t_cell)
     return x->fail(MHMHIP_ESTATE, "run: previous step not routed (call mrm_step)");
   // read_restart is a constant of a run in the reference: one value per call
@@ -1735,29 +2106,46 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   if (route && !steps[0].read_states) CHECK_RC(refresh_reg_rout(x));
   std::vector<StepDev> hs(nsteps);
   for (int32_t k = 0; k < nsteps; ++k) CHECK_RC(make_step(x, steps[k], hs[k]));
+  // a step whose forcing column, day and day/night factors equal the step
+  // before's has the same disaggregated meteo values (the fused kernel keeps
+  // them; its first step of a launch computes them regardless)
+  for (int32_t k = 1; k < nsteps; ++k) {
+    const StepDev &p = hs[k - 1], &c = hs[k];
+    hs[k].meteo_same = p.day == c.day && p.doy == c.doy && p.month0 == c.month0 &&
+                       p.f_pet == c.f_pet && p.f_prec == c.f_prec && p.f_temp == c.f_temp;
+  }
+  CHECK_RC(forcing_wait(x, hs.data(), nsteps));
   const int S = o.mode == MHMHIP_RUN_PLAIN ? 1 : (o.steps_per_launch > 0 ? o.steps_per_launch : 24);
   const int block = o.block > 0 ? o.block : 256;
   const int kind = route_kind(o.route_mode);
   const bool wg = kind == MHMHIP_K_ROUTE_WG;
   // tiled routing: windows of B steps (the plain path routes in windows too)
   const int32_t B = tile_window(o.route_window);
+  const int64_t ring_rows = route ? ring_need(x, kind, S * rep, B, nr) : 0;
   CHECK_RC(ensure_steps(x, nsteps));
   CHECK_HIP(x, hipMemcpyAsync(x->d_steps, hs.data(), sizeof(StepDev) * nsteps,
                               hipMemcpyHostToDevice, x->stream),
             "upload steps");
-  const int64_t ring_rows = route ? ring_need(x, kind, S, B, nsteps) : 0;
   if (route) {
     CHECK_RC(ensure_ring(x, ring_rows + kTileHeight));  // ring_row's rows past R
-    CHECK_RC(ensure_gauge_buf(x, nsteps));
+    CHECK_RC(ensure_gauge_buf(x, nr));
+    if (rep > 1 && !x->d_qsub) CHECK_RC(x->dalloc(&x->d_qsub, x->ld, "alloc Qmod sub-step sums"));
   }
   if (route) CHECK_RC(normalize_qtr(x));
   CellArgs a = cell_args(x, route);
-  RouteArgs r = route_args(x, 0, nsteps, 0);
+  RouteArgs r = route_args(x, 0, nr, 0);
   if (route) {
     a.ring_len = r.ring_len = ring_rows;  // this call's rows of the ring
+    a.ring_rep = r.rep = rep;
     if (kind == MHMHIP_K_ROUTE_TILE) a.lag = x->d_lag;  // qOUT written lag rows late
+    if (rep > 1) {
+      // the last model step's sub-steps summed into d_qsub: its Qmod
+      CHECK_HIP(x, hipMemsetAsync(x->d_qsub, 0, sizeof(double) * x->ld, x->stream), "run");
+      r.qacc = x->d_qsub;
+      r.acc_from = nr - rep;
+    }
   }
-  const TileArgs g = tile_args(x, B, nsteps);
+  const TileArgs g = tile_args(x, B, nr);
   const int32_t L_end = g.nwin + g.tau_max;
   const bool graph = o.use_graph && o.mode == MHMHIP_RUN_FUSED;
   // A captured fused run puts the tiled routing on a stream of its own: a
@@ -1768,7 +2156,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
   const bool two = graph && route && kind == MHMHIP_K_ROUTE_TILE;
   if (two && !x->rstream)
     CHECK_HIP(x, hipStreamCreateWithFlags(&x->rstream, hipStreamNonBlocking), "routing stream");
-  const bool wrap = ring_rows < nsteps;
+  const bool wrap = ring_rows < nr;
   // the whole loop, in steps relative to this call (replayable as a graph)
   auto enqueue = [&]() -> int {
     int32_t L = 0;  // next tiled routing launch
@@ -1777,10 +2165,12 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     if (two) CHECK_RC(capture_edge(x, x->stream, rs));  // the routing stream joins the capture
     for (int32_t k = 0; k < nsteps; k += S) {
       const int n = std::min<int32_t>(S, nsteps - k);
-      if (two && wrap && (int64_t)k + n - 1 - ring_rows >= 0) {
-        // rows of the steps up to k + n - 1 - R are overwritten: the last
-        // routing launch reading them is enqueued already
-        const int64_t Ln = ((int64_t)k + n - 1 - ring_rows) / B + g.tau_max;
+      // the launch's routing steps [u0, u1)
+      const int64_t u0 = (int64_t)k * rep, u1 = (int64_t)(k + n) * rep;
+      if (two && wrap && u1 - 1 - ring_rows >= 0) {
+        // rows of the routing steps up to u1 - 1 - R are overwritten: the
+        // last routing launch reading them is enqueued already
+        const int64_t Ln = (u1 - 1 - ring_rows) / B + g.tau_max;
         if (Ln >= L) return x->fail(MHMHIP_EINVAL, "run: ring shorter than the routing lag");
         int64_t Lw = Ln;
         while (Lw >= 0 && !evL[Lw]) --Lw;  // a launch with no bin due reads nothing
@@ -1799,12 +2189,12 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
       }
       if (!route) continue;
       if (kind == MHMHIP_K_ROUTE_TILE && x->n_inflow > 0)
-        CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, k, k + n, x->stream),
+        CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, u0, u1, x->stream, rep),
                   "inflow_patch");
       if (kind == MHMHIP_K_ROUTE_TILE) {
         // launch L needs the steps of window L (its deepest tiles' window)
         int32_t L1 = L;
-        while (L1 < L_end && std::min<int64_t>((int64_t)(L1 + 1) * B, nsteps) <= k + n) ++L1;
+        while (L1 < L_end && std::min<int64_t>((int64_t)(L1 + 1) * B, nr) <= u1) ++L1;
         if (two && L1 > L) {
           CHECK_RC(capture_edge(x, x->stream, rs));
           for (int32_t l = L; l < L1; ++l) {
@@ -1816,7 +2206,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
         }
         L = L1;
       } else {
-        CHECK_RC(route_diagonals(x, r, k, k + n, false, wg));
+        CHECK_RC(route_diagonals(x, r, u0, u1, false, wg));
       }
     }
     if (route && kind == MHMHIP_K_ROUTE_TILE) {
@@ -1830,7 +2220,7 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
         CHECK_RC(route_tiles(x, r, g, L, L_end));
       }
     }
-    if (route && kind != MHMHIP_K_ROUTE_TILE) CHECK_RC(route_diagonals(x, r, nsteps, 0, true, wg));
+    if (route && kind != MHMHIP_K_ROUTE_TILE) CHECK_RC(route_diagonals(x, r, nr, 0, true, wg));
     return MHMHIP_OK;
   };
   if (graph) {
@@ -1848,6 +2238,9 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     key.ring_len = ring_rows;
     key.B = B;
     key.qacc = r.qacc;
+    key.out = a.out.sum;
+    key.out_n = a.out.sum ? a.out.n : 0;
+    key.rep = rep;
     if (!x->graph_exec || !(key == x->gkey)) {
       drop_graph(x);
       CHECK_HIP(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -1870,22 +2263,42 @@ extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nstep
     CHECK_RC(enqueue());
   }
   if (route) {
-    credit_route(x, kind, nsteps);
-    x->qtr_slot = (int)((nsteps - 1) & 1);
+    credit_route(x, kind, nr);
+    x->qtr_slot = (int)((nr - 1) & 1);
     if (x->qacc_on) x->qacc_count += nsteps;
+    x->qsub_loops = rep;
   }
   x->t_cell += nsteps;
   x->t_route = x->t_cell;
+  if (x->out_on) x->out_count += nsteps;
+  const int64_t gper = (int64_t)x->n_sets * x->n_gauges;
+  std::vector<double> gsub(route && gauge_out && rep > 1 ? nr * gper : 0);
   if (route && gauge_out && x->n_gauges > 0) {
-    CHECK_HIP(x, hipMemcpyAsync(gauge_out, x->d_gauge_buf,
-                                sizeof(double) * nsteps * x->n_sets * x->n_gauges,
-                                hipMemcpyDeviceToHost, x->stream),
+    CHECK_HIP(x, hipMemcpyAsync(rep > 1 ? gsub.data() : gauge_out, x->d_gauge_buf,
+                                sizeof(double) * nr * gper, hipMemcpyDeviceToHost, x->stream),
               "download gauges");
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "run");
+  if (!gsub.empty()) {
+    // qAcc = qAcc + Qmod over the sub-steps, then Qmod = qAcc / rout_loop
+    // (mo_mrm_routing.f90:246-284), per gauge
+    for (int64_t t = 0; t < nsteps; ++t)
+      for (int64_t i = 0; i < gper; ++i) {
+        double acc = 0.0;
+        for (int32_t j = 0; j < rep; ++j) acc = acc + gsub[(t * rep + j) * gper + i];
+        gauge_out[t * gper + i] = acc / (double)rep;
+      }
+  }
   if (route && gauge_out) copy_dup_gauges(x, gauge_out, nsteps);
   if (graph && x->prof) CHECK_RC(prof_collect_graph(x));
   return prof_collect(x);
+}
+
+}  // namespace
+
+extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
+                          const mhmhip_run_opts *opts, double *gauge_out) {
+  return run_impl(x, steps, nsteps, opts, gauge_out);
 }
 
 extern "C" int mhmhip_set_profiling(mhmhip_ctx *x, int32_t on) {

@@ -9,7 +9,8 @@
 //   P[row][ld]   effective parameters          (ld = n_sets * sstride)
 //   S[row][ld]   states
 //   X[row][ld]   fluxes of the last step
-//   F[var][day][sstride]  forcing, shared by all parameter sets
+//   F[var][column][sstride]  forcing, shared by all parameter sets; day d
+//                (iMeteoTS) in column (d - 1) mod n_forcing_days
 //   ring[R + kTileHeight][ld]  qOUT per step (L1 total runoff in m3/s),
 //                consumed by the routing: step t of node p in row
 //                (t mod R) + lag[p] (ring_row), lag 0 for the diagonal kernels
@@ -63,7 +64,7 @@ struct StepDev {
   int32_t isday;                // 6 < hour <= 18
   int32_t init_sm;              // tt == 1 .and. .not. read_states
   int32_t doy;
-  int32_t pad;
+  int32_t meteo_same;           // same forcing column, day and factors as the step before
 };
 
 struct MonthTables {
@@ -71,6 +72,18 @@ struct MonthTables {
   double fday_pet[12], fnight_pet[12];
   double fday_prec[12], fnight_prec[12];
   double fday_temp[12], fnight_temp[12];
+};
+
+// Output aggregation of L1 states and fluxes (mHM_updateDataset,
+// src/mHM/mo_write_fluxes_states.f90:283-438): row r of the sums holds output
+// kind[r] (outputFlxState index 1..21) of horizon hor[r]; every step adds
+// each row's value to its sum (updateVariable, mo_nc_output.f90:140-148).
+constexpr int kOutRowsMax = 64;
+struct OutAcc {
+  double *sum;               // [n][ld], null: off
+  int32_t n;
+  int8_t kind[kOutRowsMax];
+  int8_t hor[kOutRowsMax];
 };
 
 struct CellArgs {
@@ -92,9 +105,11 @@ struct CellArgs {
   int32_t pet_case;
   int32_t jarvis;
   int32_t hourly;
-  int32_t pad;
+  int32_t ring_rep;        // routing sub-steps per model step: qOUT goes to that
+                           // many consecutive ring rows (1: one row per step)
   double nT;               // steps per day as real (nTstepDay_dp)
   double rnT;              // RN(1 / nT)
+  OutAcc out;              // output sums (out.sum null: none)
 };
 
 // Routing node metadata, packed in 8 bytes per node.
@@ -201,11 +216,15 @@ struct RouteArgs {
   int64_t ring_len;
   int64_t t_lo, t_hi;      // active global steps [t_lo, t_hi)
   int64_t t_base;          // global step of gauge_out/inflow_q row 0
+  int64_t acc_from;        // qacc adds the steps t >= acc_from only
   int32_t n_sets;
   int32_t n_gauges;
   int32_t n_inflow;
   int32_t dmax;            // maximum depth over the nodes this launch covers
   double tst;              // HourSecs * timestep [s]
+  int32_t rep;             // routing sub-steps per model step (inflow_q rows are
+                           // per model step: row (t - t_base) / rep)
+  int32_t pad;
 };
 
 // Extra arguments of the tiled routing kernel.

@@ -39,6 +39,14 @@ __device__ __forceinline__ void str(double *row, uint32_t boff, double x) {
 // wait it puts there (every step) also waits for the step before's ring
 // store.
 __device__ __forceinline__ void settle(double &v) { asm volatile("" : "+v"(v)); }
+// The lane's byte offset, made opaque where it is used: an address formed
+// from it inside a loop cannot be hoisted out of the loop.  Rows stored in a
+// block that runs once per launch (the last step's fluxes) then keep no
+// 64-bit per-lane address in VGPRs across the whole time loop.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
 // streaming store for outputs no kernel of the step reads back
 #ifndef MHMHIP_NT_STORE
 #define MHMHIP_NT_STORE 1
@@ -116,7 +124,9 @@ __device__ __forceinline__ void load_states(const CellArgs &a, uint32_t bv, Cell
 }
 
 template <int NH>
-__device__ __forceinline__ void store_states(const CellArgs &a, uint32_t bv, const CellRegs<NH> &c) {
+__device__ __forceinline__ void store_states(const CellArgs &a, uint32_t bv0,
+                                             const CellRegs<NH> &c) {
+  const uint32_t bv = opaque(bv0);
   auto S = [&](int r, double x) { str(a.S + (int64_t)r * a.ld, bv, x); };
   S(SR_INTER, c.inter);
   S(SR_SNOW, c.pack);
@@ -128,8 +138,9 @@ __device__ __forceinline__ void store_states(const CellArgs &a, uint32_t bv, con
 }
 
 template <int NH>
-__device__ __forceinline__ void store_fluxes_pre(const CellArgs &a, uint32_t bv,
+__device__ __forceinline__ void store_fluxes_pre(const CellArgs &a, uint32_t bv0,
                                                  const FluxRegs<NH> &x) {
+  const uint32_t bv = opaque(bv0);
   auto X = [&](int r, double y) { stn(a.X + (int64_t)r * a.ld, bv, y); };
   X(XR_PET, x.pet);
   X(XR_TEMP, x.temp);
@@ -153,8 +164,9 @@ __device__ __forceinline__ void store_fluxes_pre(const CellArgs &a, uint32_t bv,
 
 // the fluxes chain_post sets
 template <int NH>
-__device__ __forceinline__ void store_fluxes_post(const CellArgs &a, uint32_t bv,
+__device__ __forceinline__ void store_fluxes_post(const CellArgs &a, uint32_t bv0,
                                                   const FluxRegs<NH> &x) {
+  const uint32_t bv = opaque(bv0);
   auto X = [&](int r, double y) { stn(a.X + (int64_t)r * a.ld, bv, y); };
   X(XR_SLOW, x.slow);
   X(XR_PERC, x.perc);
@@ -166,6 +178,70 @@ template <int NH>
 __device__ __forceinline__ void store_fluxes(const CellArgs &a, uint32_t bv, const FluxRegs<NH> &x) {
   store_fluxes_pre<NH>(a, bv, x);
   store_fluxes_post<NH>(a, bv, x);
+}
+
+// One step's contribution to the output sums (mHM_updateDataset,
+// src/mHM/mo_write_fluxes_states.f90:283-438, updateVariable): each selected
+// output's value of the step -- states after it, its fluxes -- in the
+// reference's expression and operation order, added to its sum.  Horizon
+// sums run from 0 over the horizons in order (sum(..., dim = 2)); the
+// unsealed fraction is 1 - fSealed (mo_mhm_interface_run.f90:239).
+template <int NH>
+__device__ __forceinline__ double pick_h(const double *v, int h) {
+  double y = v[0];
+#pragma unroll
+  for (int k = 1; k < NH; ++k)
+    if (h == k) y = v[k];
+  return y;
+}
+
+template <int NH>
+__device__ __forceinline__ void add_outputs(const CellArgs &a, uint32_t bv, const CellRegs<NH> &c,
+                                         const FluxRegs<NH> &x) {
+  const double fs = c.fS, fns = 1.0 - c.fS;
+  for (int r = 0; r < a.out.n; ++r) {
+    const int h = a.out.hor[r];
+    double v = 0.0;
+    switch (a.out.kind[r]) {
+      case 1: v = c.inter; break;
+      case 2: v = c.pack; break;
+      case 3: v = pick_h<NH>(c.sm, h); break;
+      case 4: v = pick_h<NH>(c.sm, h) / pick_h<NH>(c.sat, h); break;
+      case 5: {
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) s1 = s1 + c.sm[k];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) s2 = s2 + c.sat[k];
+        v = s1 / s2;
+        break;
+      }
+      case 6: v = c.seal; break;
+      case 7: v = c.unsat; break;
+      case 8: v = c.satS; break;
+      case 9: v = x.pet; break;
+      case 10: {
+        double s1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) s1 = s1 + x.aet[k];
+        v = s1 * fns + x.ecan + x.aseal * fs;
+        break;
+      }
+      case 11: v = x.total; break;
+      case 12: v = x.rseal * fs; break;
+      case 13: v = x.fast * fns; break;
+      case 14: v = x.slow * fns; break;
+      case 15: v = x.bflow * fns; break;
+      case 16: v = x.perc * fns; break;
+      case 17: v = pick_h<NH>(x.infil, h) * fns; break;
+      case 19: v = pick_h<NH>(x.aet, h) * fns; break;
+      case 20: v = x.peff; break;
+      case 21: v = x.melt; break;
+      default: break;
+    }
+    double *row = a.out.sum + (int64_t)r * a.ld;
+    str(row, bv, ldr(row, bv) + v);
+  }
 }
 
 // Daily PET before disaggregation (get_corrected_pet, mo_meteo_handler.f90:1050-1097).
@@ -206,7 +282,7 @@ __device__ __forceinline__ StepDev read_step(const ConstStep *p) {
   st.isday = p->isday;
   st.init_sm = p->init_sm;
   st.doy = p->doy;
-  st.pad = 0;
+  st.meteo_same = p->meteo_same;
   return st;
 }
 
@@ -246,15 +322,36 @@ __device__ __forceinline__ DV<NH> horizon_pows(const CellRegs<NH> &c) {
 // unsat**(1 + alpha), percolation, baseflow, total runoff).  The canopy's
 // (interc / interc_max)**(2/3) is pow23.
 // JV: Jarvis soil-moisture stress (processCase(3) = 2, 3) instead of Feddes
+// The soil-moisture stress parameters of a cell (root fractions, field
+// capacity, wilting point, RN(1 / (fc - wp))), passed on their own: the
+// fused kernel keeps them in LDS between steps.
+template <int NH>
+struct Stress {
+  double fr[NH], fc[NH], wp[NH], rfw[NH];
+};
+
+template <int NH>
+__device__ __forceinline__ Stress<NH> stress_of(const CellRegs<NH> &c) {
+  Stress<NH> t;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    t.fr[h] = c.fr[h];
+    t.fc[h] = c.fc[h];
+    t.wp[h] = c.wp[h];
+    t.rfw[h] = c.rfw[h];
+  }
+  return t;
+}
+
 template <int NH, bool JV>
 __device__ __forceinline__ void chain_pre(CellRegs<NH> &c, FluxRegs<NH> &x, double maxinter,
                                           double rmaxinter, double evap, double revap,
-                                          const double *pwh) {
+                                          const double *pwh, const Stress<NH> &t) {
   const double ratio = canopy_fill(maxinter, rmaxinter, x.prec, c.inter, x.thr);
   canopy_evap(x.pet, maxinter, ratio, ratio > 0.0 ? pow23(ratio) : 0.0, c.inter, x.ecan);
   snow_accum_melt(c.incr, c.dmaxs, c.dnps, c.ddthr, x.prec, x.temp, c.tth, x.thr, c.pack, x.ddf,
                   x.melt, x.peff, x.rain, x.snow);
-  soil_moisture<NH, JV>(c.fS, c.thrS, c.rthrS, x.pet, evap, revap, c.sat, c.fr, c.fc, c.wp, c.rfw,
+  soil_moisture<NH, JV>(c.fS, c.thrS, c.rthrS, x.pet, evap, revap, c.sat, t.fr, t.fc, t.wp, t.rfw,
                         c.c1, x.ecan, x.peff, pwh, x.rseal, c.seal, x.infil, c.sm, x.aet, x.aseal);
   runoff_fast(c.k0s, x.infil[NH - 1], c.thrU, c.unsat, x.fast);
 }
@@ -271,13 +368,13 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
                                            double maxinter, double rmaxinter, double evap,
                                            double revap) {
   const DV<NH> pwh = horizon_pows<NH>(c);
-  chain_pre<NH, JV>(c, x, maxinter, rmaxinter, evap, revap, pwh.v);
+  chain_pre<NH, JV>(c, x, maxinter, rmaxinter, evap, revap, pwh.v, stress_of<NH>(c));
   chain_post<NH>(c, x, c.unsat > 0.0 ? exp_ylogx(1.0 + c.alpha, c.unsat) : 0.0);
 }
 
 // ---------------------------------------------------------------------------
 // Plain path: one launch per model step; every state and flux is written.
-template <int NH, bool JV>
+template <int NH, bool JV, bool OUT>
 __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, int64_t t) {
   logtab_fill();  // the chain's powers read it (before any thread returns)
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -305,10 +402,26 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
   chain_step<NH, JV>(a, r, x, maxinter, 1.0 / maxinter, st.evap, st.revap);
   store_states<NH>(a, bv, r);
   store_fluxes<NH>(a, bv, x);
-  if (a.ring)
-    a.ring[ring_row(t, a.lag ? a.lag[c] : 0, a.ring_len) * a.ld + bv / 8u] =
-        runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst);
+  if (OUT) add_outputs<NH>(a, bv, r, x);
+  if (a.ring) {
+    const double q = runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst);
+    const int32_t g = a.lag ? a.lag[c] : 0;
+    for (int32_t i = 0; i < a.ring_rep; ++i)
+      a.ring[ring_row(t * a.ring_rep + i, g, a.ring_len) * a.ld + bv / 8u] = q;
+  }
 }
+
+// Per-lane LDS slots of the fused kernel (dynamic shared memory,
+// lane_slots(NH) doubles per thread, laid out lane after lane; an odd count
+// puts the 64-bit accesses of a wave's lanes on distinct bank pairs): the
+// values that change once per forcing day, month or day/night switch, and
+// the soil-moisture stress parameters read once per step -- in LDS rather
+// than in registers across the whole time loop.
+enum LaneSlot {
+  LS_PRE = 0, LS_TAVG, LS_PD, LS_MAXI, LS_RMAXI, LS_XPET, LS_XTEMP, LS_XPREC, LS_STRESS
+};
+__host__ __device__ constexpr int lane_slots(int nh) { return (LS_STRESS + 4 * nh) | 1; }
+extern __shared__ double s_lane[];
 
 // ---------------------------------------------------------------------------
 // Fused path: nsteps model steps per launch with parameters and states held
@@ -318,7 +431,7 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
 // are written at the end and the last step's fluxes as soon as each is known.
 // Per-step fluxes are scoped to the loop body so they do not stay live across
 // steps.
-template <int NH, bool JV>
+template <int NH, bool JV, bool OUT>
 #ifndef MHMHIP_FUSED_WAVES
 #define MHMHIP_FUSED_WAVES 2
 #endif
@@ -333,17 +446,25 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
   CellRegs<NH> r;
   load_params<NH, JV>(a, bv, r);
   load_states<NH>(a, bv, r);
+  // this lane's LDS slots (lane_slots(NH) doubles, laid out lane by lane)
+  const uint32_t lane0 = threadIdx.x * (uint32_t)lane_slots(NH);
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    s_lane[lane0 + LS_STRESS + h] = r.fr[h];
+    s_lane[lane0 + LS_STRESS + NH + h] = r.fc[h];
+    s_lane[lane0 + LS_STRESS + 2 * NH + h] = r.wp[h];
+    s_lane[lane0 + LS_STRESS + 3 * NH + h] = r.rfw[h];
+  }
   const double area = a.ring ? ldr(a.area, bc) : 0.0;
   const int64_t fv = a.forcing_days * a.sstride;
   int cached_day = -1, cached_doy = -1, cached_month = -1;
-  double pre = 0.0, tavg = 0.0, pd = 0.0, maxinter = 0.0, rmaxinter = 0.0;
   const ConstStep *steps = (const ConstStep *)(a.steps + step0);
   // ring: the lane's row offset (its lag) is fixed, the step's row uniform
   double *rlane = nullptr;
   int64_t roff = 0;
   if (a.ring) {
     rlane = a.ring + (int64_t)(a.lag ? a.lag[c] : 0) * a.ld + bv / 8u;
-    roff = t0 % a.ring_len * a.ld;
+    roff = t0 * a.ring_rep % a.ring_len * a.ld;
   }
   // The horizon powers of a step depend only on its starting soil moisture,
   // known when the step before reaches its slow interflow: one call
@@ -354,40 +475,66 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
 #pragma nounroll
   for (int32_t s = 0; s < nsteps; ++s) {
     const StepDev st = read_step(steps + s);
+    // the slot index opaque each step: the slots are read where they are
+    // used, not held in registers across the loop
+    const uint32_t li = opaque(lane0);
     if (st.day != cached_day || st.doy != cached_doy) {
+      // offsets opaque here: no per-lane address of these rows is held
+      // across the time loop
+      const uint32_t oc = opaque(bc), ov = opaque(bv);
       const double *F = a.F + (int64_t)st.day * a.sstride;
-      pre = ldr(F + MHMHIP_F_PRE * fv, bc);
-      tavg = ldr(F + MHMHIP_F_TAVG * fv, bc);
-      const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, bc) : 0.0;
-      const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, bc) : 0.0;
-      pd = pet_day<NH>(a, st, bv, bc, tavg, tmax, tmin);
-      settle(pre);
-      settle(tavg);
-      settle(pd);
+      const double pre = ldr(F + MHMHIP_F_PRE * fv, oc);
+      const double tavg = ldr(F + MHMHIP_F_TAVG * fv, oc);
+      const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, oc) : 0.0;
+      const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, oc) : 0.0;
+      s_lane[li + LS_PRE] = pre;
+      s_lane[li + LS_TAVG] = tavg;
+      s_lane[li + LS_PD] = pet_day<NH>(a, st, ov, oc, tavg, tmax, tmin);
       cached_day = st.day;
       cached_doy = st.doy;
     }
     if (st.month0 != cached_month) {
-      maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld, bv);
-      rmaxinter = 1.0 / maxinter;
-      settle(maxinter);
-      settle(rmaxinter);
+      const double maxinter = ldr(a.P + (int64_t)(pr_maxinter(NH) + st.month0) * a.ld,
+                                  opaque(bv));
+      s_lane[li + LS_MAXI] = maxinter;
+      s_lane[li + LS_RMAXI] = 1.0 / maxinter;
       cached_month = st.month0;
     }
     FluxRegs<NH> x;
-    x.pet = a.hourly ? pd : disagg_flux(a.nT, a.rnT, pd, st.f_pet);
-    x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
-    x.prec = a.hourly ? pre : disagg_flux(a.nT, a.rnT, pre, st.f_prec);
+    if (s == 0 || !st.meteo_same) {
+      // day or night factors changed (or a new day): disaggregate again
+      const double pd = s_lane[li + LS_PD], tavg = s_lane[li + LS_TAVG];
+      const double pre = s_lane[li + LS_PRE];
+      x.pet = a.hourly ? pd : disagg_flux(a.nT, a.rnT, pd, st.f_pet);
+      x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
+      x.prec = a.hourly ? pre : disagg_flux(a.nT, a.rnT, pre, st.f_prec);
+      s_lane[li + LS_XPET] = x.pet;
+      s_lane[li + LS_XTEMP] = x.temp;
+      s_lane[li + LS_XPREC] = x.prec;
+    } else {
+      x.pet = s_lane[li + LS_XPET];
+      x.temp = s_lane[li + LS_XTEMP];
+      x.prec = s_lane[li + LS_XPREC];
+    }
+    Stress<NH> t;
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      t.fr[h] = s_lane[li + LS_STRESS + h];
+      t.fc[h] = s_lane[li + LS_STRESS + NH + h];
+      t.wp[h] = s_lane[li + LS_STRESS + 2 * NH + h];
+      t.rfw[h] = s_lane[li + LS_STRESS + 3 * NH + h];
+    }
     if (st.init_sm) {
 #pragma unroll
-      for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
+      for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * t.fc[h];
       pwh = horizon_pows<NH>(r);
     }
-    chain_pre<NH, JV>(r, x, maxinter, rmaxinter, st.evap, st.revap, pwh.v);
+    chain_pre<NH, JV>(r, x, s_lane[li + LS_MAXI], s_lane[li + LS_RMAXI], st.evap, st.revap,
+                      pwh.v, t);
     // the last step's fluxes leave as soon as they are known, so that they
     // are not held across the powers
     const bool last = s == nsteps - 1;
-    if (last) store_fluxes_pre<NH>(a, bv, x);
+    if (last && !OUT) store_fluxes_pre<NH>(a, bv, x);
     DV<NH + 1> py, px;
     py.v[0] = 1.0 + r.alpha;
     px.v[0] = r.unsat > 0.0 ? r.unsat : 1.0;
@@ -397,11 +544,17 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
 #pragma unroll
     for (int h = 0; h < NH; ++h) pwh.v[h] = pw.v[h + 1];
     if (a.ring) {
-      rlane[roff] = runoff_acc(x.total, area, a.tst, a.rtst);
-      roff += a.ld;
-      if (roff == a.ring_len * a.ld) roff = 0;
+      // the step's qOUT for each of its routing sub-steps (rows in routing steps)
+      const double q = runoff_acc(x.total, area, a.tst, a.rtst);
+      for (int32_t i = 0; i < a.ring_rep; ++i) {
+        rlane[roff] = q;
+        roff += a.ld;
+        if (roff == a.ring_len * a.ld) roff = 0;
+      }
     }
+    if (last && OUT) store_fluxes_pre<NH>(a, bv, x);
     if (last) store_fluxes_post<NH>(a, bv, x);
+    if (OUT) add_outputs<NH>(a, bv, r, x);
   }
   store_states<NH>(a, bv, r);
 }
@@ -419,7 +572,7 @@ __device__ __forceinline__ void route_node(const RouteArgs &r, uint32_t p, int32
   if (y & NF_INFLOW) {
     for (int32_t g = 0; g < r.n_inflow; ++g) {
       if (r.inflow_node[g] == (int32_t)p) {
-        const double qi = r.inflow_q[(t - r.t_base) * r.n_inflow + g];
+        const double qi = r.inflow_q[(t - r.t_base) / r.rep * r.n_inflow + g];
         qout = r.inflow_hw[g] ? qout + qi : qi;
       }
     }
@@ -440,7 +593,7 @@ __device__ __forceinline__ void route_node(const RouteArgs &r, uint32_t p, int32
     str(r.pub + (t & 1) * ld, bv, qtr);
   }
   str(r.qTIN, bv, s);
-  if (r.qacc) str(r.qacc, bv, ldr(r.qacc, bv) + s);
+  if (r.qacc && t >= r.acc_from) str(r.qacc, bv, ldr(r.qacc, bv) + s);
   if (y & NF_GAUGE) {
     for (int32_t g = 0; g < r.n_gauges; ++g)
       if (r.gauge_node[g] == (int32_t)p)
@@ -614,7 +767,8 @@ __device__ __forceinline__ void tile_sweep(const RouteArgs &r, TileLane &u, doub
     const bool upd = (uint32_t)j < nupd;
     if (upd && link) u.qtr1 = q;
     if (upd) u.qtin1 = sum;
-    if (ACC && upd) u.qacc = u.qacc + sum;  // Qmod of step j, added in step order
+    // Qmod of step j, added in step order
+    if (ACC && upd && u.t0 + j >= r.acc_from) u.qacc = u.qacc + sum;
     musk = u.qtr1 + u.c1 * (u.qtin1 - u.qtr1);
     if (upd && j == jlast) u.qlast = qo;
     if (STAGE) {
@@ -792,6 +946,77 @@ hipError_t launch_meteo_l2(const double *data2, const uint8_t *mask2, int32_t nr
   return hipGetLastError();
 }
 
+// Streamed forcing: a chunk of days of the variables in var_mask (bit v:
+// forcing variable v), uploaded in reference cell order (raw[k][j][cell],
+// k counting the variables given), scattered into the forcing columns in
+// device order: F[v][col(j)][pos[cell]], col(j) = (day0 + j) mod n_cols.  One
+// thread per (cell, day of the chunk): the reads are coalesced, the writes
+// follow the tile layout (cells of a tile close together).
+__global__ __launch_bounds__(256) void forcing_gather_kernel(const double *raw, int32_t var_mask,
+                                                             int32_t ndays, int64_t n_cells,
+                                                             const int32_t *pos, double *F,
+                                                             int32_t n_cols, int32_t day0,
+                                                             int64_t sstride) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t j = blockIdx.y;
+  if (c >= n_cells || j >= ndays) return;
+  const int64_t p = pos[c];
+  const int64_t col = (int64_t)((day0 + j) % n_cols);
+  int32_t k = 0;
+  for (int32_t v = 0; v < MHMHIP_F_COUNT; ++v) {
+    if (!((var_mask >> v) & 1)) continue;
+    F[((int64_t)v * n_cols + col) * sstride + p] = raw[((int64_t)k * ndays + j) * n_cells + c];
+    ++k;
+  }
+}
+
+hipError_t launch_forcing_gather(const double *raw, int32_t var_mask, int32_t ndays,
+                                 int64_t n_cells, const int32_t *pos, double *F, int32_t n_cols,
+                                 int32_t day0, int64_t sstride, hipStream_t s) {
+  if (var_mask <= 0 || var_mask >= (1 << MHMHIP_F_COUNT) || ndays < 1 || ndays > 65535 ||
+      n_cells < 1 || n_cols < 1 || day0 < 0)
+    return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((n_cells + 255) / 256), (unsigned)ndays);
+  hipLaunchKernelGGL(forcing_gather_kernel, grid, dim3(256), 0, s, raw, var_mask, ndays, n_cells,
+                     pos, F, n_cols, day0, sstride);
+  return hipGetLastError();
+}
+
+// Routing steps longer than the model step (processCase(8) = 2, 3 with
+// tsRoutFactor >= 1, mo_mhm_interface_run.f90:494-512): every model step adds
+// its L1_total_runoff to RunToRout [mm] (acc = acc + q from 0 after each
+// routing); on a routing step L11_runoff_acc converts the sum with the routing
+// step TS_rout (mo_mrm_pre_routing.f90:110-126: qAcc = 0 + RunToRout * area,
+// then * 1000 / (HourSecs * TS_rout)) into the step's ring row (lag, as the
+// cell kernels write it) and RunToRout starts again from 0.  One thread per
+// (cell, set).
+__global__ __launch_bounds__(256) void runoff_sum_kernel(const double *total, double *acc,
+                                                         double *ring, const double *area,
+                                                         const uint8_t *lag, int64_t n_cells,
+                                                         int64_t sstride, int64_t ld, double tst,
+                                                         int32_t route) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const int64_t i = (int64_t)blockIdx.y * sstride + c;
+  const double sum = acc[i] + total[i];
+  if (route) {
+    ring[(int64_t)lag[c] * ld + i] = ((0.0 + sum * area[c]) * 1000.0) / tst;
+    acc[i] = 0.0;
+  } else {
+    acc[i] = sum;
+  }
+}
+
+hipError_t launch_runoff_sum(const double *total, double *acc, double *ring, const double *area,
+                             const uint8_t *lag, int64_t n_cells, int64_t sstride, int64_t ld,
+                             int n_sets, double tst, bool route, hipStream_t s) {
+  if (n_cells < 1 || n_sets < 1 || !lag) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((n_cells + 255) / 256), (unsigned)n_sets);
+  hipLaunchKernelGGL(runoff_sum_kernel, grid, dim3(256), 0, s, total, acc, ring, area, lag,
+                     n_cells, sstride, ld, tst, route ? 1 : 0);
+  return hipGetLastError();
+}
+
 // Inflow gauges for the tiled routing (add_inflow, mo_mrm_pre_routing.f90:
 // 179-214, as route_node applies them): the qOUT of an inflow node, in the
 // ring rows of steps [t_lo, t_hi), is replaced by the gauge's discharge, or
@@ -799,14 +1024,15 @@ hipError_t launch_meteo_l2(const double *data2, const uint8_t *mask2, int32_t nr
 // One thread per (step, set) applies the gauges in their order.
 __global__ __launch_bounds__(256) void inflow_patch_kernel(RouteArgs r, double *ring,
                                                            const uint8_t *lag, int64_t t_lo,
-                                                           int64_t t_hi) {
+                                                           int64_t t_hi, int32_t rep) {
   const int64_t t = t_lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int32_t set = blockIdx.y;
   if (t >= t_hi) return;
   const int64_t ld = (int64_t)r.n_sets * r.sstride;
   for (int32_t g = 0; g < r.n_inflow; ++g) {
     const int32_t p = r.inflow_node[g];
-    const double qi = r.inflow_q[(t - r.t_base) * r.n_inflow + g];
+    // each routing sub-step of a model step adds that step's inflow
+    const double qi = r.inflow_q[(t - r.t_base) / rep * r.n_inflow + g];
     double &q = ring[ring_row(t, lag[p], r.ring_len) * ld + (int64_t)set * r.sstride + p];
     q = r.inflow_hw[g] ? q + qi : qi;
   }
@@ -818,23 +1044,38 @@ template <int NH>
 static hipError_t step_nh(const CellArgs &a, const StepDev &st, int64_t t, int block, int n_sets,
                           hipStream_t s) {
   dim3 grid((unsigned)((a.n_cells + block - 1) / block), (unsigned)n_sets);
-  if (a.jarvis)
-    hipLaunchKernelGGL((cell_step_kernel<NH, true>), grid, dim3(block), 0, s, a, st, t);
+  const bool out = a.out.sum != nullptr;
+  if (a.jarvis && out)
+    hipLaunchKernelGGL((cell_step_kernel<NH, true, true>), grid, dim3(block), 0, s, a, st, t);
+  else if (a.jarvis)
+    hipLaunchKernelGGL((cell_step_kernel<NH, true, false>), grid, dim3(block), 0, s, a, st, t);
+  else if (out)
+    hipLaunchKernelGGL((cell_step_kernel<NH, false, true>), grid, dim3(block), 0, s, a, st, t);
   else
-    hipLaunchKernelGGL((cell_step_kernel<NH, false>), grid, dim3(block), 0, s, a, st, t);
+    hipLaunchKernelGGL((cell_step_kernel<NH, false, false>), grid, dim3(block), 0, s, a, st, t);
   return hipGetLastError();
 }
 
 template <int NH>
 static hipError_t fused_nh(const CellArgs &a, int64_t t0, int nsteps, int step0, int block,
                            int n_sets, hipStream_t s) {
+  // the lanes' LDS slots of a workgroup stay within 64 KB
+  while ((size_t)block * lane_slots(NH) * sizeof(double) > 65536) block /= 2;
+  const size_t shm = (size_t)block * lane_slots(NH) * sizeof(double);
   dim3 grid((unsigned)((a.n_cells + block - 1) / block), (unsigned)n_sets);
-  if (a.jarvis)
-    hipLaunchKernelGGL((cell_fused_kernel<NH, true>), grid, dim3(block), 0, s, a, t0, nsteps,
-                       step0);
+  const bool out = a.out.sum != nullptr;
+  if (a.jarvis && out)
+    hipLaunchKernelGGL((cell_fused_kernel<NH, true, true>), grid, dim3(block), shm, s, a, t0,
+                       nsteps, step0);
+  else if (a.jarvis)
+    hipLaunchKernelGGL((cell_fused_kernel<NH, true, false>), grid, dim3(block), shm, s, a, t0,
+                       nsteps, step0);
+  else if (out)
+    hipLaunchKernelGGL((cell_fused_kernel<NH, false, true>), grid, dim3(block), shm, s, a, t0,
+                       nsteps, step0);
   else
-    hipLaunchKernelGGL((cell_fused_kernel<NH, false>), grid, dim3(block), 0, s, a, t0, nsteps,
-                       step0);
+    hipLaunchKernelGGL((cell_fused_kernel<NH, false, false>), grid, dim3(block), shm, s, a, t0,
+                       nsteps, step0);
   return hipGetLastError();
 }
 
@@ -904,10 +1145,11 @@ hipError_t launch_route_tile(const RouteArgs &r, const TileArgs &g, int64_t bin_
 }
 
 hipError_t launch_inflow_patch(const RouteArgs &r, double *ring, const uint8_t *lag, int64_t t_lo,
-                               int64_t t_hi, hipStream_t s) {
-  if (t_hi <= t_lo || t_lo < r.t_base || r.n_inflow < 1 || !lag) return hipErrorInvalidValue;
+                               int64_t t_hi, hipStream_t s, int32_t rep) {
+  if (t_hi <= t_lo || t_lo < r.t_base || r.n_inflow < 1 || !lag || rep < 1)
+    return hipErrorInvalidValue;
   dim3 grid((unsigned)((t_hi - t_lo + 255) / 256), (unsigned)r.n_sets);
-  hipLaunchKernelGGL(inflow_patch_kernel, grid, dim3(256), 0, s, r, ring, lag, t_lo, t_hi);
+  hipLaunchKernelGGL(inflow_patch_kernel, grid, dim3(256), 0, s, r, ring, lag, t_lo, t_hi, rep);
   return hipGetLastError();
 }
 

@@ -116,6 +116,24 @@ class Model:
         self._chk(N.lib().mhmhip_set_forcing(self._ctx, N.F[name], day0, data.shape[0],
                                              N.dptr(data), data.shape[1]))
 
+    def stream_forcing(self, data, day0):
+        """Days day0.. of the variables in `data` ({name: (n_days, n_cells)},
+        Fortran L1_x(n_cells, n_days) of all packed cells) uploaded
+        asynchronously into the device's forcing columns (day d in column
+        (d - 1) mod n_forcing_days); returns once the arrays are staged."""
+        ptrs = (N._dp * len(N.F_NAMES))()
+        keep = []
+        nd = None
+        for name, arr in data.items():
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            if a.shape[1] != self.n_cells or (nd is not None and a.shape[0] != nd):
+                raise ValueError(f"{name}: expected (n_days, {self.n_cells}) like the others")
+            nd = a.shape[0]
+            keep.append(a)
+            ptrs[N.F[name]] = N.dptr(a)
+        self._chk(N.lib().mhmhip_stream_forcing(self._ctx, int(day0), int(nd), ptrs,
+                                                self.n_cells))
+
     def set_forcing_l2(self, name, domain, data2, mask2, cellsize2, mask1, cellsize1, day0=1):
         """One domain's forcing on its meteo grid (L2), mapped to its L1 cells on
         the device as meteo_forcings_wrapper does (mo_meteo_helper.f90:119-143).
@@ -188,6 +206,12 @@ class Model:
         """Adaptive-timestep routing, constant celerity [m/s] (processCase(8) = 2)."""
         self._chk(N.lib().mhmhip_set_routing_celerity(self._ctx, set_, domain, float(celerity)))
 
+    def set_routing_celerity_links(self, celerity, set_=0, domain=-1):
+        """Adaptive-timestep routing with a celerity per link [m/s] (processCase(8) = 3);
+        celerity over the domain's links (or every packed link for domain < 0)."""
+        c = np.ascontiguousarray(celerity, dtype=np.float64)
+        self._chk(N.lib().mhmhip_set_routing_celerity_links(self._ctx, set_, domain, N.dptr(c)))
+
     def set_inflow(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)
         self._chk(N.lib().mhmhip_set_inflow(self._ctx, q.shape[0], N.dptr(q)))
@@ -222,13 +246,38 @@ class Model:
         self._chk(N.lib().mhmhip_get_qmod_sum(self._ctx, set_, N.dptr(out), C.byref(cnt)))
         return out, cnt.value
 
+    def output_accumulate(self, flags):
+        """Start summing the selected L1 outputs every step on the device
+        (mHM_updateDataset; flags: 21 outputFlxState switches, index k-1 for
+        output k); None stops."""
+        if flags is None:
+            self._chk(N.lib().mhmhip_output_accumulate(self._ctx, None))
+            return
+        f = np.ascontiguousarray([1 if v else 0 for v in flags], dtype=np.int32)
+        if f.shape[0] != 21:
+            raise ValueError("21 outputFlxState switches expected")
+        self._chk(N.lib().mhmhip_output_accumulate(self._ctx, N.iptr(f)))
+
+    def output_sum(self, k, horizon=1, set_=0):
+        """(per-cell sum of output k in reference cell order, steps added)."""
+        out = np.zeros(self.n_cells)
+        cnt = C.c_int64()
+        self._chk(N.lib().mhmhip_get_output_sum(self._ctx, int(k), int(horizon), set_,
+                                                N.dptr(out), C.byref(cnt)))
+        return out, cnt.value
+
     # -- stepping ----------------------------------------------------------------
     def mhm_step(self, step):
         self._chk(N.lib().mhmhip_mhm_step(self._ctx, C.byref(step)))
 
     def mrm_step(self, step):
+        """Gauges (n_sets, n_gauges) of the step, or None when the step only
+        accumulated runoff for a routing step longer than the model step."""
         g = np.zeros((self.n_sets, max(self.n_gauges, 1)))
-        self._chk(N.lib().mhmhip_mrm_step(self._ctx, C.byref(step), N.dptr(g)))
+        rc = N.lib().mhmhip_mrm_step(self._ctx, C.byref(step), N.dptr(g))
+        if rc == N.PENDING:
+            return None
+        self._chk(rc)
         return g[:, :self.n_gauges]
 
     def run(self, steps, mode=N.RUN_FUSED, steps_per_launch=0, block=0, route_mode=0,

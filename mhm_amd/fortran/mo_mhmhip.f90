@@ -10,7 +10,7 @@ module mo_mhmhip
   implicit none
   private
 
-  integer(c_int), parameter, public :: MHMHIP_OK = 0
+  integer(c_int), parameter, public :: MHMHIP_OK = 0, MHMHIP_PENDING = 1
   integer(c_int32_t), parameter, public :: MHMHIP_RUN_PLAIN = 0, MHMHIP_RUN_FUSED = 1
 
   ! enum mhmhip_var
@@ -38,7 +38,7 @@ module mo_mhmhip
   end type
 
   type, bind(C), public :: mhmhip_step
-    integer(c_int32_t) :: tt, month, hour, doy, iMeteoTS, read_states
+    integer(c_int32_t) :: tt, month, hour, doy, iMeteoTS, read_states, n_steps = 0
   end type
 
   type, bind(C), public :: mhmhip_network
@@ -63,11 +63,14 @@ module mo_mhmhip
 
   public :: mhmhip_create, mhmhip_destroy, mhmhip_last_error, mhmhip_device_bytes, mhmhip_sync, &
             mhmhip_set_cell_var, mhmhip_get_cell_var, mhmhip_default_states, &
-            mhmhip_set_forcing, mhmhip_set_forcing_l2, mhmhip_set_routing_celerity, &
+            mhmhip_set_forcing, mhmhip_stream_forcing, mhmhip_set_forcing_l2, &
+            mhmhip_set_routing_celerity, mhmhip_set_routing_celerity_links, &
             mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, &
             mhmhip_set_routing_params, mhmhip_set_inflow, mhmhip_get_routing_state, &
-            mhmhip_set_routing_state, mhmhip_mrm_step, mhmhip_run, mhmhip_set_profiling, &
-            mhmhip_get_profile, mhmhip_plan_network, mhmhip_qmod_accumulate, mhmhip_get_qmod_sum
+            mhmhip_set_routing_state, mhmhip_mrm_step, mhmhip_run, &
+            mhmhip_set_profiling, &
+            mhmhip_get_profile, mhmhip_plan_network, mhmhip_qmod_accumulate, mhmhip_get_qmod_sum, &
+            mhmhip_output_accumulate, mhmhip_get_output_sum
   public :: mhmhip_check, mhm_hip_put, mhm_hip_get, mhm_hip_do_time_step
 
   interface
@@ -110,6 +113,14 @@ module mo_mhmhip
         bind(C, name="mhmhip_default_states")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, horizon_depth
+    end function
+    integer(c_int) function mhmhip_stream_forcing(ctx, day0, ndays, host, ld) &
+        bind(C, name="mhmhip_stream_forcing")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: day0, ndays
+      type(c_ptr), intent(in) :: host(5)
+      integer(c_int64_t), value :: ld
     end function
     integer(c_int) function mhmhip_set_forcing(ctx, var, day0, ndays, host, ld) &
         bind(C, name="mhmhip_set_forcing")
@@ -157,6 +168,12 @@ module mo_mhmhip
       integer(c_int32_t), value :: set, domain
       real(c_double), intent(in) :: gamma(5)
     end function
+    integer(c_int) function mhmhip_set_routing_celerity_links(ctx, set, domain, celerity) &
+        bind(C, name="mhmhip_set_routing_celerity_links")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx, celerity
+      integer(c_int32_t), value :: set, domain
+    end function
     integer(c_int) function mhmhip_set_inflow(ctx, nsteps, q) bind(C, name="mhmhip_set_inflow")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx, q
@@ -173,6 +190,18 @@ module mo_mhmhip
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx, qTIN, qTR, C1, C2
       integer(c_int32_t), value :: set
+    end function
+    integer(c_int) function mhmhip_output_accumulate(ctx, flags) &
+        bind(C, name="mhmhip_output_accumulate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, flags
+    end function
+    integer(c_int) function mhmhip_get_output_sum(ctx, k, horizon, set, sum, count) &
+        bind(C, name="mhmhip_get_output_sum")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx, sum
+      integer(c_int32_t), value :: k, horizon, set
+      integer(c_int64_t), intent(out) :: count
     end function
     integer(c_int) function mhmhip_mrm_step(ctx, st, gauge_out) bind(C, name="mhmhip_mrm_step")
       import :: c_int, c_ptr, mhmhip_step
@@ -264,21 +293,35 @@ contains
   !> The hot part of mhm_interface_run_do_time_step: meteo getters + mHM() +
   !> mRM_routing() for one step, states resident on the device.  gauge(:)
   !> receives GaugeDischarge (mRM_runoff(tt,:)) of parameter set 0.
+  !> With routing steps longer than the model step, a step that only
+  !> accumulates runoff leaves gauge(:) alone and routed is .false.; the
+  !> reference then copies a routing step's discharge to the period's earlier
+  !> steps (src/mHM/mo_mhm_interface_run.f90:614-618), the caller's part.
   subroutine mhm_hip_do_time_step(ctx, tt, month, hour, doy, iMeteoTS, read_restart, routing, &
-                                  gauge)
+                                  gauge, n_time_steps, routed)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: tt, month, hour, doy, iMeteoTS
     logical, intent(in) :: read_restart, routing
     real(c_double), intent(inout), target, contiguous :: gauge(:)
+    integer, intent(in), optional :: n_time_steps
+    logical, intent(out), optional :: routed
     type(mhmhip_step) :: st
+    integer(c_int) :: status
     st%tt = tt
     st%month = month
     st%hour = hour
     st%doy = doy
     st%iMeteoTS = iMeteoTS
     st%read_states = merge(1, 0, read_restart)
+    st%n_steps = 0
+    if (present(n_time_steps)) st%n_steps = n_time_steps
+    if (present(routed)) routed = .false.
     call mhmhip_check(mhmhip_mhm_step(ctx, st), ctx, "mHM")
-    if (routing) call mhmhip_check(mhmhip_mrm_step(ctx, st, c_loc(gauge(1))), ctx, "mRM_routing")
+    if (.not. routing) return
+    status = mhmhip_mrm_step(ctx, st, c_loc(gauge(1)))
+    if (status == MHMHIP_PENDING) return
+    call mhmhip_check(status, ctx, "mRM_routing")
+    if (present(routed)) routed = .true.
   end subroutine
 
 end module mo_mhmhip

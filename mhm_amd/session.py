@@ -119,8 +119,16 @@ def do_time_step():
     S.model.mhm_step(st)
     if S.setup.route:
         g = S.model.mrm_step(st)
-        S.runoff[S.tt, :] = g[0]
-        S.last_gauges = g[0]
+        if g is None:
+            # a routing step longer than the model step: runoff accumulates
+            S.pending = getattr(S, "pending", 0) + 1
+        else:
+            # the routed discharge also stands for the period's earlier steps
+            # (mo_mhm_interface_run.f90:614-618)
+            k = getattr(S, "pending", 0)
+            S.runoff[S.tt - k:S.tt + 1, :] = g[0]
+            S.pending = 0
+            S.last_gauges = g[0]
     S.tt += 1
 
 

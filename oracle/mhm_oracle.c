@@ -865,17 +865,28 @@ static int or_locate(const double *x, int n, double y) {
 void or_mrm_update_param_celerity(int64_t n_links, const double *length, double celerity,
                                   int32_t timestep, double *C1, double *C2, double *ts_rout,
                                   double *ts_rout_factor, int32_t *rout_loop) {
+  or_mrm_update_param_links(n_links, length, &celerity, 0, timestep, C1, C2, ts_rout,
+                            ts_rout_factor, rout_loop);
+}
+
+/* mrm_update_param (src/mRM/mo_mrm_mpr.f90:241-329) for processCase(8) = 3:
+ * K = L11_length / L11_celerity per link (302-311), the rest as for a
+ * constant celerity.  stride 0: one celerity for every link (processCase
+ * 2), 1: celerity[i] for link i. */
+void or_mrm_update_param_links(int64_t n_links, const double *length, const double *celerity,
+                               int32_t stride, int32_t timestep, double *C1, double *C2,
+                               double *ts_rout, double *ts_rout_factor, int32_t *rout_loop) {
   const double xi = fabs(0.0); /* rout_space_weight */
   double kmin = 0.0;
   for (int64_t i = 0; i < n_links; ++i) {
-    const double K = length[i] / celerity;
+    const double K = length[i] / celerity[i * stride];
     if (i == 0 || K < kmin) kmin = K;
   }
   int ind = or_locate(kGivenTS, 19, kmin);
   if (ind < 1) ind = 1;
   const double ts = kGivenTS[ind - 1];
   for (int64_t i = 0; i < n_links; ++i) {
-    const double K = length[i] / celerity;
+    const double K = length[i] / celerity[i * stride];
     C1[i] = ts / (K * (1.0 - xi) + 0.5 * ts);
     C2[i] = 1.0 - C1[i] * K / ts;
   }

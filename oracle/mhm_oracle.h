@@ -178,6 +178,11 @@ typedef struct {
 void or_mrm_update_param_celerity(int64_t n_links, const double *length, double celerity,
                                   int32_t timestep, double *C1, double *C2, double *ts_rout,
                                   double *ts_rout_factor, int32_t *rout_loop);
+/* mrm_update_param (src/mRM/mo_mrm_mpr.f90:241-329), processCase(8) = 3:
+ * per-link celerity (stride 1) or one celerity for all links (stride 0). */
+void or_mrm_update_param_links(int64_t n_links, const double *length, const double *celerity,
+                               int32_t stride, int32_t timestep, double *C1, double *C2,
+                               double *ts_rout, double *ts_rout_factor, int32_t *rout_loop);
 
 /* One mRM_routing() call.  Reference: mo_mrm_routing.f90:104-303.
  * n_l1 cells of runoff; gauge_out receives n_gauges values (may be NULL). */

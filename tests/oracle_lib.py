@@ -103,6 +103,8 @@ def lib():
             "or_routing_order": (v, [C.c_int64, C.c_int64, _ip, _ip, _ip, _ip]),
             "or_mrm_update_param_celerity": (v, [C.c_int64, _dp, d, C.c_int32, _dp, _dp, _dp,
                                                  _dp, _ip]),
+            "or_mrm_update_param_links": (v, [C.c_int64, _dp, _dp, C.c_int32, C.c_int32, _dp,
+                                              _dp, _dp, _dp, _ip]),
             "or_meteo_l2_to_l1": (i, [C.c_int32, C.c_int32, d, C.c_void_p, C.c_int32,
                                       C.c_int32, d, C.c_void_p, C.c_int32, _dp, _dp]),
         }
@@ -205,4 +207,17 @@ def mrm_update_param_celerity(length, celerity, timestep):
     lib().or_mrm_update_param_celerity(n, dptr(length), float(celerity), int(timestep),
                                        dptr(c1), dptr(c2), C.byref(ts), C.byref(f),
                                        C.byref(loops))
+    return c1[:n], c2[:n], ts.value, f.value, loops.value
+
+
+def mrm_update_param_links(length, celerity, timestep):
+    """processCase(8) = 3: (C1, C2, ts_rout, factor, rout_loop) of a per-link celerity."""
+    length = np.ascontiguousarray(length, dtype=np.float64)
+    cel = np.ascontiguousarray(celerity, dtype=np.float64)
+    n = length.shape[0]
+    c1, c2 = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    ts, f = ref(), ref()
+    loops = C.c_int32()
+    lib().or_mrm_update_param_links(n, dptr(length), dptr(cel), 1, int(timestep), dptr(c1),
+                                    dptr(c2), C.byref(ts), C.byref(f), C.byref(loops))
     return c1[:n], c2[:n], ts.value, f.value, loops.value

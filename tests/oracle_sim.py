@@ -141,6 +141,21 @@ class OracleSim:
                 o.C2[:d.n_links] = c2
                 o.adaptive = (ts, f, loops)
 
+    def set_routing_celerity_links(self, celerity, set_=0, domain=-1):
+        """processCase(8) = 3: C1/C2 and the routing step from a celerity per link
+        (celerity over the domain's links, or all packed links for domain < 0)."""
+        off = 0
+        for di, d in enumerate(self.domains):
+            nl = d.n_links
+            if domain < 0 or di == domain:
+                cel = celerity[off:off + nl] if domain < 0 else celerity
+                o = self.ds[di][set_]
+                c1, c2, ts, f, loops = O.mrm_update_param_links(d.length[:nl], cel, self.timestep)
+                o.C1[:nl] = c1
+                o.C2[:nl] = c2
+                o.adaptive = (ts, f, loops)
+            off += nl
+
     # -- stepping ----------------------------------------------------------------------
     def step(self, st):
         """One do_time_step for every domain and set; returns gauges [set][gauge]."""
@@ -162,6 +177,7 @@ class OracleSim:
                 rc.process_case = 1 if ad is None else 2
                 rc.rout_loop = 1 if ad is None else ad[2]
                 rc.read_states = int(st["read_states"])
+                long_step = ad is not None and ad[1] > 1.0
                 rc.map_flag = 1
                 rc.n_inflow = 0
                 rc.n_gauges = d.gauge_nodes.shape[0]
@@ -178,6 +194,33 @@ class OracleSim:
                     iq = np.ascontiguousarray(inf["q"][int(st["tt"]) - 1 - inf["tt0"]])
                 else:
                     iidx = ihw = inode = iq = None
+                if long_step:
+                    # routing steps longer than the model step
+                    # (mo_mhm_interface_run.f90:494-512, 608-618): RunToRout and
+                    # InflowDischarge add up, routing every nint(tsRoutFactorIn)
+                    # steps of tt and at tt == nTimeSteps
+                    tt = int(st["tt"])
+                    last = int(st["n_steps"]) > 0 and tt == int(st["n_steps"])
+                    fin = ad[1]
+                    nn = int(np.floor(fin + 0.5))
+                    if last and tt % nn != 0:
+                        fin = float(tt % nn)
+                    if getattr(o, "run_to_rout", None) is None:
+                        o.run_to_rout = np.zeros(n)
+                        o.inflow_sum = None if iq is None else np.zeros(iq.shape[0])
+                    o.run_to_rout = o.run_to_rout + tot
+                    if iq is not None:
+                        o.inflow_sum = o.inflow_sum + iq
+                    if not (tt % int(np.floor(fin + 0.5)) == 0 or last):
+                        gauges[s].extend([np.nan] * rc.n_gauges)
+                        continue
+                    tot = np.ascontiguousarray(o.run_to_rout)
+                    if iq is not None:
+                        iq = np.ascontiguousarray(o.inflow_sum / fin)
+                        o.inflow_sum = np.zeros(iq.shape[0])
+                    o.run_to_rout = np.zeros(n)
+                    rc.timestep = self.timestep * int(np.floor(fin + 0.5))
+                    rc.rout_loop = 1
                 O.lib().or_mrm_routing(C.byref(rc), O.dptr(o.gamma), n, O.dptr(tot),
                                        O.dptr(np.ascontiguousarray(d.area_km2)), O.iptr(l1),
                                        O.dptr(np.ascontiguousarray(d.area_km2)), O.iptr(l1),
@@ -245,3 +288,66 @@ def rel_close(a, b, rtol=1e-10, atol=1e-12, allow_nan=False):
         ratio = np.where(lim > 0, err / np.where(lim > 0, lim, 1), 0)
     worst = float(np.max(np.where(np.isnan(ratio), np.inf, ratio))) if a.size else 0.0
     return (not bad.any()), worst
+
+
+def output_value(sim, set_, k, h=0):
+    """One step's value of output k (outputFlxState index) of horizon h
+    (0-based), over all packed cells, from the oracle's states after the
+    step and its fluxes: mHM_updateDataset (src/mHM/mo_write_fluxes_states.f90:
+    318-436) -- the same expressions in the same operation order; horizon
+    sums run from 0 over the horizons (sum(..., dim = 2)); fNotSealed =
+    1 - fSealed (src/mHM/mo_mhm_interface_run.f90:239)."""
+    g = lambda name: sim.get(name, set_)
+    fs = np.concatenate([o[set_].P[P_ROW["FSEALED"]] for o in sim.ds])
+    fns = 1.0 - fs
+    sat = np.concatenate([o[set_].P[17:17 + sim.nh].T for o in sim.ds], axis=0)
+
+    def hsum(a):
+        s = np.zeros(a.shape[0])
+        for j in range(a.shape[1]):
+            s = s + a[:, j]
+        return s
+    if k == 1:
+        return g("INTER")
+    if k == 2:
+        return g("SNOWPACK")
+    if k == 3:
+        return g("SOILMOIST")[:, h]
+    if k == 4:
+        return g("SOILMOIST")[:, h] / sat[:, h]
+    if k == 5:
+        return hsum(g("SOILMOIST")) / hsum(sat)
+    if k == 6:
+        return g("SEALSTW")
+    if k == 7:
+        return g("UNSATSTW")
+    if k == 8:
+        return g("SATSTW")
+    if k == 9:
+        return g("PET_CALC")
+    if k == 10:
+        return hsum(g("AETSOIL")) * fns + g("AETCANOPY") + g("AETSEALED") * fs
+    if k == 11:
+        return g("TOTAL_RUNOFF")
+    if k == 12:
+        return g("RUNOFFSEAL") * fs
+    if k == 13:
+        return g("FASTRUNOFF") * fns
+    if k == 14:
+        return g("SLOWRUNOFF") * fns
+    if k == 15:
+        return g("BASEFLOW") * fns
+    if k == 16:
+        return g("PERCOL") * fns
+    if k == 17:
+        return g("INFILSOIL")[:, h] * fns
+    if k == 19:
+        return g("AETSOIL")[:, h] * fns
+    if k == 20:
+        return g("PREEFFECT")
+    if k == 21:
+        return g("MELT")
+    raise KeyError(k)
+
+
+OUTPUT_PER_HORIZON = (3, 4, 17, 19)

@@ -10,9 +10,12 @@ START = dt.datetime(1990, 1, 1)
 
 
 def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, route=True,
-              forcing_seed=100, pet_forcing=False, start=START, inflow=None):
+              forcing_seed=100, pet_forcing=False, start=START, inflow=None,
+              forcing_columns=None):
     """HIP model and oracle on the same inputs.  inflow: {"domain", "node" (1-based),
-    "headwater", "q" (n_steps, n_inflow)} inflow gauges, given to both."""
+    "headwater", "q" (n_steps, n_inflow)} inflow gauges, given to both.
+    forcing_columns: the model gets that many forcing columns and no forcing
+    (the caller streams it); the oracle gets all of it."""
     n_days = calendar.n_forcing_days_needed(n_steps)
     doy0 = start.timetuple().tm_yday
     for i, d in enumerate(domains):
@@ -22,7 +25,8 @@ def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, 
             rng = np.random.default_rng(forcing_seed + 50 + i)
             d.forcing["PET"] = rng.gamma(2.0, 1.2, size=(d.n_days, d.n_cells))
     model = engine.Model(domains, n_sets=n_sets, timestep=timestep, process_soil=soil,
-                         process_pet=pet, route=route, n_forcing_days=n_days)
+                         process_pet=pet, route=route,
+                         n_forcing_days=forcing_columns or n_days)
     sim = OracleSim(domains, n_sets=n_sets, timestep=timestep, process_soil=soil, process_pet=pet,
                     n_forcing_days=n_days, route=route)
     if route:
@@ -49,7 +53,8 @@ def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, 
                 sim.set_routing_params(g5, s, i)
     model.default_states()
     sim.default_states()
-    model.set_domain_forcing()
+    if forcing_columns is None:
+        model.set_domain_forcing()
     if pet_forcing:
         model.set_forcing("PET", np.concatenate([d.forcing["PET"] for d in domains], axis=1))
     sim.set_domain_forcing()

@@ -362,3 +362,49 @@ def test_restart_coefficients_follow_read_states():
     assert compare_routing(model, sim, RTOL, ATOL) == []
     assert compare_cells(model, sim, RTOL, ATOL) == []
     model.close()
+
+
+@pytest.mark.parametrize("how", ["fused", "graph", "plain", "per_step"])
+def test_output_sums_match_oracle(how):
+    """Output aggregation of the L1 states and fluxes on the device
+    (mHM_updateDataset + updateVariable, mo_write_fluxes_states.f90:283-438,
+    mo_nc_output.f90:140-148): every output but the neutrons, summed per cell
+    over 30 steps in step order, against the oracle's per-step values of the
+    same expressions summed the same way; the count is the steps added and a
+    restart zeroes both.  Two basins packed, two parameter sets."""
+    from oracle_sim import OUTPUT_PER_HORIZON, output_value
+    _log(f"case: output sums {how}")
+    doms = [synth.make_domain(12, 15, seed=41), synth.make_domain(8, 9, seed=42)]
+    model, sim, steps, recs = make_pair(doms, n_sets=2, n_steps=30, seed=43)
+    flags = [1] * 21
+    flags[17] = 0  # neutrons
+    model.output_accumulate(flags)
+    if how == "per_step":
+        for k in range(30):
+            model.mhm_step(steps[k])
+            model.mrm_step(steps[k])
+    else:
+        kw = {"fused": dict(mode=1, steps_per_launch=8), "plain": dict(mode=0),
+              "graph": dict(mode=1, steps_per_launch=8, use_graph=True)}[how]
+        model.run(steps[:12], **kw)
+        model.run(steps[12:], **kw)
+    keys = [(k, h) for k in range(1, 22) if k != 18
+            for h in (range(model.n_horizons) if k in OUTPUT_PER_HORIZON else (0,))]
+    want = {(s, k, h): np.zeros(model.n_cells) for s in range(2) for k, h in keys}
+    for r in recs:
+        sim.step(r)
+        for s in range(2):
+            for k, h in keys:
+                want[(s, k, h)] = want[(s, k, h)] + output_value(sim, s, k, h)
+    for s in range(2):
+        for k, h in keys:
+            got, n = model.output_sum(k, h + 1, s)
+            assert n == 30
+            ok, worst = rel_close(got, want[(s, k, h)], RTOL, ATOL)
+            assert ok, (s, k, h, worst)
+    model.output_accumulate(flags)
+    got, n = model.output_sum(11, 1, 0)
+    assert n == 0 and not got.any()
+    with pytest.raises(RuntimeError, match="neutrons"):
+        model.output_accumulate([1] * 21)
+    model.close()

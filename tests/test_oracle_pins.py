@@ -333,3 +333,87 @@ def test_default_states_closed_form(depths, sm):
     for h in range(nh):
         assert np.all(S[5 + h] == sm[h]), h
     assert np.all(X == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# mHM_updateDataset output expressions (oracle_sim.output_value)
+
+@pytest.fixture(scope="module")
+def stepped_sim():
+    from pairs import START
+    dom = synth.make_domain(9, 11, seed=77)
+    n = 30
+    nd = calendar.n_forcing_days_needed(n)
+    synth.make_forcing(dom, nd, 78)
+    sim = OracleSim([dom], n_sets=1, n_forcing_days=nd)
+    gam = PM.draw(np.random.default_rng(79), 1)[0]
+    p = synth.cell_params(dom, gam)
+    p["FSEALED"] = np.linspace(0.05, 0.6, dom.n_cells)  # every cell partly sealed
+    sim.set_params(p, 0)
+    sim.set_routing_params(PM.routing_gamma(gam))
+    sim.default_states()
+    sim.set_domain_forcing()
+    for r in calendar.step_records(START, n, 1):
+        sim.step(r)
+    return sim
+
+
+def _close(a, b, rtol=1e-13):
+    return rel_close(a, b, rtol, 1e-300)[0]
+
+
+def test_output_runoff_components_add_up_to_total_runoff(stepped_sim):
+    """Sealed direct runoff on the sealed share plus the fast, slow and base
+    flows on the unsealed share is L1_total_runoff (mo_runoff.f90:248-274):
+    a component on the wrong share, or a missing factor, fails."""
+    from oracle_sim import output_value as ov
+    parts = ov(stepped_sim, 0, 12) + ov(stepped_sim, 0, 13) + ov(stepped_sim, 0, 14) + \
+        ov(stepped_sim, 0, 15)
+    tot = ov(stepped_sim, 0, 11)
+    assert tot.min() > 0.0
+    assert _close(parts, tot)
+
+
+def test_output_evapotranspiration_is_the_sum_of_its_parts(stepped_sim):
+    """Output 10 is the horizons' soil aET (output 19) plus canopy and sealed
+    evaporation, and with fSealed = 1 it would be canopy + sealed alone."""
+    from oracle_sim import output_value as ov
+    sim = stepped_sim
+    nh = sim.nh
+    soil = sum(ov(sim, 0, 19, h) for h in range(nh))
+    fs = sim.ds[0][0].P[0]
+    want = soil + sim.get("AETCANOPY") + sim.get("AETSEALED") * fs
+    assert _close(ov(sim, 0, 10), want)
+    # the soil part goes on the unsealed share only: on the sealed share it differs
+    wrong = sum(sim.get("AETSOIL")[:, h] * fs for h in range(nh)) + sim.get("AETCANOPY") + \
+        sim.get("AETSEALED") * fs
+    assert not _close(ov(sim, 0, 10), wrong, 1e-6)
+
+
+def test_output_soil_moisture_fractions(stepped_sim):
+    """Output 4 per horizon times the saturation is the moisture (output 3);
+    output 5 is the column's total moisture over its total saturation, which
+    lies between the smallest and largest horizon fraction."""
+    from oracle_sim import output_value as ov
+    sim = stepped_sim
+    nh = sim.nh
+    sat = sim.ds[0][0].P[17:17 + nh]
+    fr = np.stack([ov(sim, 0, 4, h) for h in range(nh)])
+    sm = np.stack([ov(sim, 0, 3, h) for h in range(nh)])
+    assert _close(fr * sat, sm, 1e-15)
+    tot = ov(sim, 0, 5)
+    assert (tot >= fr.min(axis=0) * (1 - 1e-15)).all() and (tot <= fr.max(axis=0) * (1 + 1e-15)).all()
+    assert _close(tot, sm.sum(axis=0) / sat.sum(axis=0))
+
+
+def test_output_states_and_fluxes_pass_through(stepped_sim):
+    from oracle_sim import output_value as ov
+    sim = stepped_sim
+    for k, name in ((1, "INTER"), (2, "SNOWPACK"), (6, "SEALSTW"), (7, "UNSATSTW"),
+                    (8, "SATSTW"), (9, "PET_CALC"), (11, "TOTAL_RUNOFF"), (20, "PREEFFECT"),
+                    (21, "MELT")):
+        assert np.array_equal(ov(sim, 0, k), sim.get(name)), k
+    fs = sim.ds[0][0].P[0]
+    assert _close(ov(sim, 0, 16) / (1.0 - fs), sim.get("PERCOL"))
+    for h in range(sim.nh):
+        assert _close(ov(sim, 0, 17, h) / (1.0 - fs), sim.get("INFILSOIL")[:, h])

@@ -137,23 +137,107 @@ def test_adaptive_routing_limits():
     from mhm_amd import _native as N
     print("case: adaptive routing limits", flush=True)
     model, sim, steps, recs = _pair(n_steps=4)
-    # one sub-step (TSrout = the model step): the time loop takes it
-    model.set_routing_celerity(0.25, 0)
-    model.set_routing_celerity(0.25, 1)
-    model.run(steps[:2], mode=N.RUN_FUSED, steps_per_launch=2)
-    # sub-steps: per step only
+    # sets that disagree on the routing step
     model.set_routing_celerity(1.0, 0)
-    model.set_routing_celerity(1.0, 1)
-    with pytest.raises(N.MhmHipError):
-        model.run(steps[2:4], mode=N.RUN_FUSED, steps_per_launch=2)
-    # sets that disagree on the sub-steps
     model.set_routing_celerity(0.5, 1)
-    model.mhm_step(steps[2])
-    with pytest.raises(N.MhmHipError):
-        model.mrm_step(steps[2])
-    # routing steps longer than the model step
+    model.mhm_step(steps[0])
+    with pytest.raises(N.MhmHipError, match="differ"):
+        model.mrm_step(steps[0])
+    # routing steps longer than the model step: per step only
     model.set_routing_celerity(0.1, 0)
     model.set_routing_celerity(0.1, 1)
-    with pytest.raises(N.MhmHipError):
-        model.mrm_step(steps[2])
+    model.mrm_step(steps[0])
+    with pytest.raises(N.MhmHipError, match="longer"):
+        model.run(steps[1:3], mode=N.RUN_FUSED, steps_per_launch=2)
+    model.close()
+
+
+def test_per_link_celerity_equals_constant_when_uniform():
+    """processCase 3 with the same celerity on every link gives processCase
+    2's coefficients and routing step bit for bit; per link, C1/C2 are Chow's
+    x = 0 coefficients of that link's K = length / celerity."""
+    rng = np.random.default_rng(4)
+    length = rng.uniform(800.0, 1500.0, 40)
+    a = O.mrm_update_param_celerity(length, 0.8, 1)
+    b = O.mrm_update_param_links(length, np.full(40, 0.8), 1)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    cel = rng.uniform(0.3, 3.0, 40)
+    c1, c2, ts, _, _ = O.mrm_update_param_links(length, cel, 1)
+    K = length / cel
+    assert ts == max(t for t in GIVEN_TS if t <= K.min())
+    assert np.allclose(c2, ts / (2 * K + ts), rtol=1e-14, atol=0)
+    assert np.allclose(1.0 - c1, (2 * K - ts) / (2 * K + ts), rtol=1e-13, atol=1e-16)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("route_mode,graph,mode", [(3, False, 1), (3, True, 1), (1, False, 1),
+                                                   (2, False, 0)])
+def test_time_loop_adaptive_routing_matches_oracle(route_mode, graph, mode):
+    """Routing sub-steps inside the time loop (rout_loop = 4): every model
+    step's qOUT routed four times, gauges and Qmod the average of the four,
+    against the oracle -- with a per-link celerity (processCase 3)."""
+    from oracle_sim import rel_close
+    from pairs import compare_cells, compare_routing
+    print(f"case: adaptive routing in the time loop route_mode={route_mode} graph={graph}",
+          flush=True)
+    model, sim, steps, recs = _pair(n_steps=30)
+    dom = sim.domains[0]
+    rng = np.random.default_rng(9)
+    for s in range(2):
+        cel = rng.uniform(0.8, 1.0, dom.n_links)  # K >= 1000 s on every link
+        cel[0] = dom.length[0] / 950.0             # one link with K = 950 s: TSrout 900 s
+        model.set_routing_celerity_links(cel, s)
+        sim.set_routing_celerity_links(cel, s)
+    assert sim.ds[0][0].adaptive[2] == sim.ds[0][1].adaptive[2] == 4
+    g = np.concatenate([model.run(steps[:12], mode=mode, steps_per_launch=5,
+                                  route_mode=route_mode, use_graph=graph),
+                        model.run(steps[12:], mode=mode, steps_per_launch=5,
+                                  route_mode=route_mode, use_graph=graph)])
+    go = np.stack([sim.step(r) for r in recs])
+    ok, worst = rel_close(g, go, 1e-10, 1e-12)
+    assert ok, worst
+    assert compare_cells(model, sim, 1e-10, 1e-12) == []
+    assert compare_routing(model, sim, 1e-10, 1e-12) == []
+    model.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("celerity,every,n_steps", [(0.1, 2, 24), (0.05, 4, 25)])
+def test_long_routing_steps_match_oracle(celerity, every, n_steps):
+    """Routing steps longer than the model step (tsRoutFactor = 2 and 4): the
+    model steps accumulate RunToRout and the inflow (one inflow gauge), a
+    routing step converts the sum with TS_rout and routes once; with 25
+    steps and a factor of 4 the last period has one step (tt == nTimeSteps).
+    Routed steps against the oracle, accumulating steps report nothing."""
+    from mhm_amd import synth
+    from oracle_sim import rel_close
+    from pairs import compare_cells, compare_routing, make_pair
+    print(f"case: long routing steps celerity={celerity}", flush=True)
+    dom = synth.make_domain(12, 10, seed=52)
+    indeg = np.bincount(dom.toN - 1, minlength=dom.n_cells)
+    node = int(np.argsort(-indeg, kind="stable")[2]) + 1
+    q = 1.5 + np.random.default_rng(3).gamma(2.0, 1.0, size=(n_steps, 1))
+    inflow = {"domain": [0], "node": [node], "headwater": [1], "q": q}
+    model, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=n_steps, seed=14, inflow=inflow)
+    for s in range(2):
+        model.set_routing_celerity(celerity, s)
+        sim.set_routing_celerity(celerity, s)
+    assert sim.ds[0][0].adaptive[1] == float(every)
+    routed = 0
+    for k in range(n_steps):
+        model.set_inflow(q[k:k + 1])
+        model.mhm_step(steps[k])
+        g = model.mrm_step(steps[k])
+        go = sim.step(recs[k])
+        due = (k + 1) % every == 0 or k + 1 == n_steps
+        assert (g is not None) == due, k
+        assert np.isnan(go).all() != due, k
+        if due:
+            routed += 1
+            ok, worst = rel_close(g, go, 1e-10, 1e-12)
+            assert ok, (k, worst)
+    assert routed == -(-n_steps // every)
+    assert compare_cells(model, sim, 1e-10, 1e-12) == []
+    assert compare_routing(model, sim, 1e-10, 1e-12) == []
     model.close()
