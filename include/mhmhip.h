@@ -378,6 +378,21 @@ int mhmhip_mrm_step(mhmhip_ctx *ctx, const mhmhip_step *st, double *gauge_out);
 int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
                const mhmhip_run_opts *opts, double *gauge_out);
 
+/* mhmhip_run without device memory allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+ * steps, options (use_graph must be 0), results and states as mhmhip_run;
+ * the call's step table, gauge series and runoff ring live in a 64 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends, instead of context buffers grown on demand.  A call whose scratch
+ * (256-byte aligned step table of 96 B per step, gauge series of
+ * nsteps x sets x gauges doubles, ring of (ring rows + 16) x sets x cells
+ * rounded to 64 doubles) exceeds 64 MiB fails with MHMHIP_EINVAL, as does a
+ * call with profiling on.  Calls on one GPU share the array: two calls that
+ * overlap on one GPU (contexts on different host threads) race. */
+int mhmhip_run_noalloc(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
+                       const mhmhip_run_opts *opts, double *gauge_out);
+
 /* ---- measurement (no counterpart in the reference, which has only wall
  * timers, mo_mhm_interface.F90:157-162) -------------------------------------
  * With profiling on, kernel launches are bracketed by HIP events on the

@@ -105,6 +105,8 @@ SIGNATURES = {
     "mhmhip_get_output_sum": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp,
                                         C.POINTER(C.c_int64)]),
     "mhmhip_run": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts), _dp]),
+    "mhmhip_run_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts),
+                                     _dp]),
     "mhmhip_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "mhmhip_get_profile": (C.c_int, [C.c_void_p, C.c_int32, _dp, _lp, _dp]),
 }

@@ -33,6 +33,35 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 }  // namespace
 
+// Device scratch of mhmhip_run_noalloc: the step table, gauge series and
+// runoff ring of one call, in a file-scope array that every GPU the library
+// runs on holds (its own copy) until the process ends.  Calls on one GPU
+// share it: two such calls that overlap on one GPU race.  In host code the
+// name stands for a host-side shadow; the device address of each GPU's copy
+// comes from hipGetSymbolAddress with that GPU current (g_scratch).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+// The device address of this GPU's scratch, resolved once per device with
+// the device current (the caller has made it current).
+hipError_t scratch_resolve(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev]) return hipSuccess;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch[dev] = static_cast<unsigned char *>(p);
+  return hipSuccess;
+}
+}  // namespace
+
 struct mhmhip_ctx {
   mhmhip_config cfg{};
   int nh = 0;
@@ -311,6 +340,11 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
   }
   if (hipSetDevice(c.device) != hipSuccess) {
     g_create_error = "create: hipSetDevice failed";
+    return MHMHIP_EHIP;
+  }
+  // set-up of mhmhip_run_noalloc: this GPU's scratch address, once per device
+  if (scratch_resolve(c.device) != hipSuccess) {
+    g_create_error = "create: device scratch address (hipGetSymbolAddress) failed";
     return MHMHIP_EHIP;
   }
   mhmhip_ctx *x = new mhmhip_ctx();
@@ -2066,8 +2100,55 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
 
 namespace {
 
+// This GPU's scratch (resolved when the context was created on it).
+int scratch_base(mhmhip_ctx *x, unsigned char **out) {
+  CHECK_HIP(x, scratch_resolve(x->cfg.device), "run_noalloc: scratch address");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  *out = g_scratch[x->cfg.device];
+  return MHMHIP_OK;
+}
+
+// The context's per-call buffers (step table, gauge series, runoff ring),
+// replaced by ranges of the device scratch array for one no-allocation run
+// and given back on every return path.
+struct ScratchSwap {
+  mhmhip_ctx *x;
+  bool on = false;
+  StepDev *steps = nullptr;
+  int64_t steps_cap = 0;
+  double *gauge = nullptr;
+  int64_t gauge_rows = 0;
+  double *ring = nullptr;
+  int64_t ring_len = 0;
+  explicit ScratchSwap(mhmhip_ctx *c) : x(c) {}
+  void swap_in(StepDev *st, int64_t st_cap, double *g, int64_t g_rows, double *r, int64_t r_len) {
+    steps = x->d_steps;
+    steps_cap = x->steps_cap;
+    gauge = x->d_gauge_buf;
+    gauge_rows = x->gauge_buf_rows;
+    ring = x->d_ring;
+    ring_len = x->ring_len;
+    x->d_steps = st;
+    x->steps_cap = st_cap;
+    x->d_gauge_buf = g;
+    x->gauge_buf_rows = g_rows;
+    x->d_ring = r;
+    x->ring_len = r_len;
+    on = true;
+  }
+  ~ScratchSwap() {
+    if (!on) return;
+    x->d_steps = steps;
+    x->steps_cap = steps_cap;
+    x->d_gauge_buf = gauge;
+    x->gauge_buf_rows = gauge_rows;
+    x->d_ring = ring;
+    x->ring_len = ring_len;
+  }
+};
+
 int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
-             const mhmhip_run_opts *opts, double *gauge_out) {
+             const mhmhip_run_opts *opts, double *gauge_out, bool noalloc) {
   if (!x) return MHMHIP_EINVAL;
   if (!steps || nsteps < 1) return x->fail(MHMHIP_EINVAL, "run: no steps");
   mhmhip_run_opts o{};
@@ -2122,6 +2203,41 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
   // tiled routing: windows of B steps (the plain path routes in windows too)
   const int32_t B = tile_window(o.route_window);
   const int64_t ring_rows = route ? ring_need(x, kind, S * rep, B, nr) : 0;
+  ScratchSwap swap(x);
+  if (noalloc) {
+    // graph capture and instantiation, the routing stream and the timing
+    // events would all allocate
+    if (o.use_graph) return x->fail(MHMHIP_EINVAL, "run_noalloc: use_graph is not available");
+    if (rep > 1 && !x->d_qsub)
+      return x->fail(MHMHIP_EINVAL, "run_noalloc: routing sub-steps keep their Qmod sums in a "
+                                    "context buffer that a first mhmhip_run allocates");
+    if (x->prof) return x->fail(MHMHIP_EINVAL, "run_noalloc: turn profiling off first");
+    unsigned char *base = nullptr;
+    CHECK_RC(scratch_base(x, &base));
+    size_t used = 0;
+    auto carve = [&](size_t bytes) {
+      const size_t at = used;
+      used = (size_t)round_up((int64_t)(used + bytes), 256);
+      return base + at;
+    };
+    const int64_t gper = (int64_t)x->n_sets * std::max(1, x->n_gauges);
+    const size_t need = (size_t)round_up((int64_t)sizeof(StepDev) * nsteps, 256) +
+                        (route ? (size_t)round_up(8 * nr * gper, 256) +
+                                     (size_t)(8 * (ring_rows + kTileHeight) * x->ld)
+                               : 0);
+    if (need > kScratchBytes)
+      return x->fail(MHMHIP_EINVAL,
+                     "run_noalloc: the call needs " + std::to_string(need) + " bytes of scratch "
+                     "(step table, gauge series, runoff ring), more than the " +
+                     std::to_string(kScratchBytes) + " it has (fewer steps, cells, sets or a "
+                     "smaller route_window, or mhmhip_run)");
+    StepDev *st = reinterpret_cast<StepDev *>(carve(sizeof(StepDev) * nsteps));
+    double *gb = route ? reinterpret_cast<double *>(carve(8 * (size_t)nr * gper)) : nullptr;
+    double *rg = route ? reinterpret_cast<double *>(carve(8 * (ring_rows + kTileHeight) * x->ld))
+                       : nullptr;
+    swap.swap_in(st, nsteps, route ? gb : x->d_gauge_buf, route ? nr : x->gauge_buf_rows,
+                 route ? rg : x->d_ring, route ? ring_rows + kTileHeight : x->ring_len);
+  }
   CHECK_RC(ensure_steps(x, nsteps));
   CHECK_HIP(x, hipMemcpyAsync(x->d_steps, hs.data(), sizeof(StepDev) * nsteps,
                               hipMemcpyHostToDevice, x->stream),
@@ -2298,7 +2414,12 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
 
 extern "C" int mhmhip_run(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
                           const mhmhip_run_opts *opts, double *gauge_out) {
-  return run_impl(x, steps, nsteps, opts, gauge_out);
+  return run_impl(x, steps, nsteps, opts, gauge_out, false);
+}
+
+extern "C" int mhmhip_run_noalloc(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
+                                  const mhmhip_run_opts *opts, double *gauge_out) {
+  return run_impl(x, steps, nsteps, opts, gauge_out, true);
 }
 
 extern "C" int mhmhip_set_profiling(mhmhip_ctx *x, int32_t on) {

@@ -281,8 +281,10 @@ class Model:
         return g[:, :self.n_gauges]
 
     def run(self, steps, mode=N.RUN_FUSED, steps_per_launch=0, block=0, route_mode=0,
-            want_gauges=True, use_graph=False, route_window=0):
-        """steps: ctypes array of mhmhip_step.  Returns gauges (n_steps, n_sets, n_gauges)."""
+            want_gauges=True, use_graph=False, route_window=0, noalloc=False):
+        """steps: ctypes array of mhmhip_step.  Returns gauges (n_steps, n_sets, n_gauges).
+        noalloc: mhmhip_run_noalloc (no device allocation; per-call buffers in the
+        library's fixed device scratch)."""
         n = len(steps)
         if not isinstance(steps, C.Array):
             steps = (N.Step * n)(*steps)
@@ -294,7 +296,8 @@ class Model:
         opts.use_graph = 1 if use_graph else 0
         opts.route_window = route_window
         g = np.zeros((n, self.n_sets, max(self.n_gauges, 1))) if want_gauges else None
-        self._chk(N.lib().mhmhip_run(self._ctx, steps, n, C.byref(opts), N.dptr(g)))
+        fn = N.lib().mhmhip_run_noalloc if noalloc else N.lib().mhmhip_run
+        self._chk(fn(self._ctx, steps, n, C.byref(opts), N.dptr(g)))
         return None if g is None else g[:, :, :self.n_gauges]
 
     # -- state snapshots (restart-style round trip of the device states) ------------

@@ -67,7 +67,7 @@ module mo_mhmhip
             mhmhip_set_routing_celerity, mhmhip_set_routing_celerity_links, &
             mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, &
             mhmhip_set_routing_params, mhmhip_set_inflow, mhmhip_get_routing_state, &
-            mhmhip_set_routing_state, mhmhip_mrm_step, mhmhip_run, &
+            mhmhip_set_routing_state, mhmhip_mrm_step, mhmhip_run, mhmhip_run_noalloc, &
             mhmhip_set_profiling, &
             mhmhip_get_profile, mhmhip_plan_network, mhmhip_qmod_accumulate, mhmhip_get_qmod_sum, &
             mhmhip_output_accumulate, mhmhip_get_output_sum
@@ -210,6 +210,14 @@ module mo_mhmhip
     end function
     integer(c_int) function mhmhip_run(ctx, steps, nsteps, opts, gauge_out) &
         bind(C, name="mhmhip_run")
+      import :: c_int, c_ptr, c_int32_t, mhmhip_step, mhmhip_run_opts
+      type(c_ptr), value :: ctx, gauge_out
+      type(mhmhip_step), intent(in) :: steps(*)
+      integer(c_int32_t), value :: nsteps
+      type(mhmhip_run_opts), intent(in) :: opts
+    end function
+    integer(c_int) function mhmhip_run_noalloc(ctx, steps, nsteps, opts, gauge_out) &
+        bind(C, name="mhmhip_run_noalloc")
       import :: c_int, c_ptr, c_int32_t, mhmhip_step, mhmhip_run_opts
       type(c_ptr), value :: ctx, gauge_out
       type(mhmhip_step), intent(in) :: steps(*)
