@@ -64,6 +64,9 @@ enum mhmhip_var {
   MHMHIP_P_SOILMOIST_EXP, /* L1_soilMoistExp  (nH)                 */
   MHMHIP_P_MAXINTER,      /* L1_maxInter      (12)                 */
   MHMHIP_P_PETLAICOR,     /* L1_petLAIcorFactor (12)               */
+  MHMHIP_P_PRIETAYALPHA,  /* L1_PrieTayAlpha (12, process_pet 2)   */
+  MHMHIP_P_AERORESIST,    /* L1_aeroResist   (12, process_pet 3)   */
+  MHMHIP_P_SURFRESIST,    /* L1_surfResist   (12, process_pet 3)   */
   MHMHIP_P_COUNT,
   /* states (src/mHM/mo_global_variables.f90) -- set and get */
   MHMHIP_S_INTER = 32,    /* L1_inter                              */
@@ -97,9 +100,16 @@ enum mhmhip_var {
 };
 
 /* Daily (or hourly) forcing variables: L1_pre, L1_temp, L1_tmin, L1_tmax,
- * L1_pet of the meteo handler (src/meteo/mo_meteo_handler.f90:120-160). */
+ * L1_pet, L1_netrad [W m-2], L1_absvappress [Pa], L1_windspeed [m s-1] of
+ * the meteo handler (src/meteo/mo_meteo_handler.f90:120-160).  A context
+ * holds the ones its PET case reads (get_corrected_pet, 1058-1097): pre and
+ * tavg always; pet for process_pet -1 and 0; tmin and tmax for 1 (Hargreaves-
+ * Samani); netrad for 2 (Priestley-Taylor); netrad, absvappress and windspeed
+ * for 3 (Penman-Monteith).  The others are accepted by the setters and not
+ * stored. */
 enum mhmhip_forcing { MHMHIP_F_PRE = 0, MHMHIP_F_TAVG, MHMHIP_F_TMIN, MHMHIP_F_TMAX,
-                      MHMHIP_F_PET, MHMHIP_F_COUNT };
+                      MHMHIP_F_PET, MHMHIP_F_NETRAD, MHMHIP_F_ABSVAPPRESS,
+                      MHMHIP_F_WINDSPEED, MHMHIP_F_COUNT };
 
 typedef struct mhmhip_ctx mhmhip_ctx;
 
@@ -112,7 +122,7 @@ typedef struct {
   int32_t n_sets;         /* parameter sets run side by side on shared forcing  */
   int32_t timestep;       /* model time step [h], a divisor of 24               */
   int32_t process_soil;   /* processMatrix(3,1): 1,4 Feddes; 2,3 Jarvis         */
-  int32_t process_pet;    /* processMatrix(5,1): -1, 0 or 1                     */
+  int32_t process_pet;    /* processMatrix(5,1): -1, 0, 1, 2 or 3               */
   int32_t process_route;  /* processMatrix(8,1): 0 (off) or 1 (Muskingum)       */
   int32_t hourly_forcing; /* meteo_handler%is_hourly_forcing                    */
   int32_t n_forcing_days; /* forcing columns held on the device                 */
@@ -141,11 +151,16 @@ typedef struct {
 
 /* The L11 network of every packed domain, concatenated like the
  * reference's module arrays (src/mRM/mo_mrm_global_variables.f90:190-218):
- * per-domain slices with per-domain 1-based indices.  L1 = L11 is required
- * (L1_L11_Id(k) == k within each domain, map_flag = .true.). */
+ * per-domain slices with per-domain 1-based indices.  The routing grid is
+ * the L1 grid (L1_L11_Id(k) == k within each domain) or coarser (map_flag
+ * = .true.: each L11 node's qOUT aggregates its L1 cells' runoff,
+ * mo_mrm_pre_routing.f90:110-126); a routing grid finer than L1
+ * (map_flag = .false.) is refused, and so is mhmhip_run_noalloc on a
+ * coarser one. */
 typedef struct {
-  int64_t n_nodes;               /* total L11 nodes (= cells)                    */
-  const int64_t *domain_node_start; /* n_domains+1 offsets (NULL: one domain)    */
+  int64_t n_nodes;               /* total L11 nodes (1..n_cells)                 */
+  const int64_t *domain_node_start; /* n_domains+1 offsets (NULL: one domain, or
+                                       the cell ranges when n_nodes == n_cells)  */
   const int32_t *n_outlets;      /* per domain L11_nOutlets                      */
   const int32_t *netPerm;        /* per domain nLinks entries, 1-based, packed   */
   const int32_t *fromN;          /* like netPerm                                 */
@@ -243,6 +258,17 @@ int mhmhip_set_forcing_l2(mhmhip_ctx *ctx, int32_t var, int32_t domain, int32_t 
                           int32_t ndays, const double *data2, int32_t nrows2, int32_t ncols2,
                           double cellsize2, const uint8_t *mask2, int32_t nrows1,
                           int32_t ncols1, double cellsize1, const uint8_t *mask1);
+/* Disaggregation of the daily meteo values by given weights instead of the
+ * day/night factors (read_meteo_weights, src/meteo/mo_meteo_handler.f90:
+ * 1102-1107, 1183-1189, 1265-1270 with temporal_disagg_meteo_weights,
+ * src/meteo/mo_meteo_temporal_tools.f90:32-59): pre_w, temp_w, pet_w are the
+ * handler's L1_pre_weights, L1_temp_weights, L1_pet_weights, each
+ * X(ld, 12, 24) column-major (cell, month, hour + 1) over all cells; a step
+ * of month m and hour h gets prec = pre * w, temp = (tavg + T0) * w - T0
+ * (T0 = 273.15) and pet = pet_day * w.  All three given: on (the weights are
+ * copied to the device, 3 x 288 doubles per cell); all three NULL: off. */
+int mhmhip_set_meteo_weights(mhmhip_ctx *ctx, const double *pre_w, const double *temp_w,
+                             const double *pet_w, int64_t ld);
 
 /* ---- one step ----------------------------------------------------------------
  * get_corrected_pet + get_temp + get_prec + mHM() for every cell and set:
@@ -385,7 +411,7 @@ int mhmhip_run(mhmhip_ctx *ctx, const mhmhip_step *steps, int32_t nsteps,
  * the call's step table, gauge series and runoff ring live in a 64 MiB
  * device array that each GPU the library runs on holds until the process
  * ends, instead of context buffers grown on demand.  A call whose scratch
- * (256-byte aligned step table of 96 B per step, gauge series of
+ * (256-byte aligned step table of 104 B per step, gauge series of
  * nsteps x sets x gauges doubles, ring of (ring rows + 16) x sets x cells
  * rounded to 64 doubles) exceeds 64 MiB fails with MHMHIP_EINVAL, as does a
  * call with profiling on.  Calls on one GPU share the array: two calls that

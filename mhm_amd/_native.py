@@ -20,7 +20,8 @@ MAX_HORIZONS = 6
 P_NAMES = ["FSEALED", "DEGDAY_INC", "DEGDAY_MAX", "DEGDAY_NOPRE", "TEMP_THRESH", "SEALED_THRESH",
            "JARVIS_C1", "KFASTFLOW", "KSLOWFLOW", "KBASEFLOW", "KPERCO", "ALPHA", "KARSTLOSS",
            "UNSAT_THRESH", "FASP", "HARSAMCOEFF", "LATITUDE", "SOILMOIST_SAT", "FROOTS",
-           "SOILMOIST_FC", "WILTINGPOINT", "SOILMOIST_EXP", "MAXINTER", "PETLAICOR"]
+           "SOILMOIST_FC", "WILTINGPOINT", "SOILMOIST_EXP", "MAXINTER", "PETLAICOR",
+           "PRIETAYALPHA", "AERORESIST", "SURFRESIST"]
 P = {n: i for i, n in enumerate(P_NAMES)}
 S_NAMES = ["INTER", "SNOWPACK", "SEALSTW", "UNSATSTW", "SATSTW", "SOILMOIST"]
 S = {n: 32 + i for i, n in enumerate(S_NAMES)}
@@ -30,8 +31,8 @@ X_NAMES = ["PET_CALC", "TEMP_CALC", "PREC_CALC", "THROUGHFALL", "AETCANOPY", "DE
 X = {n: 64 + i for i, n in enumerate(X_NAMES)}
 HORIZON_VARS = {"SOILMOIST_SAT", "FROOTS", "SOILMOIST_FC", "WILTINGPOINT", "SOILMOIST_EXP",
                 "SOILMOIST", "INFILSOIL", "AETSOIL"}
-MONTHLY_VARS = {"MAXINTER", "PETLAICOR"}
-F_NAMES = ["PRE", "TAVG", "TMIN", "TMAX", "PET"]
+MONTHLY_VARS = {"MAXINTER", "PETLAICOR", "PRIETAYALPHA", "AERORESIST", "SURFRESIST"}
+F_NAMES = ["PRE", "TAVG", "TMIN", "TMAX", "PET", "NETRAD", "ABSVAPPRESS", "WINDSPEED"]
 F = {n: i for i, n in enumerate(F_NAMES)}
 
 RUN_PLAIN, RUN_FUSED = 0, 1
@@ -104,6 +105,7 @@ SIGNATURES = {
     "mhmhip_output_accumulate": (C.c_int, [C.c_void_p, _ip]),
     "mhmhip_get_output_sum": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp,
                                         C.POINTER(C.c_int64)]),
+    "mhmhip_set_meteo_weights": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int64]),
     "mhmhip_run": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts), _dp]),
     "mhmhip_run_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts),
                                      _dp]),

@@ -77,7 +77,24 @@ struct mhmhip_ctx {
 
   double *P = nullptr, *S = nullptr, *X = nullptr, *F = nullptr;
   int64_t n_prows = 0, n_srows = 0, n_xrows = 0;
+  // forcing rows: frow[v] is variable v's row of F (-1: the PET case does
+  // not read it), n_frows rows in all
+  int32_t frow[MHMHIP_F_COUNT] = {};
+  int32_t n_frows = 0;
+  double *W = nullptr;  // meteo weights [3][12 * 24][sstride], null: off
   std::vector<int64_t> pos;  // reference cell -> device index (identity until a network is bound)
+  // L11 routing nodes: their count, device index of every reference node and
+  // per-domain ranges (the cells' until a network is bound).  On a routing
+  // grid coarser than L1 (coarse) the cells are grouped by node on the device
+  // and their runoff is aggregated per node (d_agg_start: the device cell
+  // range of each device node; d_cring: L1_total_runoff of a launch's steps).
+  int64_t n_nodes = 0;
+  std::vector<int64_t> npos;
+  std::vector<int64_t> dom_node_start;
+  bool coarse = false;
+  int32_t *d_agg_start = nullptr;
+  double *d_cring = nullptr;
+  int64_t cring_rows = 0;
   // Nodes of single-node domains: the reference skips L11_routing there
   // (mo_mrm_routing.f90:285-289) and leaves their qTIN as set (0 at
   // init); the device's qTIN holds Qmod for them, so their qTIN per set is
@@ -190,13 +207,13 @@ struct mhmhip_ctx {
     int32_t nsteps = -1, mode = -1, S = -1, block = -1, wg = -1, prof = -1, B = -1, out_n = -1,
             rep = -1;
     const void *ring = nullptr, *gauge = nullptr, *steps = nullptr, *inflow = nullptr,
-               *qacc = nullptr, *out = nullptr;
+               *qacc = nullptr, *out = nullptr, *cring = nullptr;
     int64_t ring_len = 0;
     bool operator==(const GraphKey &o) const {
       return nsteps == o.nsteps && mode == o.mode && S == o.S && block == o.block && wg == o.wg &&
              prof == o.prof && B == o.B && ring == o.ring && gauge == o.gauge && steps == o.steps &&
              inflow == o.inflow && qacc == o.qacc && ring_len == o.ring_len && out == o.out &&
-             out_n == o.out_n && rep == o.rep;
+             out_n == o.out_n && rep == o.rep && cring == o.cring;
     }
   };
   GraphKey gkey;
@@ -290,7 +307,7 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
   if (c.n_cells < 1 || c.n_cells > ((int64_t)1 << 31) - 64 || c.n_horizons < 1 ||
       c.n_horizons > MHMHIP_MAX_HORIZONS || c.n_sets < 1 || c.n_sets > 65535 ||
       c.timestep < 1 || c.timestep > 24 || 24 % c.timestep != 0 || c.process_soil < 1 ||
-      c.process_soil > 4 || c.process_pet < -1 || c.process_pet > 1 || c.process_route < 0 ||
+      c.process_soil > 4 || c.process_pet < -1 || c.process_pet > 3 || c.process_route < 0 ||
       c.process_route > 1 || c.n_forcing_days < 1 || c.n_domains < 1) {
     g_create_error = "create: configuration out of range";
     return MHMHIP_EINVAL;
@@ -371,6 +388,9 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
   }
   x->pos.resize(c.n_cells);
   for (int64_t i = 0; i < c.n_cells; ++i) x->pos[i] = i;
+  x->n_nodes = c.n_cells;
+  x->npos = x->pos;
+  x->dom_node_start = x->dom_cell_start;
   x->col_day.resize(c.n_forcing_days);
   for (int32_t k = 0; k < c.n_forcing_days; ++k) x->col_day[k] = k + 1;
   x->col_seq.assign(c.n_forcing_days, -1);
@@ -381,12 +401,16 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
     return MHMHIP_EHIP;
   }
   x->n_prows = pr_count(x->nh);
+  // the forcing variables get_corrected_pet / get_temp / get_prec read for
+  // this PET case (mo_meteo_handler.f90:1058-1097), one row of F each
+  for (int v = 0; v < MHMHIP_F_COUNT; ++v) x->frow[v] = forcing_row(v, c.process_pet);
+  x->n_frows = forcing_rows(c.process_pet);
   x->n_srows = sr_count(x->nh);
   x->n_xrows = xr_count(x->nh);
   if ((rc = x->dalloc(&x->P, x->n_prows * x->ld, "alloc params")) ||
       (rc = x->dalloc(&x->S, x->n_srows * x->ld, "alloc states")) ||
       (rc = x->dalloc(&x->X, x->n_xrows * x->ld, "alloc fluxes")) ||
-      (rc = x->dalloc(&x->F, (int64_t)MHMHIP_F_COUNT * c.n_forcing_days * x->sstride,
+      (rc = x->dalloc(&x->F, (int64_t)x->n_frows * c.n_forcing_days * x->sstride,
                       "alloc forcing"))) {
     g_create_error = x->err;
     mhmhip_destroy(x);
@@ -415,6 +439,9 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->S);
   (void)hipFree(x->X);
   (void)hipFree(x->F);
+  (void)hipFree(x->W);
+  (void)hipFree(x->d_agg_start);
+  (void)hipFree(x->d_cring);
   (void)hipFree(x->d_meta);
   (void)hipFree(x->d_area);
   (void)hipFree(x->d_C1);
@@ -496,6 +523,17 @@ int put_row(mhmhip_ctx *x, double *dst, const double *src, Fn f) {
   return MHMHIP_OK;
 }
 
+// one row of node values in reference node order -> device node order
+int put_node_row(mhmhip_ctx *x, double *dst, const double *src) {
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "stage wait");
+  std::fill(x->stage, x->stage + x->sstride, 0.0);
+  for (int64_t k = 0; k < x->n_nodes; ++k) x->stage[x->npos[k]] = src[k];
+  CHECK_HIP(x, hipMemcpyAsync(dst, x->stage, sizeof(double) * x->sstride, hipMemcpyHostToDevice,
+                              x->stream),
+            "upload");
+  return MHMHIP_OK;
+}
+
 int get_row(mhmhip_ctx *x, const double *srcdev, double *dst) {
   CHECK_HIP(x, hipMemcpyAsync(x->stage, srcdev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost,
                               x->stream),
@@ -518,6 +556,9 @@ int var_cols(const mhmhip_ctx *x, int var) {
       return x->nh;
     case MHMHIP_P_MAXINTER:
     case MHMHIP_P_PETLAICOR:
+    case MHMHIP_P_PRIETAYALPHA:
+    case MHMHIP_P_AERORESIST:
+    case MHMHIP_P_SURFRESIST:
       return 12;
     default:
       return 1;
@@ -577,6 +618,9 @@ extern "C" int mhmhip_set_cell_var(mhmhip_ctx *x, int32_t var, int32_t set, cons
     case MHMHIP_P_SOILMOIST_EXP: row0 = pr_exp(nh); break;
     case MHMHIP_P_MAXINTER: row0 = pr_maxinter(nh); break;
     case MHMHIP_P_PETLAICOR: row0 = pr_petlai(nh); break;
+    case MHMHIP_P_PRIETAYALPHA: row0 = pr_prietay(nh); break;
+    case MHMHIP_P_AERORESIST: row0 = pr_aero(nh); break;
+    case MHMHIP_P_SURFRESIST: row0 = pr_surf(nh); break;
     case MHMHIP_S_INTER: row0 = SR_INTER; base = x->S; break;
     case MHMHIP_S_SNOWPACK: row0 = SR_SNOW; base = x->S; break;
     case MHMHIP_S_SEALSTW: row0 = SR_SEAL; base = x->S; break;
@@ -693,13 +737,17 @@ extern "C" int mhmhip_stream_forcing(mhmhip_ctx *x, int32_t day0, int32_t ndays,
     return x->fail(MHMHIP_EINVAL, "stream_forcing: a chunk must fit the " + std::to_string(nfd) +
                                       " forcing columns (at least 2; two chunks for double "
                                       "buffering)");
-  int32_t mask = 0, nv = 0;
+  // the given variables this context holds, in variable order (the others
+  // are not read by its PET case and are skipped)
+  FRowMap map{};
   for (int v = 0; v < MHMHIP_F_COUNT; ++v)
-    if (host[v]) {
-      mask |= 1 << v;
-      ++nv;
-    }
-  if (!mask) return x->fail(MHMHIP_EINVAL, "stream_forcing: no variable given");
+    if (host[v] && x->frow[v] >= 0) map.row[map.n++] = x->frow[v];
+  const int32_t nv = map.n;
+  if (nv == 0) {
+    bool any = false;
+    for (int v = 0; v < MHMHIP_F_COUNT; ++v) any = any || host[v] != nullptr;
+    if (!any) return x->fail(MHMHIP_EINVAL, "stream_forcing: no variable given");
+  }
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "stream_forcing");
   if (!x->cstream) {
     CHECK_HIP(x, hipStreamCreateWithFlags(&x->cstream, hipStreamNonBlocking), "copy stream");
@@ -742,7 +790,7 @@ extern "C" int mhmhip_stream_forcing(mhmhip_ctx *x, int32_t day0, int32_t ndays,
   double *h = x->fstage_h[i];
   int32_t k = 0;
   for (int v = 0; v < MHMHIP_F_COUNT; ++v) {
-    if (!host[v]) continue;
+    if (!host[v] || x->frow[v] < 0) continue;
     for (int32_t j = 0; j < ndays; ++j)
       std::memcpy(h + ((int64_t)k * ndays + j) * x->n_cells, host[v] + (int64_t)j * ld,
                   sizeof(double) * x->n_cells);
@@ -756,11 +804,14 @@ extern "C" int mhmhip_stream_forcing(mhmhip_ctx *x, int32_t day0, int32_t ndays,
   if (e == hipSuccess) e = hipStreamWaitEvent(x->cstream, after, 0);
   (void)hipEventDestroy(after);
   CHECK_HIP(x, e, "stream_forcing order");
-  CHECK_HIP(x, hipMemcpyAsync(x->fstage_d[i], h, sizeof(double) * need, hipMemcpyHostToDevice,
-                              x->cstream), "stream_forcing upload");
+  if (nv > 0)
+    CHECK_HIP(x, hipMemcpyAsync(x->fstage_d[i], h, sizeof(double) * need, hipMemcpyHostToDevice,
+                                x->cstream), "stream_forcing upload");
   CHECK_HIP(x, hipEventRecord(x->fstage_ev[i], x->cstream), "stream_forcing event");
-  CHECK_HIP(x, launch_forcing_gather(x->fstage_d[i], mask, ndays, x->n_cells, x->d_pos, x->F, nfd,
-                                     day0 - 1, x->sstride, x->cstream), "stream_forcing gather");
+  if (nv > 0)
+    CHECK_HIP(x, launch_forcing_gather(x->fstage_d[i], map, ndays, x->n_cells, x->d_pos, x->F, nfd,
+                                       day0 - 1, x->sstride, x->cstream),
+              "stream_forcing gather");
   hipEvent_t done = nullptr;
   CHECK_HIP(x, hipEventCreateWithFlags(&done, hipEventDisableTiming), "stream_forcing event");
   x->chunk_ev.push_back(done);
@@ -780,11 +831,13 @@ extern "C" int mhmhip_set_forcing(mhmhip_ctx *x, int32_t var, int32_t day0, int3
   if (!host || var < 0 || var >= MHMHIP_F_COUNT || day0 < 1 || ndays < 1 ||
       (int64_t)day0 - 1 + ndays > x->cfg.n_forcing_days || ld < x->n_cells)
     return x->fail(MHMHIP_EINVAL, "set_forcing: variable or day range out of bounds");
+  if (x->frow[var] < 0) return MHMHIP_OK;  // not read by this PET case
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_forcing");
   CHECK_RC(forcing_columns_direct(x, day0, ndays));
   auto id = [](double a) { return a; };
   for (int32_t j = 0; j < ndays; ++j) {
-    double *dst = x->F + ((int64_t)var * x->cfg.n_forcing_days + (day0 - 1 + j)) * x->sstride;
+    double *dst =
+        x->F + ((int64_t)x->frow[var] * x->cfg.n_forcing_days + (day0 - 1 + j)) * x->sstride;
     CHECK_RC(put_row(x, dst, host + (int64_t)j * ld, id));
   }
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_forcing");
@@ -824,6 +877,7 @@ extern "C" int mhmhip_set_forcing_l2(mhmhip_ctx *x, int32_t var, int32_t domain,
     return x->fail(MHMHIP_EINVAL, "set_forcing_l2: mask1 must hold the domain's " +
                                       std::to_string(c_hi - c_lo) + " cells, it holds " +
                                       std::to_string(ncells));
+  if (x->frow[var] < 0) return MHMHIP_OK;  // not read by this PET case
   // cellFactorHbyM (mo_meteo_helper.f90:120) picks the mapping
   const double hbym = cs1 / cs2;
   const bool aggregate = hbym > 1.0, equal = !(hbym > 1.0) && !(hbym < 1.0);
@@ -912,10 +966,40 @@ extern "C" int mhmhip_set_forcing_l2(mhmhip_ctx *x, int32_t var, int32_t domain,
             "set_forcing_l2 upload");
   CHECK_HIP(x, hipMemcpyAsync(b.cells, cells.data(), sizeof(MeteoL2Cell) * ncells,
                               hipMemcpyHostToDevice, x->stream), "set_forcing_l2 upload");
-  double *dst = x->F + ((int64_t)var * x->cfg.n_forcing_days + (day0 - 1)) * x->sstride;
+  double *dst = x->F + ((int64_t)x->frow[var] * x->cfg.n_forcing_days + (day0 - 1)) * x->sstride;
   CHECK_HIP(x, launch_meteo_l2(b.data2, b.mask2, nr2, n2, b.cells, ncells, ndays, dst, x->sstride,
                                aggregate, x->stream), "set_forcing_l2 map");
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_forcing_l2");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_set_meteo_weights(mhmhip_ctx *x, const double *pre_w,
+                                        const double *temp_w, const double *pet_w, int64_t ld) {
+  if (!x) return MHMHIP_EINVAL;
+  const int given = (pre_w != nullptr) + (temp_w != nullptr) + (pet_w != nullptr);
+  if (given != 0 && given != 3)
+    return x->fail(MHMHIP_EINVAL, "set_meteo_weights: give all three weight arrays or none");
+  if (given == 3 && ld < x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "set_meteo_weights: leading dimension below the cell count");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_meteo_weights");
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_meteo_weights");
+  drop_graph(x);  // a captured run holds the kernels' arguments
+  if (given == 0) {
+    x->dfree(x->W, (int64_t)3 * 12 * 24 * x->sstride);
+    x->W = nullptr;
+    return MHMHIP_OK;
+  }
+  if (!x->W) CHECK_RC(x->dalloc(&x->W, (int64_t)3 * 12 * 24 * x->sstride, "alloc meteo weights"));
+  // W[v][month * 24 + hour][sstride], v = pre, temp, pet; the host arrays are
+  // L1_x_weights(cell, month, hour + 1)
+  const double *host[3] = {pre_w, temp_w, pet_w};
+  auto id = [](double a) { return a; };
+  for (int v = 0; v < 3; ++v)
+    for (int m = 0; m < 12; ++m)
+      for (int h = 0; h < 24; ++h)
+        CHECK_RC(put_row(x, x->W + ((int64_t)v * 288 + m * 24 + h) * x->sstride,
+                         host[v] + ld * ((int64_t)m + 12 * h), id));
+  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_meteo_weights");
   return MHMHIP_OK;
 }
 
@@ -954,6 +1038,7 @@ int make_step(mhmhip_ctx *x, const mhmhip_step &s, StepDev &d) {
   d.init_sm = (s.tt == 1) && !s.read_states;  // mo_mhm.f90:448
   d.doy = s.doy;
   d.meteo_same = 0;
+  d.wrow = m * 24 + s.hour;
   return MHMHIP_OK;
 }
 
@@ -979,6 +1064,7 @@ CellArgs cell_args(mhmhip_ctx *x, bool ring) {
   a.nT = x->nT;
   a.rnT = 1.0 / x->nT;
   a.ring_rep = 1;
+  a.W = x->W;
   if (x->out_on) a.out = x->out_cfg;
   else a.out.sum = nullptr;
   return a;
@@ -1033,7 +1119,7 @@ RouteArgs route_args(mhmhip_ctx *x, int64_t t_lo, int64_t t_hi, int64_t t_base) 
   r.inflow_node = x->d_inflow_node;
   r.inflow_hw = x->d_inflow_hw;
   r.inflow_q = x->d_inflow_q;
-  r.n_nodes = x->n_cells;
+  r.n_nodes = x->n_nodes;
   r.sstride = x->sstride;
   r.ring_len = x->ring_len;
   r.t_lo = t_lo;
@@ -1053,7 +1139,7 @@ RouteArgs route_args(mhmhip_ctx *x, int64_t t_lo, int64_t t_hi, int64_t t_base) 
 int64_t max_domain_nodes(const mhmhip_ctx *x) {
   int64_t m = 0;
   for (int d = 0; d < x->n_domains; ++d)
-    m = std::max(m, x->dom_cell_start[d + 1] - x->dom_cell_start[d]);
+    m = std::max(m, x->dom_node_start[d + 1] - x->dom_node_start[d]);
   return m;
 }
 
@@ -1177,7 +1263,7 @@ int prof_collect_graph(mhmhip_ctx *x) {
 // keep depths contiguous; packed domains use the full range.
 void active_range(const mhmhip_ctx *x, const RouteArgs &r, int64_t d, int64_t &lo, int64_t &hi) {
   lo = 0;
-  hi = x->n_cells;
+  hi = x->n_nodes;
   if (x->lev_start.empty()) return;
   const int64_t dm = x->lay.dmax;
   const int64_t dep_lo = std::max<int64_t>(0, r.t_lo - d + dm);
@@ -1255,7 +1341,7 @@ void copy_dup_gauges(const mhmhip_ctx *x, double *out, int64_t nsteps) {
 
 // node-steps routed, credited to the routing kernel that did them
 void credit_route(mhmhip_ctx *x, int kind, int64_t nsteps) {
-  if (x->prof) x->prof_units[kind] += (double)x->n_cells * x->n_sets * (double)nsteps;
+  if (x->prof) x->prof_units[kind] += (double)x->n_nodes * x->n_sets * (double)nsteps;
 }
 
 // Moves qTR(:,IT1) into parity slot 1, where a call's step -1 reads it.
@@ -1299,10 +1385,11 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
       return x->fail(MHMHIP_ESTATE, "mhm_step: previous step not routed (call mrm_step)");
     CHECK_RC(ensure_ring(x, 1 + kTileHeight));
   }
-  CellArgs a = cell_args(x, route);
-  // the step is step 0 of a one-step ring, routed by the tiles (mrm_step)
+  // the step is step 0 of a one-step ring, routed by the tiles (mrm_step); on
+  // a coarser routing grid mrm_step aggregates the step's L1_total_runoff
+  CellArgs a = cell_args(x, route && !x->coarse);
   a.ring_len = 1;
-  a.lag = route ? x->d_lag : nullptr;
+  a.lag = route && !x->coarse ? x->d_lag : nullptr;
   CHECK_HIP(x, timed(x, MHMHIP_K_CELL_STEP, (double)x->n_cells * x->n_sets, [&] {
               return launch_cell_step(x->nh, a, d, 0, 256, x->n_sets, x->stream);
             }),
@@ -1321,17 +1408,35 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   if (!net || !net->netPerm || !net->fromN || !net->toN || !net->length || !net->slope ||
       !net->fFPimp || !net->L1_area || !net->L1_L11_Id || !net->n_outlets)
     return x->fail(MHMHIP_EINVAL, "bind_network: missing arrays");
-  if (net->n_nodes != x->n_cells)
-    return x->fail(MHMHIP_EINVAL, "bind_network: L1 = L11 required (n_nodes == n_cells)");
+  // L1_L11_Id maps every L1 cell to an L11 node of its domain; the routing
+  // grid is the L1 grid (identity) or coarser (map_flag, the runoff of a
+  // node's cells is aggregated: mo_mrm_pre_routing.f90:110-126).  A routing
+  // grid finer than L1 (map_flag = .false., the gathering branch) is refused.
+  if (net->n_nodes < 1 || net->n_nodes > x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "bind_network: n_nodes must be 1..n_cells (a routing grid finer "
+                                  "than the hydrology grid is not supported)");
+  if (net->n_nodes != x->n_cells && !net->domain_node_start && x->n_domains > 1)
+    return x->fail(MHMHIP_EINVAL, "bind_network: domain_node_start required when n_nodes != n_cells");
   std::vector<int64_t> ds(x->n_domains + 1);
   for (int d = 0; d <= x->n_domains; ++d)
-    ds[d] = net->domain_node_start ? net->domain_node_start[d] : x->dom_cell_start[d];
-  if (ds != x->dom_cell_start)
-    return x->fail(MHMHIP_EINVAL, "bind_network: domain node ranges differ from cell ranges");
-  for (int d = 0; d < x->n_domains; ++d)
-    for (int64_t k = ds[d]; k < ds[d + 1]; ++k)
-      if (net->L1_L11_Id[k] != k - ds[d] + 1)
-        return x->fail(MHMHIP_EINVAL, "bind_network: L1_L11_Id must map each cell to its own node");
+    ds[d] = net->domain_node_start ? net->domain_node_start[d]
+                                   : (d == 0 ? 0 : (d == x->n_domains ? net->n_nodes
+                                                                      : x->dom_cell_start[d]));
+  if (ds[0] != 0 || ds[x->n_domains] != net->n_nodes)
+    return x->fail(MHMHIP_EINVAL, "bind_network: domain node ranges must cover the n_nodes nodes");
+  bool identity = net->n_nodes == x->n_cells && ds == x->dom_cell_start;
+  for (int d = 0; d < x->n_domains; ++d) {
+    if (ds[d + 1] <= ds[d])
+      return x->fail(MHMHIP_EINVAL, "bind_network: every domain needs a node");
+    const int64_t nd = ds[d + 1] - ds[d];
+    for (int64_t k = x->dom_cell_start[d]; k < x->dom_cell_start[d + 1]; ++k) {
+      const int32_t id = net->L1_L11_Id[k];
+      if (id < 1 || id > nd)
+        return x->fail(MHMHIP_EINVAL, "bind_network: L1_L11_Id out of its domain's node range");
+      identity = identity && id == k - x->dom_cell_start[d] + 1;
+    }
+  }
+  const bool coarse = !identity;
   if (net->n_gauges < 0 || net->n_inflow < 0 ||
       (net->n_gauges > 0 && (!net->gauge_node || !net->gauge_domain)) ||
       (net->n_inflow > 0 && (!net->inflow_node || !net->inflow_domain || !net->inflow_headwater)))
@@ -1339,7 +1444,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "bind_network");
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "bind_network");
 
-  LayoutIn in{x->n_cells, x->n_domains, ds.data(), net->n_outlets, net->netPerm, net->fromN,
+  LayoutIn in{net->n_nodes, x->n_domains, ds.data(), net->n_outlets, net->netPerm, net->fromN,
               net->toN};
   LayoutOut lay;
   std::string e;
@@ -1375,11 +1480,33 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
     }
   }
 
-  // re-lay any cell data already on the device into routing order
+  // device cells: in routing order (cell = node) on an L1 = L11 grid, else
+  // grouped by their node's device index, reference order inside a node
+  std::vector<int64_t> cpos(x->n_cells);
+  std::vector<int32_t> agg_start;
+  if (!coarse) {
+    cpos = lay.pos;
+  } else {
+    std::vector<int64_t> node_dev(x->n_cells);
+    for (int d = 0; d < x->n_domains; ++d)
+      for (int64_t k = x->dom_cell_start[d]; k < x->dom_cell_start[d + 1]; ++k)
+        node_dev[k] = lay.pos[ds[d] + net->L1_L11_Id[k] - 1];
+    std::vector<int64_t> order(x->n_cells);
+    for (int64_t k = 0; k < x->n_cells; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return node_dev[a] < node_dev[b]; });
+    agg_start.assign(net->n_nodes + 1, 0);
+    for (int64_t r = 0; r < x->n_cells; ++r) {
+      cpos[order[r]] = r;
+      agg_start[node_dev[order[r]] + 1] += 1;
+    }
+    for (int64_t p = 0; p < net->n_nodes; ++p) agg_start[p + 1] += agg_start[p];
+  }
+  // re-lay any cell data already on the device into the new cell order
   if (x->cstream) CHECK_HIP(x, hipStreamSynchronize(x->cstream), "bind_network");
   std::vector<int64_t> old_pos = x->pos;
   bool moved = false;
-  for (int64_t c = 0; c < x->n_cells && !moved; ++c) moved = old_pos[c] != lay.pos[c];
+  for (int64_t c = 0; c < x->n_cells && !moved; ++c) moved = old_pos[c] != cpos[c];
   if (moved) {
     std::vector<double> a(x->sstride), b(x->sstride);
     auto relayout = [&](double *base, int64_t rows, int64_t stride, int64_t nsl) -> int {
@@ -1389,7 +1516,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
           CHECK_HIP(x, sync_copy(x, a.data(), row, sizeof(double) * x->sstride, hipMemcpyDeviceToHost),
                     "relayout");
           std::fill(b.begin(), b.end(), 0.0);
-          for (int64_t c = 0; c < x->n_cells; ++c) b[lay.pos[c]] = a[old_pos[c]];
+          for (int64_t c = 0; c < x->n_cells; ++c) b[cpos[c]] = a[old_pos[c]];
           CHECK_HIP(x, sync_copy(x, row, b.data(), sizeof(double) * x->sstride, hipMemcpyHostToDevice),
                     "relayout");
         }
@@ -1398,10 +1525,12 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
     CHECK_RC(relayout(x->P, x->n_prows, x->ld, x->n_sets));
     CHECK_RC(relayout(x->S, x->n_srows, x->ld, x->n_sets));
     CHECK_RC(relayout(x->X, x->n_xrows, x->ld, x->n_sets));
-    CHECK_RC(relayout(x->F, (int64_t)MHMHIP_F_COUNT * x->cfg.n_forcing_days, x->sstride, 1));
+    CHECK_RC(relayout(x->F, (int64_t)x->n_frows * x->cfg.n_forcing_days, x->sstride, 1));
+    if (x->W) CHECK_RC(relayout(x->W, (int64_t)3 * 12 * 24, x->sstride, 1));
   }
-  x->pos = lay.pos;
+  x->pos = cpos;
   x->d_pos_stale = true;
+  const int64_t old_nodes = x->n_nodes;
 
   // per-link host copies for reg_rout, packed by domain
   x->h_nout.assign(net->n_outlets, net->n_outlets + x->n_domains);
@@ -1424,8 +1553,11 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->gauge_buf_rows = 0;
   x->dfree(x->d_inflow_q, x->inflow_q_rows * x->n_inflow);
   x->inflow_q_rows = 0;
-  x->dfree(x->d_meta, x->n_cells);
+  x->dfree(x->d_meta, old_nodes);
   x->dfree(x->d_area, x->n_cells);
+  x->dfree(x->d_agg_start, old_nodes + 1);
+  x->dfree(x->d_cring, x->cring_rows * x->ld);
+  x->cring_rows = 0;
   x->dfree(x->d_C1, x->ld);
   x->dfree(x->d_C2, x->ld);
   x->dfree(x->d_qTIN, x->ld);
@@ -1440,8 +1572,8 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->dfree(x->d_bnd, x->n_bnd_dev);
   x->n_child_dev = x->n_slots_dev = x->n_bins_dev = x->n_bnd_dev = 0;
   x->dfree(x->d_gauge_node, std::max(1, x->n_gauges));
-  x->dfree(x->d_node_gauge, x->n_cells);
-  x->dfree(x->d_lag, x->n_cells);
+  x->dfree(x->d_node_gauge, old_nodes);
+  x->dfree(x->d_lag, old_nodes);
   x->dfree(x->d_qacc, x->ld);  // a new network restarts the output aggregation
   x->dfree(x->d_qsub, x->ld);
   x->dfree(x->d_runacc, x->ld);
@@ -1451,8 +1583,19 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->qacc_count = 0;
   x->dfree(x->d_inflow_node, std::max(1, x->n_inflow));
   x->dfree(x->d_inflow_hw, std::max(1, x->n_inflow));
-  CHECK_RC(x->dalloc(&x->d_meta, x->n_cells, "alloc meta"));
+  const int64_t nn = net->n_nodes;
+  x->n_nodes = nn;
+  x->npos = lay.pos;
+  x->dom_node_start = ds;
+  x->coarse = coarse;
+  CHECK_RC(x->dalloc(&x->d_meta, nn, "alloc meta"));
   CHECK_RC(x->dalloc(&x->d_area, x->n_cells, "alloc area"));
+  if (coarse) {
+    CHECK_RC(x->dalloc(&x->d_agg_start, nn + 1, "alloc node cell ranges"));
+    CHECK_HIP(x, sync_copy(x, x->d_agg_start, agg_start.data(), sizeof(int32_t) * (nn + 1),
+                           hipMemcpyHostToDevice),
+              "upload node cell ranges");
+  }
   CHECK_RC(x->dalloc(&x->d_C1, x->ld, "alloc C1"));
   CHECK_RC(x->dalloc(&x->d_C2, x->ld, "alloc C2"));
   CHECK_RC(x->dalloc(&x->d_qTIN, x->ld, "alloc qTIN"));
@@ -1510,40 +1653,40 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   x->n_gauges = net->n_gauges;
   x->n_inflow = net->n_inflow;
   CHECK_RC(x->dalloc(&x->d_gauge_node, std::max(1, x->n_gauges), "alloc gauges"));
-  CHECK_RC(x->dalloc(&x->d_node_gauge, x->n_cells, "alloc gauges"));
-  CHECK_RC(x->dalloc(&x->d_lag, x->n_cells, "alloc ring lags"));
+  CHECK_RC(x->dalloc(&x->d_node_gauge, nn, "alloc gauges"));
+  CHECK_RC(x->dalloc(&x->d_lag, nn, "alloc ring lags"));
   CHECK_RC(x->dalloc(&x->d_inflow_node, std::max(1, x->n_inflow), "alloc inflow"));
   CHECK_RC(x->dalloc(&x->d_inflow_hw, std::max(1, x->n_inflow), "alloc inflow"));
 
-  std::vector<NodeMeta> meta(x->n_cells);
+  std::vector<NodeMeta> meta(nn);
   std::vector<double> area(x->n_cells);
-  for (int64_t p = 0; p < x->n_cells; ++p) {
+  for (int64_t p = 0; p < nn; ++p) {
     meta[p].child = lay.child[p];
     meta[p].packed = lay.packed[p];
-    area[p] = net->L1_area[lay.node_at[p]];
   }
+  for (int64_t k = 0; k < x->n_cells; ++k) area[cpos[k]] = net->L1_area[k];  // per L1 cell
   std::vector<int64_t> dstart(x->n_domains + 1);
   for (int d = 0; d <= x->n_domains; ++d) dstart[d] = ds[d];  // BFS keeps domains contiguous
-  CHECK_HIP(x, sync_copy(x, x->d_meta, meta.data(), sizeof(NodeMeta) * x->n_cells, hipMemcpyHostToDevice), "upload meta");
+  CHECK_HIP(x, sync_copy(x, x->d_meta, meta.data(), sizeof(NodeMeta) * nn, hipMemcpyHostToDevice), "upload meta");
   CHECK_HIP(x, sync_copy(x, x->d_area, area.data(), sizeof(double) * x->n_cells, hipMemcpyHostToDevice), "upload area");
   CHECK_HIP(x, sync_copy(x, x->d_dom_start, dstart.data(), sizeof(int64_t) * (x->n_domains + 1), hipMemcpyHostToDevice), "upload domains");
   CHECK_HIP(x, sync_copy(x, x->d_dom_dmax, lay.dom_dmax.data(), sizeof(int32_t) * x->n_domains, hipMemcpyHostToDevice), "upload domains");
   CHECK_HIP(x, sync_copy(x, x->d_gauge_node, gnode.data(), sizeof(int32_t) * gnode.size(), hipMemcpyHostToDevice), "upload gauges");
   // the tiled kernel writes a node's first gauge; later gauges at the same
   // node are copies of it
-  std::vector<int32_t> node_gauge(x->n_cells, -1);
+  std::vector<int32_t> node_gauge(nn, -1);
   x->gauge_dup.clear();
   for (int g = 0; g < net->n_gauges; ++g) {
     int32_t &first = node_gauge[gnode[g]];
     if (first < 0) first = g;
     else x->gauge_dup.emplace_back(g, first);
   }
-  CHECK_HIP(x, sync_copy(x, x->d_node_gauge, node_gauge.data(), sizeof(int32_t) * x->n_cells, hipMemcpyHostToDevice), "upload gauges");
+  CHECK_HIP(x, sync_copy(x, x->d_node_gauge, node_gauge.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice), "upload gauges");
   // ring lag of each node: the lag of its real slot
-  std::vector<uint8_t> lag(x->n_cells, 0);
+  std::vector<uint8_t> lag(nn, 0);
   for (const SlotMeta &m : lay.slots)
     if (m.a & SF_REAL) lag[m.pos] = (uint8_t)(m.a & 0xFFF);
-  CHECK_HIP(x, sync_copy(x, x->d_lag, lag.data(), x->n_cells, hipMemcpyHostToDevice), "upload ring lags");
+  CHECK_HIP(x, sync_copy(x, x->d_lag, lag.data(), nn, hipMemcpyHostToDevice), "upload ring lags");
   CHECK_HIP(x, sync_copy(x, x->d_inflow_node, inode.data(), sizeof(int32_t) * inode.size(), hipMemcpyHostToDevice), "upload inflow");
   CHECK_HIP(x, sync_copy(x, x->d_inflow_hw, ihw.data(), sizeof(int32_t) * ihw.size(), hipMemcpyHostToDevice), "upload inflow");
   x->h_inflow_node_dev = inode;
@@ -1589,7 +1732,7 @@ int upload_coefs(mhmhip_ctx *x, int32_t set, Fn coef) {
     if (!coef(d, l0, l1, c1.data(), c2.data())) continue;
     for (int64_t i = l0; i < l1; ++i) touched[i] = 1;
   }
-  for (int64_t q = 0; q < x->n_cells; ++q) {
+  for (int64_t q = 0; q < x->n_nodes; ++q) {
     const int64_t l = x->lay.link_of_node[q];
     if (l >= 0 && touched[l]) {
       n1[q] = c1[l];
@@ -1832,13 +1975,13 @@ extern "C" int mhmhip_get_routing_state(mhmhip_ctx *x, int32_t set, double *qOUT
   auto node_out = [&](const double *dev, double *dst) -> int {
     if (!dst) return MHMHIP_OK;
     CHECK_HIP(x, sync_copy(x, buf.data(), dev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download");
-    for (int64_t k = 0; k < x->n_cells; ++k) dst[k] = buf[x->pos[k]];
+    for (int64_t k = 0; k < x->n_nodes; ++k) dst[k] = buf[x->npos[k]];
     return MHMHIP_OK;
   };
   auto link_out = [&](const double *dev, double *dst) -> int {
     if (!dst) return MHMHIP_OK;
     CHECK_HIP(x, sync_copy(x, buf.data(), dev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost), "download");
-    for (int64_t q = 0; q < x->n_cells; ++q)
+    for (int64_t q = 0; q < x->n_nodes; ++q)
       if (x->lay.link_of_node[q] >= 0) dst[x->lay.link_of_node[q]] = buf[q];
     return MHMHIP_OK;
   };
@@ -1847,7 +1990,7 @@ extern "C" int mhmhip_get_routing_state(mhmhip_ctx *x, int32_t set, double *qOUT
   if (Qmod && x->qsub_loops > 1) {
     // the average of the last step's routing sub-steps (qAcc / rout_loop)
     CHECK_RC(node_out(x->d_qsub + off, Qmod));
-    for (int64_t k = 0; k < x->n_cells; ++k) Qmod[k] = Qmod[k] / (double)x->qsub_loops;
+    for (int64_t k = 0; k < x->n_nodes; ++k) Qmod[k] = Qmod[k] / (double)x->qsub_loops;
   }
   if (qTIN)
     for (size_t i = 0; i < x->lone_nodes.size(); ++i)
@@ -1885,7 +2028,7 @@ extern "C" int mhmhip_get_qmod_sum(mhmhip_ctx *x, int32_t set, double *sum, int6
     CHECK_HIP(x, sync_copy(x, buf.data(), x->d_qacc + (int64_t)set * x->sstride,
                            sizeof(double) * x->sstride, hipMemcpyDeviceToHost),
               "download");
-    for (int64_t k = 0; k < x->n_cells; ++k) sum[k] = buf[x->pos[k]];
+    for (int64_t k = 0; k < x->n_nodes; ++k) sum[k] = buf[x->npos[k]];
   }
   if (count) *count = x->qacc_count;
   return MHMHIP_OK;
@@ -1972,12 +2115,11 @@ extern "C" int mhmhip_set_routing_state(mhmhip_ctx *x, int32_t set, const double
     return x->fail(MHMHIP_EINVAL, "set_routing_state: give C1 and C2 together");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_routing_state");
   const int64_t off = (int64_t)set * x->sstride;
-  auto id = [](double a) { return a; };
-  if (qTIN) CHECK_RC(put_row(x, x->d_qTIN + off, qTIN, id));
+  if (qTIN) CHECK_RC(put_node_row(x, x->d_qTIN + off, qTIN));
   if (qTIN)
     for (size_t i = 0; i < x->lone_nodes.size(); ++i)
       x->lone_qtin[(size_t)set * x->lone_nodes.size() + i] = qTIN[x->lone_nodes[i]];
-  if (qTR) CHECK_RC(put_row(x, x->d_pub + x->qtr_slot * x->ld + off, qTR, id));
+  if (qTR) CHECK_RC(put_node_row(x, x->d_pub + x->qtr_slot * x->ld + off, qTR));
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_routing_state");
   if (C1) {
     // the restart's L11_C1/L11_C2 (mo_mrm_restart.f90:548-555), per link
@@ -2031,10 +2173,19 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
         x->inflow_acc[g] = x->inflow_acc[g] + x->h_inflow_row[g];
     }
     const double tst = kHourSecs * (double)(x->cfg.timestep * (int32_t)std::floor(fin + 0.5));
+    // RunToRout per L1 cell; on a coarser routing grid it is aggregated per
+    // node on the routing step and starts again from 0
     CHECK_HIP(x, launch_runoff_sum(x->X + XR_TOTAL * x->ld, x->d_runacc, x->d_ring, x->d_area,
-                                   x->d_lag, x->n_cells, x->sstride, x->ld, x->n_sets, tst, route,
-                                   x->stream),
+                                   x->d_lag, x->n_cells, x->sstride, x->ld, x->n_sets, tst,
+                                   route && !x->coarse, x->stream),
               "runoff_sum");
+    if (route && x->coarse) {
+      CHECK_HIP(x, launch_runoff_agg(x->d_runacc, 1, x->d_agg_start, x->d_area, x->n_nodes,
+                                     x->sstride, x->ld, x->n_sets, x->d_ring, x->d_lag, 1, 1, 0,
+                                     1, tst, x->stream),
+                "runoff_agg");
+      CHECK_HIP(x, hipMemsetAsync(x->d_runacc, 0, sizeof(double) * x->ld, x->stream), "mrm_step");
+    }
     if (!route) {
       x->t_route++;
       CHECK_HIP(x, hipStreamSynchronize(x->stream), "mrm_step");
@@ -2052,6 +2203,13 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
       inflow_row = x->d_inflow_avg;
     }
   }
+  if (x->coarse && factor <= 1.0)
+    // L11_runoff_acc on the coarser routing grid: this step's L1_total_runoff
+    // (mhm_step's flux row) aggregated into ring row 0 at each node's lag
+    CHECK_HIP(x, launch_runoff_agg(x->X + XR_TOTAL * x->ld, 1, x->d_agg_start, x->d_area,
+                                   x->n_nodes, x->sstride, x->ld, x->n_sets, x->d_ring, x->d_lag,
+                                   1, 1, 0, 1, kHourSecs * (double)x->cfg.timestep, x->stream),
+              "runoff_agg");
   const int kind = MHMHIP_K_ROUTE_TILE;
   const TileArgs g = tile_args(x, 1, 1);
   const int64_t ng = (int64_t)x->n_sets * x->n_gauges;
@@ -2187,13 +2345,15 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
   if (route && !steps[0].read_states) CHECK_RC(refresh_reg_rout(x));
   std::vector<StepDev> hs(nsteps);
   for (int32_t k = 0; k < nsteps; ++k) CHECK_RC(make_step(x, steps[k], hs[k]));
-  // a step whose forcing column, day and day/night factors equal the step
-  // before's has the same disaggregated meteo values (the fused kernel keeps
-  // them; its first step of a launch computes them regardless)
+  // a step whose forcing column, day and day/night factors (and, with meteo
+  // weights, month and hour) equal the step before's has the same
+  // disaggregated meteo values (the fused kernel keeps them; its first step
+  // of a launch computes them regardless)
   for (int32_t k = 1; k < nsteps; ++k) {
     const StepDev &p = hs[k - 1], &c = hs[k];
     hs[k].meteo_same = p.day == c.day && p.doy == c.doy && p.month0 == c.month0 &&
-                       p.f_pet == c.f_pet && p.f_prec == c.f_prec && p.f_temp == c.f_temp;
+                       p.f_pet == c.f_pet && p.f_prec == c.f_prec && p.f_temp == c.f_temp &&
+                       (!x->W || p.wrow == c.wrow);
   }
   CHECK_RC(forcing_wait(x, hs.data(), nsteps));
   const int S = o.mode == MHMHIP_RUN_PLAIN ? 1 : (o.steps_per_launch > 0 ? o.steps_per_launch : 24);
@@ -2208,6 +2368,9 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
     // graph capture and instantiation, the routing stream and the timing
     // events would all allocate
     if (o.use_graph) return x->fail(MHMHIP_EINVAL, "run_noalloc: use_graph is not available");
+    if (route && x->coarse)
+      return x->fail(MHMHIP_EINVAL, "run_noalloc: routing on a grid coarser than L1 is not "
+                                    "available (use mhmhip_run)");
     if (rep > 1 && !x->d_qsub)
       return x->fail(MHMHIP_EINVAL, "run_noalloc: routing sub-steps keep their Qmod sums in a "
                                     "context buffer that a first mhmhip_run allocates");
@@ -2248,6 +2411,13 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
     if (rep > 1 && !x->d_qsub) CHECK_RC(x->dalloc(&x->d_qsub, x->ld, "alloc Qmod sub-step sums"));
   }
   if (route) CHECK_RC(normalize_qtr(x));
+  if (route && x->coarse && x->cring_rows < S) {
+    // L1_total_runoff of one launch's steps, aggregated per node after it
+    x->dfree(x->d_cring, x->cring_rows * x->ld);
+    x->cring_rows = 0;
+    CHECK_RC(x->dalloc(&x->d_cring, (int64_t)S * x->ld, "alloc cell runoff rows"));
+    x->cring_rows = S;
+  }
   CellArgs a = cell_args(x, route);
   RouteArgs r = route_args(x, 0, nr, 0);
   if (route) {
@@ -2260,6 +2430,17 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
       r.qacc = x->d_qsub;
       r.acc_from = nr - rep;
     }
+  }
+  // on a routing grid coarser than L1 the cell kernels write each step's
+  // L1_total_runoff to the launch's cell rows, and runoff_agg_kernel turns
+  // them into the nodes' qOUT at the ring rows the cell kernels would write
+  const uint8_t *node_lag = a.lag;
+  if (route && x->coarse) {
+    a.ring = x->d_cring;
+    a.ring_len = S;
+    a.lag = nullptr;
+    a.ring_rep = 1;
+    a.raw_total = 1;
   }
   const TileArgs g = tile_args(x, B, nr);
   const int32_t L_end = g.nwin + g.tau_max;
@@ -2304,6 +2485,12 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
                   "cell_fused");
       }
       if (!route) continue;
+      if (x->coarse)
+        CHECK_HIP(x, launch_runoff_agg(x->d_cring, S, x->d_agg_start, x->d_area, x->n_nodes,
+                                       x->sstride, x->ld, x->n_sets, x->d_ring, node_lag,
+                                       ring_rows, rep, k, n,
+                                       kHourSecs * (double)x->cfg.timestep, x->stream),
+                  "runoff_agg");
       if (kind == MHMHIP_K_ROUTE_TILE && x->n_inflow > 0)
         CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, u0, u1, x->stream, rep),
                   "inflow_patch");
@@ -2357,6 +2544,7 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
     key.out = a.out.sum;
     key.out_n = a.out.sum ? a.out.n : 0;
     key.rep = rep;
+    key.cring = route && x->coarse ? x->d_cring : nullptr;
     if (!x->graph_exec || !(key == x->gkey)) {
       drop_graph(x);
       CHECK_HIP(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal), "capture");

@@ -477,6 +477,58 @@ __device__ MHMHIP_OUTLINE double pet_hargreaves(double coeff, double tavg, doubl
   return coeff * ra * (tavg + kHarSamConst) * sqrt(dT);
 }
 
+// FORCES mo_constants: DaySecs, SpecHeatET_dp, Psychro_dp, rho0_dp, cp0_dp, T0_dp;
+// src/mHM/mo_mhm_constants.f90:44-48: tetens_c1..c3, satpressureslope1
+constexpr double kDaySecs = 86400.0;
+constexpr double kSpecHeatET = 2.45e06;
+constexpr double kPsychro = 0.0646;
+constexpr double kRho0 = 1.225;
+constexpr double kCp0 = 1005.0;
+constexpr double kT0 = 273.15;
+constexpr double kTetensC1 = 0.6108;
+constexpr double kTetensC2 = 17.270;
+constexpr double kTetensC3 = 237.30;
+constexpr double kSatPressureSlope1 = 4098.0;
+
+// src/mHM/mo_pet.f90:426-441
+__device__ __forceinline__ double sat_vap_pressure(double tavg) {
+  return kTetensC1 * exp_fin(kTetensC2 * tavg / (tavg + kTetensC3));
+}
+
+// src/mHM/mo_pet.f90:384-399: exp(2 log(tavg + c3)) as written (NaN below
+// tavg = -c3, as log there)
+__device__ __forceinline__ double slope_satpressure(double tavg) {
+  const double b = tavg + kTetensC3;
+  const double sq = b > 0.0 ? exp_fin(2.0 * log_pos(b)) : (b == 0.0 ? 0.0 : __builtin_nan(""));
+  return kSatPressureSlope1 * sat_vap_pressure(tavg) / sq;
+}
+
+// src/mHM/mo_pet.f90:149-174 (Rn already max(netrad, 0), mo_meteo_handler.f90:1081-1086)
+__device__ MHMHIP_OUTLINE double pet_priestly(double alpha, double rn, double tavg) {
+  const double delta = slope_satpressure(tavg);
+  return alpha * delta / (kPsychro + delta) * (rn * kDaySecs / kSpecHeatET);
+}
+
+// src/mHM/mo_pet.f90:235-272 with the getter's arguments
+// (mo_meteo_handler.f90:1088-1095): net_rad = max(netrad, 0), act_vap_pressure =
+// absvappress / 1000, aerodyn_resistance = aeroResist / windspeed, a_s = a_sh = 1.
+// slope_satpressure is evaluated once (the reference's two calls give one value).
+__device__ MHMHIP_OUTLINE double pet_penman(double net_rad, double tavg, double act_vap_pressure,
+                                             double aerodyn_resistance,
+                                             double bulksurface_resistance) {
+  const double a_s = 1.0, a_sh = 1.0;
+  const double slope = slope_satpressure(tavg);
+  return kDaySecs / kSpecHeatET *
+         (slope * net_rad +
+          kRho0 * kCp0 * (sat_vap_pressure(tavg) - act_vap_pressure) * a_sh / aerodyn_resistance) /
+         (slope + kPsychro * a_sh / a_s * (1.0 + bulksurface_resistance / aerodyn_resistance));
+}
+
+// src/meteo/mo_meteo_temporal_tools.f90:32-59: (v + c) * w - c
+__device__ __forceinline__ double disagg_weights(double v, double w, double c) {
+  return (v + c) * w - c;
+}
+
 // src/meteo/mo_meteo_temporal_tools.f90:54-79 with the day or night factor f
 // already chosen: 2 * v * f / ntimesteps_day
 // (rnT = RN(1/nT): the division by nT as div_rcp, same bits)

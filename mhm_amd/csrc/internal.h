@@ -9,7 +9,8 @@
 //   P[row][ld]   effective parameters          (ld = n_sets * sstride)
 //   S[row][ld]   states
 //   X[row][ld]   fluxes of the last step
-//   F[var][column][sstride]  forcing, shared by all parameter sets; day d
+//   F[row][column][sstride]  forcing, shared by all parameter sets: the
+//                variables the PET case reads, one row each (frow); day d
 //                (iMeteoTS) in column (d - 1) mod n_forcing_days
 //   ring[R + kTileHeight][ld]  qOUT per step (L1 total runoff in m3/s),
 //                consumed by the routing: step t of node p in row
@@ -17,6 +18,8 @@
 #pragma once
 
 #include <stdint.h>
+
+#include "../../include/mhmhip.h"
 
 namespace mhmhip {
 
@@ -38,7 +41,10 @@ __host__ __device__ constexpr int pr_wp(int nh) { return PR_NSCALAR + 3 * nh; }
 __host__ __device__ constexpr int pr_exp(int nh) { return PR_NSCALAR + 4 * nh; }
 __host__ __device__ constexpr int pr_maxinter(int nh) { return PR_NSCALAR + 5 * nh; }
 __host__ __device__ constexpr int pr_petlai(int nh) { return PR_NSCALAR + 5 * nh + 12; }
-__host__ __device__ constexpr int pr_count(int nh) { return PR_NSCALAR + 5 * nh + 24; }
+__host__ __device__ constexpr int pr_prietay(int nh) { return PR_NSCALAR + 5 * nh + 24; }
+__host__ __device__ constexpr int pr_aero(int nh) { return PR_NSCALAR + 5 * nh + 36; }
+__host__ __device__ constexpr int pr_surf(int nh) { return PR_NSCALAR + 5 * nh + 48; }
+__host__ __device__ constexpr int pr_count(int nh) { return PR_NSCALAR + 5 * nh + 60; }
 
 enum SRow { SR_INTER = 0, SR_SNOW, SR_SEAL, SR_UNSAT, SR_SAT, SR_NSCALAR };
 __host__ __device__ constexpr int sr_count(int nh) { return SR_NSCALAR + nh; }
@@ -65,6 +71,7 @@ struct StepDev {
   int32_t init_sm;              // tt == 1 .and. .not. read_states
   int32_t doy;
   int32_t meteo_same;           // same forcing column, day and factors as the step before
+  int32_t wrow;                 // month0 * 24 + hour: the row of the meteo weights
 };
 
 struct MonthTables {
@@ -107,9 +114,37 @@ struct CellArgs {
   int32_t hourly;
   int32_t ring_rep;        // routing sub-steps per model step: qOUT goes to that
                            // many consecutive ring rows (1: one row per step)
+  int32_t raw_total;       // 1: the ring gets L1_total_runoff per cell (routing on a
+                           // coarser grid aggregates it, runoff_agg_kernel), 0: qOUT
   double nT;               // steps per day as real (nTstepDay_dp)
   double rnT;              // RN(1 / nT)
   OutAcc out;              // output sums (out.sum null: none)
+  // meteo weights [3][12 * 24][sstride] (pre, temp, pet), null: the day/night
+  // factors disaggregate
+  const double *W;
+};
+
+// The row of forcing variable v in F for a PET case (-1: not read by that
+// case, mo_meteo_handler.f90:1058-1097): pre and tavg, then the case's own
+// inputs in variable order.
+__host__ __device__ constexpr int forcing_row(int v, int pet_case) {
+  return v == MHMHIP_F_PRE ? 0
+         : v == MHMHIP_F_TAVG ? 1
+         : pet_case <= 0 ? (v == MHMHIP_F_PET ? 2 : -1)
+         : pet_case == 1 ? (v == MHMHIP_F_TMIN ? 2 : v == MHMHIP_F_TMAX ? 3 : -1)
+         : v == MHMHIP_F_NETRAD ? 2
+         : pet_case == 3 ? (v == MHMHIP_F_ABSVAPPRESS ? 3 : v == MHMHIP_F_WINDSPEED ? 4 : -1)
+                         : -1;
+}
+__host__ __device__ constexpr int forcing_rows(int pet_case) {
+  return pet_case == 1 ? 4 : pet_case == 3 ? 5 : 3;
+}
+
+// Streamed forcing: the forcing row (in F) of the k-th variable of a staged
+// chunk, k < n.
+struct FRowMap {
+  int32_t row[MHMHIP_F_COUNT];
+  int32_t n;
 };
 
 // Routing node metadata, packed in 8 bytes per node.

@@ -248,6 +248,10 @@ __device__ __forceinline__ void add_outputs(const CellArgs &a, uint32_t bv, cons
 // Its per-cell parameters are read here, when a new forcing day starts, rather
 // than held in registers across steps.  bc: byte offset of the cell in a
 // forcing row; bv: byte offset of (cell, set) in a parameter row.
+__device__ __forceinline__ const double *frow_ptr(const CellArgs &a, int v, int32_t day) {
+  return a.F + ((int64_t)forcing_row(v, a.pet_case) * a.forcing_days + day) * a.sstride;
+}
+
 template <int NH>
 __device__ __forceinline__ double pet_day(const CellArgs &a, const StepDev &st, uint32_t bv,
                                           uint32_t bc, double tavg, double tmax, double tmin) {
@@ -256,16 +260,51 @@ __device__ __forceinline__ double pet_day(const CellArgs &a, const StepDev &st, 
     return P(PR_FASP) * pet_hargreaves(P(PR_HARSAM), tavg, tmax, tmin, P(PR_TANLAT),
                                        P(PR_SINLAT), P(PR_COSLAT), st.dr, st.tand, st.sind,
                                        st.cosd);
-  const double petin =
-      ldr(a.F + ((int64_t)MHMHIP_F_PET * a.forcing_days + st.day) * a.sstride, bc);
+  if (a.pet_case >= 2) {
+    // Priestley-Taylor and Penman-Monteith are not defined for netrad < 0
+    const double rn = vmax(ldr(frow_ptr(a, MHMHIP_F_NETRAD, st.day), bc), 0.0);
+    if (a.pet_case == 2) return pet_priestly(P(pr_prietay(NH) + st.month0), rn, tavg);
+    const double avp = ldr(frow_ptr(a, MHMHIP_F_ABSVAPPRESS, st.day), bc) / 1000.0;
+    const double ws = ldr(frow_ptr(a, MHMHIP_F_WINDSPEED, st.day), bc);
+    return pet_penman(rn, tavg, avp, P(pr_aero(NH) + st.month0) / ws,
+                      P(pr_surf(NH) + st.month0));
+  }
+  const double petin = ldr(frow_ptr(a, MHMHIP_F_PET, st.day), bc);
   if (a.pet_case == 0) return P(PR_FASP) * petin;
   return P(pr_petlai(NH) + st.month0) * petin;
+}
+
+// The step's pet, temp and prec from the daily values: the meteo weights of
+// the step's month and hour when given, else the day/night factors
+// (get_corrected_pet 1099-1118, get_temp 1183-1200, get_prec 1265-1282).
+template <int NH>
+__device__ __forceinline__ void disagg_step(const CellArgs &a, const StepDev &st, uint32_t bc,
+                                            double pd, double tavg, double pre, double &pet,
+                                            double &temp, double &prec) {
+  if (a.hourly) {
+    pet = pd;
+    temp = tavg;
+    prec = pre;
+  } else if (a.W) {
+    const int64_t wv = (int64_t)12 * 24 * a.sstride;
+    const double *w = a.W + (int64_t)st.wrow * a.sstride;
+    pet = disagg_weights(pd, ldr(w + 2 * wv, bc), 0.0);
+    temp = disagg_weights(tavg, ldr(w + wv, bc), kT0);
+    prec = disagg_weights(pre, ldr(w, bc), 0.0);
+  } else {
+    pet = disagg_flux(a.nT, a.rnT, pd, st.f_pet);
+    temp = disagg_state_add(a.nT, tavg, st.f_temp);
+    prec = disagg_flux(a.nT, a.rnT, pre, st.f_prec);
+  }
 }
 
 // The step table is read-only during a launch and the same for every lane:
 // read through the constant address space, it comes in with scalar loads
 // into SGPRs instead of vector loads into VGPRs.
 typedef const __attribute__((address_space(4))) StepDev ConstStep;
+// read_step copies every field: a field added to StepDev is added there too
+static_assert(sizeof(StepDev) == 9 * sizeof(double) + 7 * sizeof(int32_t) + 4,
+              "StepDev changed: update read_step");
 __device__ __forceinline__ StepDev read_step(const ConstStep *p) {
   StepDev st;
   st.dr = p->dr;
@@ -283,6 +322,7 @@ __device__ __forceinline__ StepDev read_step(const ConstStep *p) {
   st.init_sm = p->init_sm;
   st.doy = p->doy;
   st.meteo_same = p->meteo_same;
+  st.wrow = p->wrow;
   return st;
 }
 
@@ -385,15 +425,12 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
   FluxRegs<NH> x;
   load_params<NH, JV>(a, bv, r);
   load_states<NH>(a, bv, r);
-  const double *F = a.F + (int64_t)st.day * a.sstride;
-  const int64_t fv = a.forcing_days * a.sstride;
-  const double pre = ldr(F + MHMHIP_F_PRE * fv, bc), tavg = ldr(F + MHMHIP_F_TAVG * fv, bc);
-  const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, bc) : 0.0;
-  const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, bc) : 0.0;
+  const double pre = ldr(frow_ptr(a, MHMHIP_F_PRE, st.day), bc);
+  const double tavg = ldr(frow_ptr(a, MHMHIP_F_TAVG, st.day), bc);
+  const double tmax = a.pet_case == 1 ? ldr(frow_ptr(a, MHMHIP_F_TMAX, st.day), bc) : 0.0;
+  const double tmin = a.pet_case == 1 ? ldr(frow_ptr(a, MHMHIP_F_TMIN, st.day), bc) : 0.0;
   const double pd = pet_day<NH>(a, st, bv, bc, tavg, tmax, tmin);
-  x.pet = a.hourly ? pd : disagg_flux(a.nT, a.rnT, pd, st.f_pet);
-  x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
-  x.prec = a.hourly ? pre : disagg_flux(a.nT, a.rnT, pre, st.f_prec);
+  disagg_step<NH>(a, st, bc, pd, tavg, pre, x.pet, x.temp, x.prec);
   if (st.init_sm) {
 #pragma unroll
     for (int h = 0; h < NH; ++h) r.sm[h] = 0.5 * r.fc[h];
@@ -404,7 +441,7 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
   store_fluxes<NH>(a, bv, x);
   if (OUT) add_outputs<NH>(a, bv, r, x);
   if (a.ring) {
-    const double q = runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst);
+    const double q = a.raw_total ? x.total : runoff_acc(x.total, ldr(a.area, bc), a.tst, a.rtst);
     const int32_t g = a.lag ? a.lag[c] : 0;
     for (int32_t i = 0; i < a.ring_rep; ++i)
       a.ring[ring_row(t * a.ring_rep + i, g, a.ring_len) * a.ld + bv / 8u] = q;
@@ -456,7 +493,6 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
     s_lane[lane0 + LS_STRESS + 3 * NH + h] = r.rfw[h];
   }
   const double area = a.ring ? ldr(a.area, bc) : 0.0;
-  const int64_t fv = a.forcing_days * a.sstride;
   int cached_day = -1, cached_doy = -1, cached_month = -1;
   const ConstStep *steps = (const ConstStep *)(a.steps + step0);
   // ring: the lane's row offset (its lag) is fixed, the step's row uniform
@@ -482,11 +518,12 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
       // offsets opaque here: no per-lane address of these rows is held
       // across the time loop
       const uint32_t oc = opaque(bc), ov = opaque(bv);
-      const double *F = a.F + (int64_t)st.day * a.sstride;
-      const double pre = ldr(F + MHMHIP_F_PRE * fv, oc);
-      const double tavg = ldr(F + MHMHIP_F_TAVG * fv, oc);
-      const double tmax = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMAX * fv, oc) : 0.0;
-      const double tmin = a.pet_case == 1 ? ldr(F + MHMHIP_F_TMIN * fv, oc) : 0.0;
+      const double pre = ldr(frow_ptr(a, MHMHIP_F_PRE, st.day), oc);
+      const double tavg = ldr(frow_ptr(a, MHMHIP_F_TAVG, st.day), oc);
+      const double tmax =
+          a.pet_case == 1 ? ldr(frow_ptr(a, MHMHIP_F_TMAX, st.day), oc) : 0.0;
+      const double tmin =
+          a.pet_case == 1 ? ldr(frow_ptr(a, MHMHIP_F_TMIN, st.day), oc) : 0.0;
       s_lane[li + LS_PRE] = pre;
       s_lane[li + LS_TAVG] = tavg;
       s_lane[li + LS_PD] = pet_day<NH>(a, st, ov, oc, tavg, tmax, tmin);
@@ -505,9 +542,7 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
       // day or night factors changed (or a new day): disaggregate again
       const double pd = s_lane[li + LS_PD], tavg = s_lane[li + LS_TAVG];
       const double pre = s_lane[li + LS_PRE];
-      x.pet = a.hourly ? pd : disagg_flux(a.nT, a.rnT, pd, st.f_pet);
-      x.temp = a.hourly ? tavg : disagg_state_add(a.nT, tavg, st.f_temp);
-      x.prec = a.hourly ? pre : disagg_flux(a.nT, a.rnT, pre, st.f_prec);
+      disagg_step<NH>(a, st, opaque(bc), pd, tavg, pre, x.pet, x.temp, x.prec);
       s_lane[li + LS_XPET] = x.pet;
       s_lane[li + LS_XTEMP] = x.temp;
       s_lane[li + LS_XPREC] = x.prec;
@@ -545,7 +580,7 @@ void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
     for (int h = 0; h < NH; ++h) pwh.v[h] = pw.v[h + 1];
     if (a.ring) {
       // the step's qOUT for each of its routing sub-steps (rows in routing steps)
-      const double q = runoff_acc(x.total, area, a.tst, a.rtst);
+      const double q = a.raw_total ? x.total : runoff_acc(x.total, area, a.tst, a.rtst);
       for (int32_t i = 0; i < a.ring_rep; ++i) {
         rlane[roff] = q;
         roff += a.ld;
@@ -952,7 +987,7 @@ hipError_t launch_meteo_l2(const double *data2, const uint8_t *mask2, int32_t nr
 // device order: F[v][col(j)][pos[cell]], col(j) = (day0 + j) mod n_cols.  One
 // thread per (cell, day of the chunk): the reads are coalesced, the writes
 // follow the tile layout (cells of a tile close together).
-__global__ __launch_bounds__(256) void forcing_gather_kernel(const double *raw, int32_t var_mask,
+__global__ __launch_bounds__(256) void forcing_gather_kernel(const double *raw, FRowMap m,
                                                              int32_t ndays, int64_t n_cells,
                                                              const int32_t *pos, double *F,
                                                              int32_t n_cols, int32_t day0,
@@ -962,23 +997,59 @@ __global__ __launch_bounds__(256) void forcing_gather_kernel(const double *raw, 
   if (c >= n_cells || j >= ndays) return;
   const int64_t p = pos[c];
   const int64_t col = (int64_t)((day0 + j) % n_cols);
-  int32_t k = 0;
-  for (int32_t v = 0; v < MHMHIP_F_COUNT; ++v) {
-    if (!((var_mask >> v) & 1)) continue;
-    F[((int64_t)v * n_cols + col) * sstride + p] = raw[((int64_t)k * ndays + j) * n_cells + c];
-    ++k;
-  }
+  for (int32_t k = 0; k < m.n; ++k)
+    F[((int64_t)m.row[k] * n_cols + col) * sstride + p] = raw[((int64_t)k * ndays + j) * n_cells + c];
 }
 
-hipError_t launch_forcing_gather(const double *raw, int32_t var_mask, int32_t ndays,
+hipError_t launch_forcing_gather(const double *raw, const FRowMap &m, int32_t ndays,
                                  int64_t n_cells, const int32_t *pos, double *F, int32_t n_cols,
                                  int32_t day0, int64_t sstride, hipStream_t s) {
-  if (var_mask <= 0 || var_mask >= (1 << MHMHIP_F_COUNT) || ndays < 1 || ndays > 65535 ||
-      n_cells < 1 || n_cols < 1 || day0 < 0)
+  if (m.n < 1 || m.n > MHMHIP_F_COUNT || ndays < 1 || ndays > 65535 || n_cells < 1 ||
+      n_cols < 1 || day0 < 0)
     return hipErrorInvalidValue;
+  for (int32_t k = 0; k < m.n; ++k)
+    if (m.row[k] < 0 || m.row[k] >= MHMHIP_F_COUNT) return hipErrorInvalidValue;
   const dim3 grid((unsigned)((n_cells + 255) / 256), (unsigned)ndays);
-  hipLaunchKernelGGL(forcing_gather_kernel, grid, dim3(256), 0, s, raw, var_mask, ndays, n_cells,
-                     pos, F, n_cols, day0, sstride);
+  hipLaunchKernelGGL(forcing_gather_kernel, grid, dim3(256), 0, s, raw, m, ndays, n_cells, pos, F,
+                     n_cols, day0, sstride);
+  return hipGetLastError();
+}
+
+// L11_runoff_acc with map_flag on a routing grid coarser than the L1 grid
+// (mo_mrm_pre_routing.f90:110-126): qAcc(node) = 0, plus qAll(k) * area(k)
+// for the node's L1 cells k in increasing reference order, then
+// * 1000 / TST.  The cells of device node p are device cells
+// [cstart[p], cstart[p + 1]) (grouped by node, reference order inside).  tot
+// holds tot_rows rows of L1_total_runoff (row (t0 + j) mod tot_rows for step
+// t0 + j); the step's qOUT goes to its rep ring rows at the node's lag, as the
+// cell kernels write them on an L1 = L11 grid.  One thread per (node, set),
+// blockIdx.z the step.
+__global__ __launch_bounds__(256) void runoff_agg_kernel(
+    const double *tot, int64_t tot_rows, const int32_t *cstart, const double *area,
+    int64_t n_nodes, int64_t sstride, int64_t ld, double *ring, const uint8_t *lag,
+    int64_t ring_len, int32_t rep, int64_t t0, double tst, double rtst) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_nodes) return;
+  const int64_t set0 = (int64_t)blockIdx.y * sstride;
+  const int64_t t = t0 + blockIdx.z;
+  const double *src = tot + (t % tot_rows) * ld + set0;
+  double sum = 0.0;
+  for (int32_t c = cstart[p]; c < cstart[p + 1]; ++c) sum = sum + src[c] * area[c];
+  const double q = div_rcp(sum * 1000.0, tst, rtst);
+  const int32_t g = lag ? lag[p] : 0;
+  for (int32_t i = 0; i < rep; ++i) ring[ring_row(t * rep + i, g, ring_len) * ld + set0 + p] = q;
+}
+
+hipError_t launch_runoff_agg(const double *tot, int64_t tot_rows, const int32_t *cstart,
+                             const double *area, int64_t n_nodes, int64_t sstride, int64_t ld,
+                             int n_sets, double *ring, const uint8_t *lag, int64_t ring_len,
+                             int32_t rep, int64_t t0, int32_t nsteps, double tst, hipStream_t s) {
+  if (n_nodes < 1 || n_sets < 1 || nsteps < 1 || nsteps > 65535 || tot_rows < 1 || ring_len < 1 ||
+      rep < 1 || !cstart || !tot || !ring || !(tst > 0.0))
+    return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((n_nodes + 255) / 256), (unsigned)n_sets, (unsigned)nsteps);
+  hipLaunchKernelGGL(runoff_agg_kernel, grid, dim3(256), 0, s, tot, tot_rows, cstart, area,
+                     n_nodes, sstride, ld, ring, lag, ring_len, rep, t0, tst, 1.0 / tst);
   return hipGetLastError();
 }
 

@@ -21,12 +21,16 @@ hipError_t launch_inflow_patch(const RouteArgs &r, double *ring, const uint8_t *
 hipError_t launch_meteo_l2(const double *data2, const uint8_t *mask2, int32_t nr2, int64_t n2,
                            const MeteoL2Cell *cells, int64_t ncells, int32_t ndays, double *F,
                            int64_t day_stride, bool aggregate, hipStream_t s);
-hipError_t launch_forcing_gather(const double *raw, int32_t var_mask, int32_t ndays,
+hipError_t launch_forcing_gather(const double *raw, const FRowMap &m, int32_t ndays,
                                  int64_t n_cells, const int32_t *pos, double *F, int32_t n_cols,
                                  int32_t day0, int64_t sstride, hipStream_t s);
 hipError_t launch_runoff_sum(const double *total, double *acc, double *ring, const double *area,
                              const uint8_t *lag, int64_t n_cells, int64_t sstride, int64_t ld,
                              int n_sets, double tst, bool route, hipStream_t s);
+hipError_t launch_runoff_agg(const double *tot, int64_t tot_rows, const int32_t *cstart,
+                             const double *area, int64_t n_nodes, int64_t sstride, int64_t ld,
+                             int n_sets, double *ring, const uint8_t *lag, int64_t ring_len,
+                             int32_t rep, int64_t t0, int32_t nsteps, double tst, hipStream_t s);
 hipError_t launch_route_wg(const RouteArgs &r, const int64_t *dom_start, const int32_t *dom_dmax,
                            int n_domains, int64_t d_lo, int64_t d_hi, int drain, int block,
                            hipStream_t s);

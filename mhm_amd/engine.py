@@ -150,9 +150,22 @@ class Model:
             self._ctx, N.F[name], int(domain), int(day0), nd, N.dptr(d2), nr2, nc2,
             float(cellsize2), m2.ctypes.data, nr1, nc1, float(cellsize1), m1.ctypes.data))
 
+    def set_meteo_weights(self, pre_w=None, temp_w=None, pet_w=None):
+        """Disaggregation by weights (read_meteo_weights): each (n_cells, 12, 24)
+        = L1_x_weights(cell, month, hour + 1) of all packed cells; no
+        arguments: back to the day/night factors."""
+        if pre_w is None:
+            self._chk(N.lib().mhmhip_set_meteo_weights(self._ctx, None, None, None, self.n_cells))
+            return
+        # Fortran X(ld, 12, 24): cell fastest, then month, then hour
+        ws = [np.asfortranarray(np.asarray(w, dtype=np.float64).reshape(self.n_cells, 12, 24))
+              for w in (pre_w, temp_w, pet_w)]
+        self._chk(N.lib().mhmhip_set_meteo_weights(
+            self._ctx, *[w.ctypes.data_as(N._dp) for w in ws], self.n_cells))
+
     def set_domain_forcing(self, day0=1):
         """Upload every domain's own forcing (synth.make_forcing) for all packed cells."""
-        for name in ("PRE", "TAVG", "TMIN", "TMAX"):
+        for name in ("PRE", "TAVG", "TMIN", "TMAX", "NETRAD", "ABSVAPPRESS", "WINDSPEED"):
             if all(name in d.forcing for d in self.domains):
                 data = np.concatenate([d.forcing[name] for d in self.domains], axis=1)
                 self.set_forcing(name, data, day0)
@@ -161,18 +174,22 @@ class Model:
     def bind_network(self, inflow=None):
         doms = self.domains
         cat = lambda f, dt: np.ascontiguousarray(np.concatenate([f(d) for d in doms]).astype(dt))
+        self.node_start = np.zeros(len(doms) + 1, dtype=np.int64)
+        self.node_start[1:] = np.cumsum([d.n_nodes for d in doms])
+        self.n_nodes = int(self.node_start[-1])
         self._net_arrays = arrs = dict(
-            start=np.ascontiguousarray(self.cell_start),
+            start=np.ascontiguousarray(self.node_start),
             nout=np.array([d.n_outlets for d in doms], np.int32),
             netPerm=cat(lambda d: d.netPerm, np.int32), fromN=cat(lambda d: d.fromN, np.int32),
             toN=cat(lambda d: d.toN, np.int32), length=cat(lambda d: d.length, np.float64),
             slope=cat(lambda d: d.slope, np.float64), fFPimp=cat(lambda d: d.fFPimp, np.float64),
             area=cat(lambda d: d.area_km2, np.float64),
-            l1l11=cat(lambda d: np.arange(1, d.n_cells + 1), np.int32),
+            l1l11=cat(lambda d: np.arange(1, d.n_cells + 1) if d.l1_l11 is None else d.l1_l11,
+                      np.int32),
             gdom=cat(lambda d: np.full(d.gauge_nodes.shape[0], doms.index(d)), np.int32),
             gnode=cat(lambda d: d.gauge_nodes, np.int32))
         net = N.Network()
-        net.n_nodes = self.n_cells
+        net.n_nodes = self.n_nodes
         net.domain_node_start = N.lptr(arrs["start"])
         net.n_outlets = N.iptr(arrs["nout"])
         net.netPerm = N.iptr(arrs["netPerm"])
@@ -217,7 +234,7 @@ class Model:
         self._chk(N.lib().mhmhip_set_inflow(self._ctx, q.shape[0], N.dptr(q)))
 
     def routing_state(self, set_=0):
-        n = self.n_cells
+        n = self.n_nodes
         nl = self._net_arrays["netPerm"].shape[0]
         out = {k: np.zeros(n) for k in ("qOUT", "qTIN", "qTR", "Qmod")}
         out["C1"] = np.zeros(max(nl, 1))
@@ -241,7 +258,7 @@ class Model:
 
     def qmod_sum(self, set_=0):
         """(per-node Qmod sum in reference node order, steps added)."""
-        out = np.zeros(self.n_cells)
+        out = np.zeros(self.n_nodes)
         cnt = C.c_int64()
         self._chk(N.lib().mhmhip_get_qmod_sum(self._ctx, set_, N.dptr(out), C.byref(cnt)))
         return out, cnt.value

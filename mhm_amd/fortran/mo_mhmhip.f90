@@ -20,14 +20,15 @@ module mo_mhmhip
     P_KPERCO = 10, P_ALPHA = 11, P_KARSTLOSS = 12, P_UNSAT_THRESH = 13, P_FASP = 14, &
     P_HARSAMCOEFF = 15, P_LATITUDE = 16, P_SOILMOIST_SAT = 17, P_FROOTS = 18, &
     P_SOILMOIST_FC = 19, P_WILTINGPOINT = 20, P_SOILMOIST_EXP = 21, P_MAXINTER = 22, &
-    P_PETLAICOR = 23, &
+    P_PETLAICOR = 23, P_PRIETAYALPHA = 24, P_AERORESIST = 25, P_SURFRESIST = 26, &
     S_INTER = 32, S_SNOWPACK = 33, S_SEALSTW = 34, S_UNSATSTW = 35, S_SATSTW = 36, &
     S_SOILMOIST = 37, &
     X_PET_CALC = 64, X_TEMP_CALC = 65, X_PREC_CALC = 66, X_THROUGHFALL = 67, X_AETCANOPY = 68, &
     X_DEGDAY = 69, X_MELT = 70, X_PREEFFECT = 71, X_RAIN = 72, X_SNOW = 73, X_RUNOFFSEAL = 74, &
     X_AETSEALED = 75, X_FASTRUNOFF = 76, X_SLOWRUNOFF = 77, X_PERCOL = 78, X_BASEFLOW = 79, &
     X_TOTAL_RUNOFF = 80, X_INFILSOIL = 81, X_AETSOIL = 82
-  integer(c_int32_t), parameter, public :: F_PRE = 0, F_TAVG = 1, F_TMIN = 2, F_TMAX = 3, F_PET = 4
+  integer(c_int32_t), parameter, public :: F_PRE = 0, F_TAVG = 1, F_TMIN = 2, F_TMAX = 3, &
+    F_PET = 4, F_NETRAD = 5, F_ABSVAPPRESS = 6, F_WINDSPEED = 7, F_COUNT = 8
 
   type, bind(C), public :: mhmhip_config
     integer(c_int64_t) :: n_cells
@@ -64,6 +65,7 @@ module mo_mhmhip
   public :: mhmhip_create, mhmhip_destroy, mhmhip_last_error, mhmhip_device_bytes, mhmhip_sync, &
             mhmhip_set_cell_var, mhmhip_get_cell_var, mhmhip_default_states, &
             mhmhip_set_forcing, mhmhip_stream_forcing, mhmhip_set_forcing_l2, &
+            mhmhip_set_meteo_weights, &
             mhmhip_set_routing_celerity, mhmhip_set_routing_celerity_links, &
             mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, &
             mhmhip_set_routing_params, mhmhip_set_inflow, mhmhip_get_routing_state, &
@@ -119,7 +121,13 @@ module mo_mhmhip
       import :: c_int, c_ptr, c_int32_t, c_int64_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: day0, ndays
-      type(c_ptr), intent(in) :: host(5)
+      type(c_ptr), intent(in) :: host(8)
+      integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function mhmhip_set_meteo_weights(ctx, pre_w, temp_w, pet_w, ld) &
+        bind(C, name="mhmhip_set_meteo_weights")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, pre_w, temp_w, pet_w
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function mhmhip_set_forcing(ctx, var, day0, ndays, host, ld) &

@@ -85,6 +85,12 @@ LAI = np.stack([
 ])
 
 
+def lai_l1(fractions):
+    """(n, 12) monthly leaf area index of cells with these cover fractions."""
+    fF, fI, fP = fractions
+    return fF[:, None] * LAI[0] + fI[:, None] * LAI[1] + fP[:, None] * LAI[2]
+
+
 def mpr_lite(g, fractions, texture, aspect, horizon_depth):
     """L1 effective parameter fields for one gamma vector.
 
@@ -113,7 +119,7 @@ def mpr_lite(g, fractions, texture, aspect, horizon_depth):
     # iper_thres_runoff (883-894)
     out["SEALED_THRESH"] = np.full(n, G("imperviousStorageCapacity"))
     # interception: LAI * factor
-    lai = fF[:, None] * LAI[0] + fI[:, None] * LAI[1] + fP[:, None] * LAI[2]
+    lai = lai_l1(fractions)
     out["MAXINTER"] = G("canopyInterceptionFactor") * lai
     out["PETLAICOR"] = np.clip(0.6 + 0.08 * lai, 0.3, 1.3)
     # soil hydraulics per horizon
@@ -156,4 +162,21 @@ def mpr_lite(g, fractions, texture, aspect, horizon_depth):
     out["FASP"] = np.where(aspect < thr, mn + mx * (thr - aspect) / thr,
                            mn + mx * (aspect - thr) / (360.0 - thr))
     out["HARSAMCOEFF"] = np.full(n, G("HargreavesSamaniCoeff"))
+    return out
+
+
+def pet_radiation_params(lai, pet_case):
+    """Monthly PET parameters of the radiation-based cases, stand-ins shaped
+    like the reference's fields: Priestley-Taylor's alpha (L1_PrieTayAlpha,
+    process_pet 2) around 1.26; Penman-Monteith's aerodynamic resistance
+    per unit wind speed (L1_aeroResist, divided by the day's wind speed in
+    get_corrected_pet) around 208 s m-1 and bulk surface resistance
+    (L1_surfResist) around 70 s m-1, both falling with the leaf area.
+    lai: (n, 12) monthly leaf area index."""
+    out = {}
+    if pet_case == 2:
+        out["PRIETAYALPHA"] = 1.05 + 0.06 * lai
+    elif pet_case == 3:
+        out["AERORESIST"] = 208.0 / (0.6 + 0.1 * lai)
+        out["SURFRESIST"] = 40.0 + 120.0 / (1.0 + lai)
     return out

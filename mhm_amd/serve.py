@@ -61,15 +61,20 @@ TILE_WINDOW_MAX = 64  # longest routing window; boundary series are allocated at
 TILE_HEIGHT = 16  # levels per routing tile; the runoff ring has this many rows past its length
 
 
+STEP_BYTES = 104  # one step of the device step table (StepDev)
+# forcing rows a context holds per PET case (pre, tavg and what the case reads)
+FORCING_ROWS = {-1: 3, 0: 3, 1: 4, 2: 3, 3: 5}
+
+
 def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_steps, n_steps,
-                   n_gauges=1, n_outlets=1):
+                   n_gauges=1, n_outlets=1, process_pet=1):
     """Device footprint of one context run in chunks of `chunk_steps` fused
     steps, allocation by allocation as api.hip makes them; `plan` is the
     network's mhmhip_plan (N.plan_network)."""
     sstride = (n_cells + 63) // 64 * 64
     ld = sstride * n_sets
-    rows = (19 + 5 * n_horizons + 24) + (5 + n_horizons) + (17 + 2 * n_horizons)
-    cells = rows * ld * 8 + 5 * n_forcing_days * sstride * 8
+    rows = (19 + 5 * n_horizons + 60) + (5 + n_horizons) + (17 + 2 * n_horizons)
+    cells = rows * ld * 8 + FORCING_ROWS[process_pet] * n_forcing_days * sstride * 8
     window = 8  # default routing window (mhmhip_run_opts)
     rows_run = min(chunk_steps, n_steps)  # steps of the longest call
     # ring rows of a call: (tile depth + 1) windows + S + 1, at most the
@@ -80,7 +85,7 @@ def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_step
                + 20 * plan.n_slots + 16 * plan.n_bins
                + 2 * n_sets * max(1, plan.n_bnd) * TILE_WINDOW_MAX * 8
                + 4 * max(1, n_gauges) + 8)
-    series = rows_run * 96 + rows_run * n_sets * max(1, n_gauges) * 8
+    series = rows_run * STEP_BYTES + rows_run * n_sets * max(1, n_gauges) * 8
     return cells + ring + routing + series
 
 

@@ -55,10 +55,18 @@ class Domain:
     n_outlets: int = 1
     forcing: dict = field(default_factory=dict)
     n_days: int = 0
+    # routing on a coarser L11 grid: the L11 node (1-based) of every L1 cell
+    # (L1_L11_Id) and the node count; None: L1 = L11
+    l1_l11: np.ndarray = None
+    nodes_l11: int = 0
 
     @property
     def n_cells(self):
         return self.nrows * self.ncols
+
+    @property
+    def n_nodes(self):
+        return self.nodes_l11 if self.l1_l11 is not None else self.n_cells
 
     @property
     def n_links(self):
@@ -132,6 +140,35 @@ def make_domain(nrows, ncols, seed, cellsize=1000.0, horizon_depth=HORIZON_DEPTH
                   gauge_nodes=np.array([outlet + 1], np.int32))
 
 
+def refine_domain(dom11, f, seed):
+    """A basin whose hydrology (L1) grid is f x f times finer than the routing
+    grid of dom11 (L11, resolution_Hydrology = resolution_Routing / f as in
+    the reference's check case_04): the network, gauges and links of dom11,
+    L1 land cover, soils, terrain aspect and latitude on the fine grid, and
+    L1_L11_Id mapping fine cell (i, j) to coarse node (i // f, j // f)
+    (column-major numbering, 1-based)."""
+    rng = np.random.default_rng(seed)
+    nr, nc = dom11.nrows * f, dom11.ncols * f
+    n = nr * nc
+    ii, jj = np.meshgrid(np.arange(nr), np.arange(nc), indexing="ij")
+    i = ii.ravel(order="F")
+    j = jj.ravel(order="F")
+    l1_l11 = ((j // f) * dom11.nrows + (i // f) + 1).astype(np.int32)
+    cls = rng.integers(0, 3, size=(n, 4))
+    fractions = np.stack([(cls == c).mean(axis=1) for c in range(3)])
+    texture = np.clip(_fbm(rng, nr, nc, 0.5).ravel(order="F"), 0.0, 0.999999)
+    aspect = rng.uniform(0.0, 360.0, n)
+    cs = dom11.cellsize / f
+    lat = dom11.lat[l1_l11 - 1] + ((i % f) - (f - 1) / 2.0) * cs / 111000.0
+    return Domain(nrows=nr, ncols=nc, cellsize=cs, seed=seed, horizon_depth=dom11.horizon_depth,
+                  lat=lat, fractions=fractions, texture=texture, aspect=aspect,
+                  elev=dom11.elev[l1_l11 - 1], fdir=dom11.fdir, fromN=dom11.fromN, toN=dom11.toN,
+                  netPerm=dom11.netPerm, rOrder=dom11.rOrder, length=dom11.length,
+                  slope=dom11.slope, fFPimp=dom11.fFPimp, area_km2=np.full(n, cs * cs * 1.0e-6),
+                  depth=dom11.depth, gauge_nodes=dom11.gauge_nodes, n_outlets=dom11.n_outlets,
+                  l1_l11=l1_l11, nodes_l11=dom11.n_cells)
+
+
 def make_forcing(dom, n_days, seed, doy0=1):
     """Daily pre/tavg/tmin/tmax, each (n_days, n_cells) = Fortran L1_x(n_cells, n_days)."""
     rng = np.random.default_rng(seed)
@@ -147,8 +184,46 @@ def make_forcing(dom, n_days, seed, doy0=1):
     return dom.forcing
 
 
-def cell_params(dom, gamma):
-    """L1 effective parameters of one gamma vector (MPR-lite) plus latitude."""
+def cell_params(dom, gamma, pet_case=1):
+    """L1 effective parameters of one gamma vector (MPR-lite) plus latitude;
+    pet_case 2 or 3 adds that case's monthly PET parameters."""
     p = PM.mpr_lite(gamma, dom.fractions, dom.texture, dom.aspect, dom.horizon_depth)
+    p.update(PM.pet_radiation_params(PM.lai_l1(dom.fractions), pet_case))
     p["LATITUDE"] = dom.lat.copy()
     return p
+
+
+def make_pet_forcing(dom, seed, doy0=1):
+    """The radiation-based PET cases' daily inputs beside make_forcing's
+    (same days): net radiation [W m-2] = 70 + 90 sin(2 pi (doy - 80)/365) +
+    N(0, 25), negative on some winter days; actual vapour pressure [Pa] =
+    RH * 1000 * 0.6108 exp(17.27 tavg / (tavg + 237.3)) with RH ~ U(0.35, 0.95);
+    wind speed [m s-1] ~ 0.3 + Gamma(k=2, theta=1.2)."""
+    rng = np.random.default_rng(seed)
+    n, nd = dom.n_cells, dom.n_days
+    doy = (doy0 - 1 + np.arange(nd)) % 365 + 1
+    rn = 70.0 + 90.0 * np.sin(2.0 * np.pi * (doy - 80) / 365.0)[:, None] + rng.normal(0.0, 25.0, (nd, n))
+    tavg = dom.forcing["TAVG"]
+    es = 0.6108 * np.exp(17.27 * tavg / (tavg + 237.3))
+    avp = rng.uniform(0.35, 0.95, (nd, n)) * 1000.0 * es
+    ws = 0.3 + rng.gamma(2.0, 1.2, (nd, n))
+    dom.forcing.update(NETRAD=np.ascontiguousarray(rn), ABSVAPPRESS=np.ascontiguousarray(avp),
+                       WINDSPEED=np.ascontiguousarray(ws))
+    return dom.forcing
+
+
+def make_meteo_weights(n_cells, seed):
+    """Per-cell meteo weights (n_cells, 12, 24) for pre, temp and pet in the
+    reference's sense (L1_x_weights(cell, month, hour + 1)): precipitation
+    and PET weights sum to 1 over the day's hours (PET peaking at noon), the
+    temperature weights scale the absolute temperature around 1."""
+    rng = np.random.default_rng(seed)
+    h = np.arange(24)
+    pre = rng.gamma(1.0, 1.0, (n_cells, 12, 24))
+    pre /= pre.sum(axis=2, keepdims=True)
+    shape = np.maximum(np.sin(np.pi * (h - 5) / 15.0), 0.02)
+    pet = shape[None, None, :] * rng.uniform(0.8, 1.2, (n_cells, 12, 24))
+    pet /= pet.sum(axis=2, keepdims=True)
+    temp = 1.0 + 0.012 * np.sin(2.0 * np.pi * (h - 9) / 24.0)[None, None, :] \
+        + rng.normal(0.0, 0.002, (n_cells, 12, 24))
+    return pre, temp, pet

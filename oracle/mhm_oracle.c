@@ -42,6 +42,7 @@ static const double TETENS_C2 = 17.270;
 static const double TETENS_C3 = 237.30;
 static const double SATPRESSURESLOPE1 = 4098.0;
 static const double HARSAMCONST = 17.800;
+static const double T0_DP = 273.15; /* FORCES mo_constants: T0_dp */
 
 /* ---------------------------------------------------------------------------
  * src/mHM/mo_canopy_interc.f90:75-133
@@ -390,7 +391,7 @@ double or_disagg_state_daynight(int isday, double ntimesteps_day, double v_day, 
 /* ---------------------------------------------------------------------------
  * row-layout helpers
  */
-int or_n_param_rows(int nH) { return OR_P_NSCALAR + 5 * nH + 24; }
+int or_n_param_rows(int nH) { return OR_P_NSCALAR + 5 * nH + 60; }
 int or_n_state_rows(int nH) { return OR_S_NSCALAR + nH; }
 int or_n_flux_rows(int nH) { return OR_X_NSCALAR + 2 * nH; }
 
@@ -418,6 +419,11 @@ void or_mhm_step(const or_cells_cfg *cfg, const or_step_info *st, const double *
   const double *f_tmin = forcing + ((int64_t)OR_F_TMIN * cfg->forcing_days + day) * fld;
   const double *f_tmax = forcing + ((int64_t)OR_F_TMAX * cfg->forcing_days + day) * fld;
   const double *f_pet = forcing + ((int64_t)OR_F_PET * cfg->forcing_days + day) * fld;
+  const double *f_rn = forcing + ((int64_t)OR_F_NETRAD * cfg->forcing_days + day) * fld;
+  const double *f_avp = forcing + ((int64_t)OR_F_ABSVAPPRESS * cfg->forcing_days + day) * fld;
+  const double *f_ws = forcing + ((int64_t)OR_F_WINDSPEED * cfg->forcing_days + day) * fld;
+  /* L1_x_weights(cell, month, hour + 1), here [var][month * 24 + hour][cell] */
+  const double *w_pre = NULL, *w_temp = NULL, *w_pet = NULL;
   /* mo_meteo_handler.f90:404-406 */
   const double fday_prec = 1.0 - cfg->fnight_prec[m];
   const double fday_pet = 1.0 - cfg->fnight_pet[m];
@@ -432,9 +438,16 @@ void or_mhm_step(const or_cells_cfg *cfg, const or_step_info *st, const double *
   const int pet_case = cfg->process_pet;
   const int soil_case = cfg->process_soil;
   const int rSat = OR_P_NSCALAR, rFr = rSat + nH, rFC = rFr + nH, rWP = rFC + nH,
-            rExp = rWP + nH, rMaxI = rExp + nH, rLAIc = rMaxI + 12;
+            rExp = rWP + nH, rMaxI = rExp + nH, rLAIc = rMaxI + 12, rPT = rLAIc + 12,
+            rAero = rPT + 12, rSurf = rAero + 12;
   double *pet_calc = XROW(OR_X_PET), *temp_calc = XROW(OR_X_TEMP), *prec_calc = XROW(OR_X_PREC);
   int64_t k;
+  if (cfg->meteo_weights) {
+    const int64_t wr = (int64_t)m * 24 + st->hour;
+    w_pre = cfg->meteo_weights + wr * fld;
+    w_temp = cfg->meteo_weights + ((int64_t)288 + wr) * fld;
+    w_pet = cfg->meteo_weights + ((int64_t)576 + wr) * fld;
+  }
 #ifdef _OPENMP
   int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #else
@@ -453,11 +466,18 @@ void or_mhm_step(const or_cells_cfg *cfg, const or_step_info *st, const double *
                               f_tmin[k], PROW(OR_P_LAT)[k], st->doy);
     } else if (pet_case == 0) {
       pet = PROW(OR_P_FASP)[k] * f_pet[k];
+    } else if (pet_case == 2) { /* Priestley-Taylor (1078-1083), iLAI = month */
+      pet = or_pet_priestly(PROW(rPT + m)[k], fmax(f_rn[k], 0.0), f_tavg[k]);
+    } else if (pet_case == 3) { /* Penman-Monteith (1085-1095) */
+      pet = or_pet_penman(fmax(f_rn[k], 0.0), f_tavg[k], f_avp[k] / 1000.0,
+                          PROW(rAero + m)[k] / f_ws[k], PROW(rSurf + m)[k], 1.0, 1.0);
     } else { /* -1: LAI-driven correction, iLAI = month */
       pet = PROW(rLAIc + m)[k] * f_pet[k];
     }
     if (hourly) {
       pet_calc[k] = pet;
+    } else if (w_pet) { /* read_meteo_weights (1102-1107) */
+      pet_calc[k] = or_disagg_meteo_weights(pet, w_pet[k], 0.0);
     } else {
       pet_calc[k] = or_disagg_flux_daynight(isday, nT, pet, fday_pet, fnight_pet);
     }
@@ -467,17 +487,19 @@ void or_mhm_step(const or_cells_cfg *cfg, const or_step_info *st, const double *
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
   for (k = 0; k < n; ++k) {
-    temp_calc[k] = hourly ? f_tavg[k]
-                          : or_disagg_state_daynight(isday, nT, f_tavg[k], fday_temp,
-                                                     fnight_temp, 1);
+    temp_calc[k] = hourly   ? f_tavg[k]
+                   : w_temp ? or_disagg_meteo_weights(f_tavg[k], w_temp[k], T0_DP) /* 1183-1189 */
+                            : or_disagg_state_daynight(isday, nT, f_tavg[k], fday_temp,
+                                                       fnight_temp, 1);
   }
   /* get_prec (1256-1283) */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
 #endif
   for (k = 0; k < n; ++k) {
-    prec_calc[k] = hourly ? f_pre[k]
-                          : or_disagg_flux_daynight(isday, nT, f_pre[k], fday_prec, fnight_prec);
+    prec_calc[k] = hourly  ? f_pre[k]
+                   : w_pre ? or_disagg_meteo_weights(f_pre[k], w_pre[k], 0.0) /* 1265-1270 */
+                           : or_disagg_flux_daynight(isday, nT, f_pre[k], fday_prec, fnight_prec);
   }
 
   /* mHM(): initial soil moisture (mo_mhm.f90:448-450) */

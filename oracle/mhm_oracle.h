@@ -79,10 +79,14 @@ double or_disagg_state_daynight(int isday, double ntimesteps_day, double v_day,
  * Row layout of the packed per-cell arrays (each row holds ld doubles):
  *   params : OR_P_* scalar rows, then 5 horizon blocks of nH rows
  *            (SAT, FROOTS, FC, WP, EXP), then 12 MAXINTER rows (month 1..12),
- *            then 12 PETLAICOR rows (month 1..12)
+ *            then 12 PETLAICOR, 12 PRIETAYALPHA, 12 AERORESIST and
+ *            12 SURFRESIST rows (month 1..12)
  *   states : OR_S_* rows, then nH SOILMOIST rows
  *   fluxes : OR_X_* rows, then nH INFIL rows, then nH AET_SOIL rows
- *   forcing: [var][day][cell], var in OR_F_*
+ *   forcing: [var][day][cell], var in OR_F_* (netrad W m-2, absvappress Pa,
+ *            windspeed m s-1)
+ *   meteo weights (optional): [var][month * 24 + hour][cell], var = pre,
+ *            temp, pet
  */
 enum {
   OR_P_FSEALED = 0, OR_P_DEGDAY_INCR, OR_P_DEGDAY_MAX, OR_P_DEGDAY_NOPRE, OR_P_TEMP_THRESH,
@@ -95,7 +99,8 @@ enum {
   OR_X_MELT, OR_X_PREC_EFFECT, OR_X_RAIN, OR_X_SNOW, OR_X_RUNOFF_SEALED, OR_X_AET_SEALED,
   OR_X_FAST, OR_X_SLOW, OR_X_PERC, OR_X_BASEFLOW, OR_X_TOTAL_RUNOFF, OR_X_NSCALAR
 };
-enum { OR_F_PRE = 0, OR_F_TAVG, OR_F_TMIN, OR_F_TMAX, OR_F_PET, OR_F_NVAR };
+enum { OR_F_PRE = 0, OR_F_TAVG, OR_F_TMIN, OR_F_TMAX, OR_F_PET, OR_F_NETRAD, OR_F_ABSVAPPRESS,
+       OR_F_WINDSPEED, OR_F_NVAR };
 
 int or_n_param_rows(int nH);
 int or_n_state_rows(int nH);
@@ -106,7 +111,7 @@ typedef struct {
   int64_t ld;           /* leading dimension of param/state/flux rows        */
   int32_t n_h;          /* soil horizons                                     */
   int32_t process_soil; /* processMatrix(3,1)                                */
-  int32_t process_pet;  /* processMatrix(5,1): -1, 0, 1                      */
+  int32_t process_pet;  /* processMatrix(5,1): -1, 0, 1, 2, 3                */
   int32_t hourly_forcing;
   double  c2TSTu;       /* timestep / 24                                     */
   double  nTstepDay;    /* 24 / timestep as real                             */
@@ -117,6 +122,7 @@ typedef struct {
   double fnight_prec[12];
   double fnight_pet[12];
   double fnight_temp[12];
+  const double *meteo_weights; /* NULL: day/night factors                    */
 } or_cells_cfg;
 
 typedef struct {

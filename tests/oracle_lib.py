@@ -37,7 +37,8 @@ class CellsCfg(C.Structure):
                 ("nTstepDay", C.c_double), ("forcing_ld", C.c_int64),
                 ("forcing_days", C.c_int32), ("pad", C.c_int32),
                 ("evap_coeff", C.c_double * 12), ("fnight_prec", C.c_double * 12),
-                ("fnight_pet", C.c_double * 12), ("fnight_temp", C.c_double * 12)]
+                ("fnight_pet", C.c_double * 12), ("fnight_temp", C.c_double * 12),
+                ("meteo_weights", C.c_void_p)]
 
 
 class StepInfo(C.Structure):

@@ -21,7 +21,11 @@ X_ROW = {"PET_CALC": 0, "TEMP_CALC": 1, "PREC_CALC": 2, "THROUGHFALL": 3, "AETCA
          "DEGDAY": 5, "MELT": 6, "PREEFFECT": 7, "RAIN": 8, "SNOW": 9, "RUNOFFSEAL": 10,
          "AETSEALED": 11, "FASTRUNOFF": 12, "SLOWRUNOFF": 13, "PERCOL": 14, "BASEFLOW": 15,
          "TOTAL_RUNOFF": 16}
-F_IDX = {"PRE": 0, "TAVG": 1, "TMIN": 2, "TMAX": 3, "PET": 4}
+F_IDX = {"PRE": 0, "TAVG": 1, "TMIN": 2, "TMAX": 3, "PET": 4, "NETRAD": 5, "ABSVAPPRESS": 6,
+         "WINDSPEED": 7}
+# monthly parameter blocks after the horizon blocks (offset in rows)
+MONTHLY = {"MAXINTER": 0, "PETLAICOR": 12, "PRIETAYALPHA": 24, "AERORESIST": 36,
+           "SURFRESIST": 48}
 
 
 class _DomSet:
@@ -73,14 +77,16 @@ class OracleSim:
                 nl = d.n_links
                 o.C1 = np.zeros(max(nl, 1))
                 o.C2 = np.zeros(max(nl, 1))
-                o.qOUT = np.zeros(n)
-                o.qTIN = np.zeros(2 * n)
-                o.qTR = np.zeros(2 * n)
-                o.Qmod = np.zeros(n)
+                nn = d.n_nodes
+                o.qOUT = np.zeros(nn)
+                o.qTIN = np.zeros(2 * nn)
+                o.qTR = np.zeros(2 * nn)
+                o.Qmod = np.zeros(nn)
                 o.gamma = np.zeros(5)
                 sets.append(o)
             self.ds.append(sets)
-            d._oracle_F = np.zeros((5, n_forcing_days, n))
+            d._oracle_F = np.zeros((len(F_IDX), n_forcing_days, n))
+            d._oracle_W = None
         self.gauge_series = []
 
     # -- data -------------------------------------------------------------------------
@@ -99,12 +105,9 @@ class OracleSim:
                 elif k in H_BLOCK:
                     for h in range(self.nh):
                         o.P[17 + H_BLOCK[k] * self.nh + h] = v[lo:hi, h]
-                elif k == "MAXINTER":
+                elif k in MONTHLY:
                     for m in range(12):
-                        o.P[17 + 5 * self.nh + m] = v[lo:hi, m]
-                elif k == "PETLAICOR":
-                    for m in range(12):
-                        o.P[17 + 5 * self.nh + 12 + m] = v[lo:hi, m]
+                        o.P[17 + 5 * self.nh + MONTHLY[k] + m] = v[lo:hi, m]
                 else:
                     raise KeyError(k)
 
@@ -119,6 +122,21 @@ class OracleSim:
         for d in self.domains:
             for k, v in d.forcing.items():
                 d._oracle_F[F_IDX[k], day0 - 1:day0 - 1 + v.shape[0]] = v
+
+    def set_meteo_weights(self, pre_w, temp_w, pet_w):
+        """read_meteo_weights: each (n_cells over all domains, 12, 24) =
+        L1_x_weights(cell, month, hour + 1); None turns them off."""
+        starts = np.cumsum([0] + [d.n_cells for d in self.domains])
+        for di, (d, cfg) in enumerate(zip(self.domains, self.cfgs)):
+            if pre_w is None:
+                d._oracle_W = None
+                cfg.meteo_weights = None
+                continue
+            lo, hi = starts[di], starts[di + 1]
+            w = np.stack([np.asarray(x, dtype=np.float64)[lo:hi] for x in (pre_w, temp_w, pet_w)])
+            # [var][cell][month][hour] -> [var][month * 24 + hour][cell]
+            d._oracle_W = np.ascontiguousarray(w.transpose(0, 2, 3, 1).reshape(3, 288, hi - lo))
+            cfg.meteo_weights = d._oracle_W.ctypes.data
 
     def set_forcing_domain(self, di, name, data, day0=1):
         d = self.domains[di]
@@ -170,7 +188,7 @@ class OracleSim:
                 if not self.route:
                     continue
                 rc = O.RouteCfg()
-                rc.n_nodes = d.n_cells
+                rc.n_nodes = d.n_nodes
                 rc.n_outlets = d.n_outlets
                 rc.timestep = self.timestep
                 ad = getattr(o, "adaptive", None)
@@ -182,7 +200,10 @@ class OracleSim:
                 rc.n_inflow = 0
                 rc.n_gauges = d.gauge_nodes.shape[0]
                 n = d.n_cells
-                l1 = np.arange(1, n + 1, dtype=np.int32)
+                l1 = (np.arange(1, n + 1, dtype=np.int32) if d.l1_l11 is None
+                      else np.ascontiguousarray(d.l1_l11, np.int32))
+                l11_area = np.zeros(d.n_nodes)  # used only when routing is finer than L1
+                l11_l1 = np.zeros(d.n_nodes, np.int32)
                 g = np.zeros(max(rc.n_gauges, 1))
                 tot = np.ascontiguousarray(o.X[16])
                 inf = getattr(d, "_oracle_inflow", None)
@@ -223,7 +244,7 @@ class OracleSim:
                     rc.rout_loop = 1
                 O.lib().or_mrm_routing(C.byref(rc), O.dptr(o.gamma), n, O.dptr(tot),
                                        O.dptr(np.ascontiguousarray(d.area_km2)), O.iptr(l1),
-                                       O.dptr(np.ascontiguousarray(d.area_km2)), O.iptr(l1),
+                                       O.dptr(l11_area), O.iptr(l11_l1),
                                        O.iptr(d.netPerm), O.iptr(d.fromN), O.iptr(d.toN),
                                        O.dptr(d.length), O.dptr(d.slope), O.dptr(d.fFPimp),
                                        O.iptr(iidx), O.iptr(ihw), O.iptr(inode), O.dptr(iq),
@@ -256,8 +277,8 @@ class OracleSim:
         out = {}
         for k in ("qOUT", "Qmod"):
             out[k] = np.concatenate([sets[set_].__dict__[k] for sets in self.ds])
-        out["qTIN"] = np.concatenate([sets[set_].qTIN[:d.n_cells] for d, sets in zip(self.domains, self.ds)])
-        out["qTR"] = np.concatenate([sets[set_].qTR[:d.n_cells] for d, sets in zip(self.domains, self.ds)])
+        out["qTIN"] = np.concatenate([sets[set_].qTIN[:d.n_nodes] for d, sets in zip(self.domains, self.ds)])
+        out["qTR"] = np.concatenate([sets[set_].qTR[:d.n_nodes] for d, sets in zip(self.domains, self.ds)])
         out["C1"] = np.concatenate([sets[set_].C1[:d.n_links] for d, sets in zip(self.domains, self.ds)])
         out["C2"] = np.concatenate([sets[set_].C2[:d.n_links] for d, sets in zip(self.domains, self.ds)])
         return out

@@ -11,16 +11,20 @@ START = dt.datetime(1990, 1, 1)
 
 def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, route=True,
               forcing_seed=100, pet_forcing=False, start=START, inflow=None,
-              forcing_columns=None):
+              forcing_columns=None, weights=False):
     """HIP model and oracle on the same inputs.  inflow: {"domain", "node" (1-based),
     "headwater", "q" (n_steps, n_inflow)} inflow gauges, given to both.
     forcing_columns: the model gets that many forcing columns and no forcing
-    (the caller streams it); the oracle gets all of it."""
+    (the caller streams it); the oracle gets all of it.
+    pet 2, 3: the radiation-based PET cases get their daily inputs and
+    monthly parameters; weights: meteo weights instead of day/night factors."""
     n_days = calendar.n_forcing_days_needed(n_steps)
     doy0 = start.timetuple().tm_yday
     for i, d in enumerate(domains):
         if not d.forcing or d.n_days < n_days:
             synth.make_forcing(d, n_days, forcing_seed + i, doy0=doy0)
+        if pet >= 2 and "NETRAD" not in d.forcing:
+            synth.make_pet_forcing(d, forcing_seed + 70 + i, doy0=doy0)
         if pet_forcing and "PET" not in d.forcing:
             rng = np.random.default_rng(forcing_seed + 50 + i)
             d.forcing["PET"] = rng.gamma(2.0, 1.2, size=(d.n_days, d.n_cells))
@@ -42,7 +46,7 @@ def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, 
     rng = np.random.default_rng(seed)
     gam = PM.draw(rng, n_sets * len(domains)).reshape(n_sets, len(domains), -1)
     for s in range(n_sets):
-        parts = [synth.cell_params(d, gam[s, i]) for i, d in enumerate(domains)]
+        parts = [synth.cell_params(d, gam[s, i], pet_case=pet) for i, d in enumerate(domains)]
         params = {k: np.concatenate([p[k] for p in parts], axis=0) for k in parts[0]}
         model.set_params(params, s)
         sim.set_params(params, s)
@@ -58,6 +62,10 @@ def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, 
     if pet_forcing:
         model.set_forcing("PET", np.concatenate([d.forcing["PET"] for d in domains], axis=1))
     sim.set_domain_forcing()
+    if weights:
+        w = synth.make_meteo_weights(sum(d.n_cells for d in domains), seed + 900)
+        model.set_meteo_weights(*w)
+        sim.set_meteo_weights(*w)
     steps = calendar.step_table(start, n_steps, timestep)
     recs = calendar.step_records(start, n_steps, timestep)
     return model, sim, steps, recs

@@ -408,3 +408,36 @@ def test_output_sums_match_oracle(how):
     with pytest.raises(RuntimeError, match="neutrons"):
         model.output_accumulate([1] * 21)
     model.close()
+
+
+@pytest.mark.parametrize("pet,weights,how", [
+    (2, False, "fused"), (2, False, "plain"), (2, False, "per_step"),
+    (3, False, "fused"), (3, False, "graph"), (3, False, "per_step"),
+    (1, True, "fused"), (1, True, "plain"), (3, True, "fused"), (3, True, "per_step")])
+def test_radiation_pet_and_meteo_weights_match_oracle(pet, weights, how):
+    """Priestley-Taylor (process_pet 2) and Penman-Monteith (3) on the device
+    (get_corrected_pet, mo_meteo_handler.f90:1078-1095, mo_pet.f90:149-272)
+    and the disaggregation by meteo weights (read_meteo_weights,
+    1102-1107, 1183-1189, 1265-1270): every state, flux and gauge against
+    the oracle over 30 daily-forced hourly steps crossing a day, two basins,
+    two parameter sets."""
+    _log(f"case: pet {pet} weights {weights} {how}")
+    doms = [synth.make_domain(12, 15, seed=51), synth.make_domain(8, 9, seed=52)]
+    model, sim, steps, recs = make_pair(doms, n_sets=2, n_steps=30, seed=53, pet=pet,
+                                        weights=weights)
+    if how == "per_step":
+        g = []
+        for k in range(30):
+            model.mhm_step(steps[k])
+            g.append(model.mrm_step(steps[k]))
+        g = np.stack(g)
+    else:
+        kw = {"fused": dict(mode=1, steps_per_launch=8), "plain": dict(mode=0),
+              "graph": dict(mode=1, steps_per_launch=8, use_graph=True)}[how]
+        g = np.concatenate([model.run(steps[:12], **kw), model.run(steps[12:], **kw)])
+    go = np.stack([sim.step(r) for r in recs])
+    ok, worst = rel_close(g, go, RTOL, ATOL)
+    assert ok, worst
+    assert compare_cells(model, sim, RTOL, ATOL) == []
+    assert compare_routing(model, sim, RTOL, ATOL) == []
+    model.close()

@@ -26,18 +26,21 @@ def _log(msg):
 
 
 def _push(model, dom, k, chunk):
-    model.stream_forcing({v: dom.forcing[v][k * chunk:(k + 1) * chunk] for v in VARS},
+    # every variable the domain has; the ones the PET case does not read are
+    # accepted and skipped
+    model.stream_forcing({v: a[k * chunk:(k + 1) * chunk] for v, a in dom.forcing.items()},
                          day0=k * chunk + 1)
 
 
-@pytest.mark.parametrize("how", ["fused", "graph", "plain", "per_step"])
-def test_streamed_forcing_equals_resident(how):
-    _log(f"case: streamed forcing {how}")
+@pytest.mark.parametrize("how,pet", [("fused", 1), ("graph", 1), ("plain", 1), ("per_step", 1),
+                                     ("fused", 3), ("per_step", 3)])
+def test_streamed_forcing_equals_resident(how, pet):
+    _log(f"case: streamed forcing {how} pet {pet}")
     dom = synth.make_domain(18, 22, seed=61)
     n_days, chunk = 10, 2
     n_steps = 24 * n_days
-    ref, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=n_steps, seed=62)
-    sut, _, _, _ = make_pair([dom], n_sets=2, n_steps=n_steps, seed=62,
+    ref, sim, steps, recs = make_pair([dom], n_sets=2, n_steps=n_steps, seed=62, pet=pet)
+    sut, _, _, _ = make_pair([dom], n_sets=2, n_steps=n_steps, seed=62, pet=pet,
                              forcing_columns=2 * chunk)
     kw = {"fused": dict(mode=1, steps_per_launch=12, route_mode=3),
           "graph": dict(mode=1, steps_per_launch=12, route_mode=3, use_graph=True),

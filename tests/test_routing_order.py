@@ -47,3 +47,23 @@ def test_random_forests(seed):
     nl = fromN.shape[0]
     assert np.array_equal(ro, ro2[:nl])
     assert np.array_equal(pm, pm2[:nl])
+
+
+def test_refined_domain_maps_every_l1_cell_to_its_coarse_node():
+    """The synthetic basin with a routing grid f times coarser than L1 (as in
+    the reference's check case_04): each L11 node holds the f x f L1 cells of
+    its block, column-major numbering on both grids, and the total area is
+    kept."""
+    from mhm_amd import synth
+    d11 = synth.make_domain(5, 6, seed=3)
+    for f in (2, 3):
+        d1 = synth.refine_domain(d11, f, 4)
+        assert d1.n_cells == f * f * d11.n_cells and d1.n_nodes == d11.n_cells
+        counts = np.bincount(d1.l1_l11 - 1, minlength=d11.n_cells)
+        assert np.all(counts == f * f)
+        # L1 cell (i, j) lies in node (i // f, j // f)
+        k = np.arange(d1.n_cells)
+        i, j = k % d1.nrows, k // d1.nrows
+        assert np.array_equal(d1.l1_l11 - 1, (j // f) * d11.nrows + i // f)
+        assert d1.area_km2.sum() == pytest.approx(d11.area_km2.sum(), rel=1e-12)
+        assert np.array_equal(d1.fromN, d11.fromN) and np.array_equal(d1.gauge_nodes, d11.gauge_nodes)
