@@ -328,7 +328,7 @@ def bench_serving(serve, n_req, qps, seed=11):
                                            horizon=(30, 30), sets=(1, 1))[0])
     srv = serve.Server(hbm_budget_bytes=64 << 30, max_slots=8, chunk_steps=73)
     res = srv.serve(reqs)
-    s = serve.summarize(res, reqs, srv.wall)
+    s = serve.summarize(res, reqs, srv.wall, srv.timings)
     s.update({"qps": qps, "peak_slots": srv.peak_slots,
               "peak_admitted_GiB": round(srv.peak_bytes / 2**30, 2),
               "workload": f"{n_req} Poisson arrivals at {qps} req/s: log-uniform 1k..100k cells, "

@@ -82,6 +82,9 @@ struct mhmhip_ctx {
   int32_t frow[MHMHIP_F_COUNT] = {};
   int32_t n_frows = 0;
   double *W = nullptr;  // meteo weights [3][12 * 24][sstride], null: off
+  // whether P, S, X and F hold anything written since creation (a network
+  // bound before any data re-lays nothing)
+  bool p_dirty = false, s_dirty = false, x_dirty = false, f_dirty = false;
   std::vector<int64_t> pos;  // reference cell -> device index (identity until a network is bound)
   // L11 routing nodes: their count, device index of every reference node and
   // per-domain ranges (the cells' until a network is bound).  On a routing
@@ -121,7 +124,13 @@ struct mhmhip_ctx {
   std::vector<char> gam_set, coef_restart;
   double *d_qsub = nullptr;
   int32_t qsub_loops = 0;
-  double *stage = nullptr;   // pinned staging row (sstride doubles)
+  // pinned staging rows (kStageSlots x sstride doubles): an upload fills the
+  // next slot once that slot's previous copy has left it (stage_ev), so
+  // several row uploads are in flight at once
+  static constexpr int kStageSlots = 4;
+  double *stage = nullptr;
+  hipEvent_t stage_ev[kStageSlots] = {};
+  int stage_next = 0;
 
   // Forcing columns.  Day d (1-based, iMeteoTS) lives in column
   // (d - 1) mod n_forcing_days; col_day[c] is the day column c holds (c + 1
@@ -219,6 +228,18 @@ struct mhmhip_ctx {
   GraphKey gkey;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // one routed step of mhmhip_mrm_step (the coarser grid's aggregation, the
+  // inflow patch and the 1 + tile-depth tile launches) captured once and
+  // replayed every step; rebuilt when a buffer it reads or writes changes
+  struct StepKey {
+    const void *ring = nullptr, *gauge = nullptr, *inflow = nullptr, *qacc = nullptr;
+    bool operator==(const StepKey &o) const {
+      return ring == o.ring && gauge == o.gauge && inflow == o.inflow && qacc == o.qacc;
+    }
+  };
+  StepKey skey;
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
 
   // kernel timing (mhmhip_set_profiling)
   struct Pending {
@@ -416,12 +437,19 @@ extern "C" int mhmhip_create(const mhmhip_config *cfg, mhmhip_ctx **out) {
     mhmhip_destroy(x);
     return rc;
   }
-  if (hipHostMalloc((void **)&x->stage, sizeof(double) * x->sstride, hipHostMallocDefault) !=
-      hipSuccess) {
+  if (hipHostMalloc((void **)&x->stage, sizeof(double) * x->sstride * mhmhip_ctx::kStageSlots,
+                    hipHostMallocDefault) != hipSuccess) {
     g_create_error = "create: pinned staging allocation failed";
     mhmhip_destroy(x);
     return MHMHIP_ENOMEM;
   }
+  std::fill(x->stage, x->stage + x->sstride * mhmhip_ctx::kStageSlots, 0.0);
+  for (int i = 0; i < mhmhip_ctx::kStageSlots; ++i)
+    if (hipEventCreateWithFlags(&x->stage_ev[i], hipEventDisableTiming) != hipSuccess) {
+      g_create_error = "create: staging event creation failed";
+      mhmhip_destroy(x);
+      return MHMHIP_EHIP;
+    }
   if (hipStreamSynchronize(x->stream) != hipSuccess) {
     g_create_error = "create: initial clear failed";
     mhmhip_destroy(x);
@@ -470,6 +498,8 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
   (void)hipFree(x->d_ring);
   (void)hipFree(x->d_gauge_buf);
   (void)hipFree(x->d_steps);
+  for (hipEvent_t e : x->stage_ev)
+    if (e) (void)hipEventDestroy(e);
   if (x->stage) (void)hipHostFree(x->stage);
   if (x->cstream) (void)hipStreamSynchronize(x->cstream);
   for (int i = 0; i < 2; ++i) {
@@ -510,36 +540,55 @@ extern "C" int mhmhip_sync(mhmhip_ctx *x) {
 
 namespace {
 
+// The next pinned staging row, free once its previous upload has left it.
+// Positions past `used` (the stride's padding) keep the zeros they were
+// cleared to; a download through the row clears them again.
+int stage_take(mhmhip_ctx *x, double **row) {
+  const int i = x->stage_next;
+  x->stage_next = (i + 1) % mhmhip_ctx::kStageSlots;
+  CHECK_HIP(x, hipEventSynchronize(x->stage_ev[i]), "stage wait");
+  *row = x->stage + (int64_t)i * x->sstride;
+  return MHMHIP_OK;
+}
+
+int stage_send(mhmhip_ctx *x, double *dst, const double *row) {
+  const int i = (int)((row - x->stage) / x->sstride);
+  CHECK_HIP(x, hipMemcpyAsync(dst, row, sizeof(double) * x->sstride, hipMemcpyHostToDevice,
+                              x->stream),
+            "upload");
+  CHECK_HIP(x, hipEventRecord(x->stage_ev[i], x->stream), "upload");
+  return MHMHIP_OK;
+}
+
 // upload one device row slice from host column col (reference cell order),
 // applying f to every value
 template <typename Fn>
 int put_row(mhmhip_ctx *x, double *dst, const double *src, Fn f) {
-  CHECK_HIP(x, hipStreamSynchronize(x->stream), "stage wait");
-  std::fill(x->stage, x->stage + x->sstride, 0.0);
-  for (int64_t c = 0; c < x->n_cells; ++c) x->stage[x->pos[c]] = f(src[c]);
-  CHECK_HIP(x, hipMemcpyAsync(dst, x->stage, sizeof(double) * x->sstride, hipMemcpyHostToDevice,
-                              x->stream),
-            "upload");
-  return MHMHIP_OK;
+  double *row = nullptr;
+  CHECK_RC(stage_take(x, &row));
+  for (int64_t c = 0; c < x->n_cells; ++c) row[x->pos[c]] = f(src[c]);
+  return stage_send(x, dst, row);
 }
 
 // one row of node values in reference node order -> device node order
 int put_node_row(mhmhip_ctx *x, double *dst, const double *src) {
-  CHECK_HIP(x, hipStreamSynchronize(x->stream), "stage wait");
-  std::fill(x->stage, x->stage + x->sstride, 0.0);
-  for (int64_t k = 0; k < x->n_nodes; ++k) x->stage[x->npos[k]] = src[k];
-  CHECK_HIP(x, hipMemcpyAsync(dst, x->stage, sizeof(double) * x->sstride, hipMemcpyHostToDevice,
-                              x->stream),
-            "upload");
-  return MHMHIP_OK;
+  double *row = nullptr;
+  CHECK_RC(stage_take(x, &row));
+  // past the nodes: zeros (a cell row written later overwrites [0, n_cells))
+  std::fill(row + x->n_nodes, row + x->sstride, 0.0);
+  for (int64_t k = 0; k < x->n_nodes; ++k) row[x->npos[k]] = src[k];
+  return stage_send(x, dst, row);
 }
 
 int get_row(mhmhip_ctx *x, const double *srcdev, double *dst) {
-  CHECK_HIP(x, hipMemcpyAsync(x->stage, srcdev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost,
+  double *row = nullptr;
+  CHECK_RC(stage_take(x, &row));
+  CHECK_HIP(x, hipMemcpyAsync(row, srcdev, sizeof(double) * x->sstride, hipMemcpyDeviceToHost,
                               x->stream),
             "download");
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "download");
-  for (int64_t c = 0; c < x->n_cells; ++c) dst[c] = x->stage[x->pos[c]];
+  for (int64_t c = 0; c < x->n_cells; ++c) dst[c] = row[x->pos[c]];
+  std::fill(row + x->n_cells, row + x->sstride, 0.0);  // the padding a cell row leaves
   return MHMHIP_OK;
 }
 
@@ -575,6 +624,7 @@ extern "C" int mhmhip_set_cell_var(mhmhip_ctx *x, int32_t var, int32_t set, cons
   const bool is_param = var >= 0 && var < MHMHIP_P_COUNT;
   const bool is_state = var >= MHMHIP_S_INTER && var < MHMHIP_S_END;
   if (!is_param && !is_state) return x->fail(MHMHIP_EINVAL, "set_cell_var: unknown or read-only variable");
+  (is_param ? x->p_dirty : x->s_dirty) = true;
   if (ncols != var_cols(x, var)) return x->fail(MHMHIP_EINVAL, "set_cell_var: wrong column count");
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_cell_var");
   const int64_t off = (int64_t)set * x->sstride;
@@ -629,9 +679,10 @@ extern "C" int mhmhip_set_cell_var(mhmhip_ctx *x, int32_t var, int32_t set, cons
     case MHMHIP_S_SOILMOIST: row0 = SR_NSCALAR; base = x->S; break;
     default: return x->fail(MHMHIP_EINVAL, "set_cell_var: unknown variable");
   }
+  // the host values are in the staging rows when this returns; the uploads
+  // finish on the context's stream ahead of any later work
   for (int col = 0; col < ncols; ++col)
     CHECK_RC(put_row(x, base + (row0 + col) * x->ld + off, host + col * ld, id));
-  CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_cell_var");
   return MHMHIP_OK;
 }
 
@@ -668,6 +719,7 @@ extern "C" int mhmhip_get_cell_var(mhmhip_ctx *x, int32_t var, int32_t set, doub
 extern "C" int mhmhip_default_states(mhmhip_ctx *x, const double *hd) {
   if (!x) return MHMHIP_EINVAL;
   if (!hd && x->nh > 1) return x->fail(MHMHIP_EINVAL, "default_states: horizon depths required");
+  x->s_dirty = x->x_dirty = true;
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "default_states");
   std::vector<double> v(x->n_cells);
   auto fill = [&](double val) { std::fill(v.begin(), v.end(), val); };
@@ -748,6 +800,7 @@ extern "C" int mhmhip_stream_forcing(mhmhip_ctx *x, int32_t day0, int32_t ndays,
     for (int v = 0; v < MHMHIP_F_COUNT; ++v) any = any || host[v] != nullptr;
     if (!any) return x->fail(MHMHIP_EINVAL, "stream_forcing: no variable given");
   }
+  x->f_dirty = true;
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "stream_forcing");
   if (!x->cstream) {
     CHECK_HIP(x, hipStreamCreateWithFlags(&x->cstream, hipStreamNonBlocking), "copy stream");
@@ -832,6 +885,7 @@ extern "C" int mhmhip_set_forcing(mhmhip_ctx *x, int32_t var, int32_t day0, int3
       (int64_t)day0 - 1 + ndays > x->cfg.n_forcing_days || ld < x->n_cells)
     return x->fail(MHMHIP_EINVAL, "set_forcing: variable or day range out of bounds");
   if (x->frow[var] < 0) return MHMHIP_OK;  // not read by this PET case
+  x->f_dirty = true;
   CHECK_HIP(x, hipSetDevice(x->cfg.device), "set_forcing");
   CHECK_RC(forcing_columns_direct(x, day0, ndays));
   auto id = [](double a) { return a; };
@@ -878,6 +932,7 @@ extern "C" int mhmhip_set_forcing_l2(mhmhip_ctx *x, int32_t var, int32_t domain,
                                       std::to_string(c_hi - c_lo) + " cells, it holds " +
                                       std::to_string(ncells));
   if (x->frow[var] < 0) return MHMHIP_OK;  // not read by this PET case
+  x->f_dirty = true;
   // cellFactorHbyM (mo_meteo_helper.f90:120) picks the mapping
   const double hbym = cs1 / cs2;
   const bool aggregate = hbym > 1.0, equal = !(hbym > 1.0) && !(hbym < 1.0);
@@ -1355,6 +1410,11 @@ int normalize_qtr(mhmhip_ctx *x) {
 }
 
 void drop_graph(mhmhip_ctx *x) {
+  if (x->step_exec) (void)hipGraphExecDestroy(x->step_exec);
+  if (x->step_graph) (void)hipGraphDestroy(x->step_graph);
+  x->step_exec = nullptr;
+  x->step_graph = nullptr;
+  x->skey = mhmhip_ctx::StepKey();
   if (x->graph_exec) (void)hipGraphExecDestroy(x->graph_exec);
   if (x->graph) (void)hipGraphDestroy(x->graph);
   for (auto &p : x->graph_events) {
@@ -1387,6 +1447,7 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
   }
   // the step is step 0 of a one-step ring, routed by the tiles (mrm_step); on
   // a coarser routing grid mrm_step aggregates the step's L1_total_runoff
+  x->s_dirty = x->x_dirty = true;
   CellArgs a = cell_args(x, route && !x->coarse);
   a.ring_len = 1;
   a.lag = route && !x->coarse ? x->d_lag : nullptr;
@@ -1522,10 +1583,12 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
         }
       return MHMHIP_OK;
     };
-    CHECK_RC(relayout(x->P, x->n_prows, x->ld, x->n_sets));
-    CHECK_RC(relayout(x->S, x->n_srows, x->ld, x->n_sets));
-    CHECK_RC(relayout(x->X, x->n_xrows, x->ld, x->n_sets));
-    CHECK_RC(relayout(x->F, (int64_t)x->n_frows * x->cfg.n_forcing_days, x->sstride, 1));
+    // buffers never written since creation hold zeros in any order
+    if (x->p_dirty) CHECK_RC(relayout(x->P, x->n_prows, x->ld, x->n_sets));
+    if (x->s_dirty) CHECK_RC(relayout(x->S, x->n_srows, x->ld, x->n_sets));
+    if (x->x_dirty) CHECK_RC(relayout(x->X, x->n_xrows, x->ld, x->n_sets));
+    if (x->f_dirty)
+      CHECK_RC(relayout(x->F, (int64_t)x->n_frows * x->cfg.n_forcing_days, x->sstride, 1));
     if (x->W) CHECK_RC(relayout(x->W, (int64_t)3 * 12 * 24, x->sstride, 1));
   }
   x->pos = cpos;
@@ -2203,13 +2266,6 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
       inflow_row = x->d_inflow_avg;
     }
   }
-  if (x->coarse && factor <= 1.0)
-    // L11_runoff_acc on the coarser routing grid: this step's L1_total_runoff
-    // (mhm_step's flux row) aggregated into ring row 0 at each node's lag
-    CHECK_HIP(x, launch_runoff_agg(x->X + XR_TOTAL * x->ld, 1, x->d_agg_start, x->d_area,
-                                   x->n_nodes, x->sstride, x->ld, x->n_sets, x->d_ring, x->d_lag,
-                                   1, 1, 0, 1, kHourSecs * (double)x->cfg.timestep, x->stream),
-              "runoff_agg");
   const int kind = MHMHIP_K_ROUTE_TILE;
   const TileArgs g = tile_args(x, 1, 1);
   const int64_t ng = (int64_t)x->n_sets * x->n_gauges;
@@ -2218,19 +2274,64 @@ extern "C" int mhmhip_mrm_step(mhmhip_ctx *x, const mhmhip_step *st, double *gau
     if (!x->d_qsub) CHECK_RC(x->dalloc(&x->d_qsub, x->ld, "alloc Qmod sub-step sums"));
     CHECK_HIP(x, hipMemsetAsync(x->d_qsub, 0, sizeof(double) * x->ld, x->stream), "mrm_step");
   }
-  // loops routings of this step's qOUT (mRM_routing, mo_mrm_routing.f90:224-284):
-  // the states advance each time, Qmod is their average
-  for (int32_t it = 0; it < loops; ++it) {
-    CHECK_RC(normalize_qtr(x));
-    // this step is step 0 of a one-step ring: mhm_step wrote its qOUT to ring
-    // row lag (the tiles' layout)
+  // this step is step 0 of a one-step ring: mhm_step wrote its qOUT to ring
+  // row lag (the tiles' layout), or, on a coarser routing grid, its
+  // L1_total_runoff, which L11_runoff_acc aggregates into that row here
+  auto enqueue = [&](int32_t it) -> int {
     RouteArgs r = route_args(x, 0, 1, 0);
     r.ring_len = 1;
     r.inflow_q = inflow_row;
     if (loops > 1) r.qacc = x->d_qsub;
+    if (x->coarse && factor <= 1.0 && it == 0)
+      CHECK_HIP(x, launch_runoff_agg(x->X + XR_TOTAL * x->ld, 1, x->d_agg_start, x->d_area,
+                                     x->n_nodes, x->sstride, x->ld, x->n_sets, x->d_ring,
+                                     x->d_lag, 1, 1, 0, 1, kHourSecs * (double)x->cfg.timestep,
+                                     x->stream),
+                "runoff_agg");
     if (x->n_inflow > 0 && it == 0)
       CHECK_HIP(x, launch_inflow_patch(r, x->d_ring, x->d_lag, 0, 1, x->stream), "inflow_patch");
-    CHECK_RC(route_tiles(x, r, g, 0, g.nwin + g.tau_max));
+    return route_tiles(x, r, g, 0, g.nwin + g.tau_max);
+  };
+  // the ordinary step (one routing, the model step's own runoff) replays a
+  // captured graph: one launch of 1 + tile-depth tile kernels instead of as
+  // many stream launches
+  const bool replay = loops == 1 && factor <= 1.0 && !x->prof;
+  if (replay) {
+    CHECK_RC(normalize_qtr(x));
+    mhmhip_ctx::StepKey key;
+    key.ring = x->d_ring;
+    key.gauge = x->d_gauge_buf;
+    key.inflow = inflow_row;
+    key.qacc = x->qacc_on ? x->d_qacc : nullptr;
+    if (!x->step_exec || !(key == x->skey)) {
+      if (x->step_exec) (void)hipGraphExecDestroy(x->step_exec);
+      if (x->step_graph) (void)hipGraphDestroy(x->step_graph);
+      x->step_exec = nullptr;
+      x->step_graph = nullptr;
+      CHECK_HIP(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal), "capture");
+      x->capturing = true;
+      const int rc = enqueue(0);
+      x->capturing = false;
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(x->stream, &gr);
+      if (rc != MHMHIP_OK) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      CHECK_HIP(x, e, "capture");
+      x->step_graph = gr;
+      CHECK_HIP(x, hipGraphInstantiate(&x->step_exec, gr, nullptr, nullptr, 0), "instantiate");
+      x->skey = key;
+    }
+    CHECK_HIP(x, hipGraphLaunch(x->step_exec, x->stream), "graph launch");
+    credit_route(x, kind, 1);
+    x->qtr_slot = 0;
+  }
+  // loops routings of this step's qOUT (mRM_routing, mo_mrm_routing.f90:224-284):
+  // the states advance each time, Qmod is their average
+  for (int32_t it = 0; it < (replay ? 0 : loops); ++it) {
+    CHECK_RC(normalize_qtr(x));
+    CHECK_RC(enqueue(it));
     credit_route(x, kind, 1);
     x->qtr_slot = 0;
     if (loops > 1 && gauge_out && ng > 0) {
@@ -2356,6 +2457,7 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
                        (!x->W || p.wrow == c.wrow);
   }
   CHECK_RC(forcing_wait(x, hs.data(), nsteps));
+  x->s_dirty = x->x_dirty = true;
   const int S = o.mode == MHMHIP_RUN_PLAIN ? 1 : (o.steps_per_launch > 0 ? o.steps_per_launch : 24);
   const int block = o.block > 0 ? o.block : 256;
   const int kind = route_kind(o.route_mode);

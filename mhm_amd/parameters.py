@@ -91,15 +91,39 @@ def lai_l1(fractions):
     return fF[:, None] * LAI[0] + fI[:, None] * LAI[1] + fP[:, None] * LAI[2]
 
 
-def mpr_lite(g, fractions, texture, aspect, horizon_depth):
+def static_fields(fractions, texture, horizon_depth):
+    """The fields of mpr_lite that do not depend on gamma (leaf area index,
+    PET-LAI factor, soil water contents per horizon), computed once per
+    domain and shared read-only by its parameter sets."""
+    lai = lai_l1(fractions)
+    hd = np.asarray(horizon_depth, dtype=np.float64)
+    top = np.concatenate([[0.0], hd[:-1]])
+    dz = hd - top
+    poro = 0.38 + 0.12 * texture
+    fcr = 0.55 + 0.25 * (1.0 - texture)
+    wpr = 0.30 + 0.25 * texture
+    sat = dz[None, :] * poro[:, None]
+    fc = sat * fcr[:, None]
+    wp = fc * wpr[:, None]
+    out = {"LAI": lai, "PETLAICOR": np.clip(0.6 + 0.08 * lai, 0.3, 1.3), "SOILMOIST_SAT": sat,
+           "SOILMOIST_FC": fc, "WILTINGPOINT": wp, "hd": hd, "KARSTLOSS": np.ones(texture.shape[0])}
+    for v in out.values():
+        v.setflags(write=False)
+    return out
+
+
+def mpr_lite(g, fractions, texture, aspect, horizon_depth, static=None):
     """L1 effective parameter fields for one gamma vector.
 
     fractions: (3, n) forest / impervious / pervious cover fractions
     texture:   (n,) in [0, 1), a soil texture index
     aspect:    (n,) terrain aspect in degrees
     horizon_depth: lower boundaries of the horizons [mm]
-    Returns a dict of arrays: 1-D (n,), horizon (n, nH) or monthly (n, 12).
+    static: static_fields of the same inputs (computed here when None)
+    Returns a dict of arrays: 1-D (n,), horizon (n, nH) or monthly (n, 12);
+    the gamma-independent ones are the read-only arrays of `static`.
     """
+    st = static if static is not None else static_fields(fractions, texture, horizon_depth)
     fF, fI, fP = fractions
     n = fF.shape[0]
     G = lambda name: g[INDEX[name]]
@@ -119,23 +143,14 @@ def mpr_lite(g, fractions, texture, aspect, horizon_depth):
     # iper_thres_runoff (883-894)
     out["SEALED_THRESH"] = np.full(n, G("imperviousStorageCapacity"))
     # interception: LAI * factor
-    lai = lai_l1(fractions)
-    out["MAXINTER"] = G("canopyInterceptionFactor") * lai
-    out["PETLAICOR"] = np.clip(0.6 + 0.08 * lai, 0.3, 1.3)
+    out["MAXINTER"] = G("canopyInterceptionFactor") * st["LAI"]
+    out["PETLAICOR"] = st["PETLAICOR"]
     # soil hydraulics per horizon
-    hd = np.asarray(horizon_depth, dtype=np.float64)
+    hd = st["hd"]
     nH = hd.shape[0]
-    top = np.concatenate([[0.0], hd[:-1]])
-    dz = hd - top
-    poro = 0.38 + 0.12 * texture
-    fcr = 0.55 + 0.25 * (1.0 - texture)
-    wpr = 0.30 + 0.25 * texture
-    sat = dz[None, :] * poro[:, None]
-    fc = sat * fcr[:, None]
-    wp = fc * wpr[:, None]
-    out["SOILMOIST_SAT"] = sat
-    out["SOILMOIST_FC"] = fc
-    out["WILTINGPOINT"] = wp
+    out["SOILMOIST_SAT"] = st["SOILMOIST_SAT"]
+    out["SOILMOIST_FC"] = st["SOILMOIST_FC"]
+    out["WILTINGPOINT"] = st["WILTINGPOINT"]
     out["SOILMOIST_EXP"] = np.repeat((G("infiltrationShapeFactor") * (1.0 + 0.3 * texture))[:, None],
                                      nH, axis=1)
     # root fractions: cumulative 1 - c**(z/10 mm), class-weighted, normalised
@@ -154,7 +169,7 @@ def mpr_lite(g, fractions, texture, aspect, horizon_depth):
     out["KPERCO"] = np.maximum(G("rechargeCoefficient") * (0.5 + texture), 2.0)
     out["KBASEFLOW"] = np.maximum(G("GeoParam") * (0.5 + 0.5 * texture), out["KSLOWFLOW"])
     out["ALPHA"] = np.full(n, G("exponentSlowInterflow"))
-    out["KARSTLOSS"] = np.ones(n)
+    out["KARSTLOSS"] = st["KARSTLOSS"]
     out["UNSAT_THRESH"] = G("interflowStorageCapacityFactor") * (0.2 + 0.3 * texture)
     # PET aspect correction
     thr = G("aspectTresholdPET")

@@ -89,13 +89,17 @@ def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_step
     return cells + ring + routing + series
 
 
-def prepare_model(req, device=0):
-    """Context with network, parameters, default states and forcing of a request."""
+def prepare_model(req, device=0, timings=None):
+    """Context with network, parameters, default states and forcing of a
+    request; timings (a dict, optional) receives the seconds of each part."""
     dom = req.domain
     n_days = calendar.n_forcing_days_needed(req.n_steps)
+    t0 = time.perf_counter()
     m = engine.Model([dom], n_sets=req.gammas.shape[0], timestep=req.timestep, route=True,
                      n_forcing_days=n_days, device=device)
+    t1 = time.perf_counter()
     m.bind_network()
+    t2 = time.perf_counter()
     # the sets' parameter fields are independent host work (numpy releases the
     # GIL in its array loops): build them side by side, upload them in order
     # (calls on one context are serialised)
@@ -106,8 +110,13 @@ def prepare_model(req, device=0):
             for i, f in enumerate(fields):
                 m.set_params(f, s0 + i)
                 m.set_routing_params(PM.routing_gamma(req.gammas[s0 + i]), s0 + i)
+    t3 = time.perf_counter()
     m.default_states()
     m.set_domain_forcing()
+    m.sync()
+    if timings is not None:
+        timings.update(create_s=t1 - t0, network_s=t2 - t1, params_s=t3 - t2,
+                       states_forcing_s=time.perf_counter() - t3)
     return m
 
 
@@ -121,11 +130,19 @@ def run_chunks(m, req, chunk_steps=73):
     return np.concatenate(parts)
 
 
-def run_alone(req, chunk_steps=73, device=0):
-    """The request on its own (reference result for the mix)."""
-    m = prepare_model(req, device)
+def run_alone(req, chunk_steps=73, device=0, timings=None):
+    """The request on its own (reference result for the mix); timings (a
+    dict, optional) receives the seconds spent preparing the context on the
+    host (parameter fields, network plan, uploads) and running it."""
+    t0 = time.perf_counter()
+    m = prepare_model(req, device, timings)
+    t1 = time.perf_counter()
     g = run_chunks(m, req, chunk_steps)
+    t2 = time.perf_counter()
     m.close()
+    if timings is not None:
+        timings["prepare_s"] = t1 - t0
+        timings["run_s"] = t2 - t1
     return g
 
 
@@ -135,7 +152,9 @@ class Server:
         """runner(req) -> gauges runs one request (default: its own device
         context, chunked fused runs); footprint(req) -> bytes (default:
         estimate_bytes of its context)."""
-        self.runner = runner or (lambda req: run_alone(req, chunk_steps, device))
+        self.timings = {}
+        self.runner = runner or (lambda req: run_alone(req, chunk_steps, device,
+                                                       self.timings.setdefault(req.rid, {})))
         self.footprint = footprint or self._footprint
         self.budget = int(hbm_budget_bytes)
         self.max_slots = int(max_slots)
@@ -256,9 +275,16 @@ def poisson_requests(n, qps, seed=0, cells=(1000, 100000), horizon=(30, 365), se
     return reqs
 
 
-def summarize(results, requests, wall):
+def summarize(results, requests, wall, timings=None):
     lat = np.array([results[r.rid].latency for r in requests])
     cs = sum(r.cell_steps for r in requests)
-    return {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)),
-            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
-            "cell_steps_per_s": cs / wall, "wall_s": wall}
+    out = {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)),
+           "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+           "cell_steps_per_s": cs / wall, "wall_s": wall}
+    if timings:
+        for k in ("prepare_s", "run_s", "create_s", "network_s", "params_s", "states_forcing_s"):
+            v = np.array([timings[r.rid][k] for r in requests if k in timings.get(r.rid, {})])
+            if v.size:
+                out[k.replace("_s", "") + "_p50_s"] = float(np.percentile(v, 50))
+                out[k.replace("_s", "") + "_p99_s"] = float(np.percentile(v, 99))
+    return out

@@ -187,9 +187,13 @@ def make_forcing(dom, n_days, seed, doy0=1):
 def cell_params(dom, gamma, pet_case=1):
     """L1 effective parameters of one gamma vector (MPR-lite) plus latitude;
     pet_case 2 or 3 adds that case's monthly PET parameters."""
-    p = PM.mpr_lite(gamma, dom.fractions, dom.texture, dom.aspect, dom.horizon_depth)
-    p.update(PM.pet_radiation_params(PM.lai_l1(dom.fractions), pet_case))
-    p["LATITUDE"] = dom.lat.copy()
+    st = getattr(dom, "_mpr_static", None)
+    if st is None:
+        st = dom._mpr_static = PM.static_fields(dom.fractions, dom.texture, dom.horizon_depth)
+    p = PM.mpr_lite(gamma, dom.fractions, dom.texture, dom.aspect, dom.horizon_depth, static=st)
+    if pet_case >= 2:
+        p.update(PM.pet_radiation_params(st["LAI"], pet_case))
+    p["LATITUDE"] = dom.lat
     return p
 
 
