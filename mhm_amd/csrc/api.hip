@@ -1590,6 +1590,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
     if (x->f_dirty)
       CHECK_RC(relayout(x->F, (int64_t)x->n_frows * x->cfg.n_forcing_days, x->sstride, 1));
     if (x->W) CHECK_RC(relayout(x->W, (int64_t)3 * 12 * 24, x->sstride, 1));
+    if (x->d_out) CHECK_RC(relayout(x->d_out, x->out_rows_cap, x->ld, x->n_sets));  // output sums
   }
   x->pos = cpos;
   x->d_pos_stale = true;

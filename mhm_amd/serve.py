@@ -16,6 +16,7 @@ parameter-set counts arrive over time and are run with continuous batching.
 
 Latency = completion - arrival; p50/p99 are reported beside cell-steps/s.
 """
+from collections import deque
 from concurrent.futures import ThreadPoolExecutor
 import math
 import queue
@@ -62,6 +63,8 @@ TILE_HEIGHT = 16  # levels per routing tile; the runoff ring has this many rows 
 
 
 STEP_BYTES = 104  # one step of the device step table (StepDev)
+PREP_WORKERS = 6  # threads building one request's parameter fields
+PREP_AHEAD = 8    # parameter sets built ahead of their upload (host memory bound)
 # forcing rows a context holds per PET case (pre, tavg and what the case reads)
 FORCING_ROWS = {-1: 3, 0: 3, 1: 4, 2: 3, 3: 5}
 
@@ -101,15 +104,19 @@ def prepare_model(req, device=0, timings=None):
     m.bind_network()
     t2 = time.perf_counter()
     # the sets' parameter fields are independent host work (numpy releases the
-    # GIL in its array loops): build them side by side, upload them in order
+    # GIL in its array loops): built on worker threads, at most PREP_AHEAD sets
+    # ahead of the upload, and uploaded in order while the next ones are built
     # (calls on one context are serialised)
     n = len(req.gammas)
-    with ThreadPoolExecutor(max_workers=min(4, n)) as pool:
-        for s0 in range(0, n, 4):  # four sets' fields held at a time
-            fields = list(pool.map(lambda g: synth.cell_params(dom, g), req.gammas[s0:s0 + 4]))
-            for i, f in enumerate(fields):
-                m.set_params(f, s0 + i)
-                m.set_routing_params(PM.routing_gamma(req.gammas[s0 + i]), s0 + i)
+    with ThreadPoolExecutor(max_workers=min(PREP_WORKERS, n)) as pool:
+        pending = deque()
+        nxt = 0
+        for s_ in range(n):
+            while nxt < n and len(pending) < PREP_AHEAD:
+                pending.append(pool.submit(synth.cell_params, dom, req.gammas[nxt]))
+                nxt += 1
+            m.set_params(pending.popleft().result(), s_)
+            m.set_routing_params(PM.routing_gamma(req.gammas[s_]), s_)
     t3 = time.perf_counter()
     m.default_states()
     m.set_domain_forcing()

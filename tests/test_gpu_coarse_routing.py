@@ -114,3 +114,52 @@ def test_coarse_routing_grid_limits():
     with pytest.raises(RuntimeError, match="coarser"):
         model.run(steps, mode=1, steps_per_launch=8, noalloc=True)
     model.close()
+
+
+@pytest.mark.parametrize("coarse", [False, True])
+def test_binding_after_data_keeps_every_cell_value(coarse):
+    """Binding a network re-lays the cell data already on the device into the
+    new cell order (routing order, or grouped by node on a coarser routing
+    grid): a context that ran 6 steps and started output sums before its
+    network was bound continues exactly like one bound from the start --
+    every state, flux and output sum bit for bit."""
+    from mhm_amd import calendar, engine, parameters as PM
+    from pairs import START
+    _log(f"case: binding after data coarse={coarse}")
+    dom = (synth.refine_domain(synth.make_domain(6, 7, seed=81), 2, 82) if coarse
+           else synth.make_domain(12, 11, seed=81))
+    n = 16
+    synth.make_forcing(dom, calendar.n_forcing_days_needed(n), 83)
+    steps = calendar.step_table(START, n, 1)
+    gam = PM.draw(np.random.default_rng(84), 2)
+    flags = [1] * 21
+    flags[17] = 0
+    models = []
+    for bind_first in (True, False):
+        m = engine.Model([dom], n_sets=2, timestep=1, route=True,
+                         n_forcing_days=calendar.n_forcing_days_needed(n))
+        if bind_first:
+            m.bind_network()
+        for s in range(2):
+            m.set_params(synth.cell_params(dom, gam[s]), s)
+        m.default_states()
+        m.set_domain_forcing()
+        m.output_accumulate(flags)
+        m.run(steps[:6], mode=1, steps_per_launch=3, want_gauges=False)
+        if not bind_first:
+            m.bind_network()
+        for s in range(2):
+            m.set_routing_params(PM.routing_gamma(gam[s]), s)
+        m.run(steps[6:], mode=1, steps_per_launch=5)
+        models.append(m)
+    a, b = models
+    for s in range(2):
+        for name in ("INTER", "SNOWPACK", "SEALSTW", "UNSATSTW", "SATSTW", "SOILMOIST",
+                     "TOTAL_RUNOFF", "AETSOIL", "PET_CALC"):
+            assert np.array_equal(a.get(name, s), b.get(name, s)), (s, name)
+        for k in (2, 3, 11, 19):
+            sa, ca = a.output_sum(k, 1, s)
+            sb, cb = b.output_sum(k, 1, s)
+            assert ca == cb == n and np.array_equal(sa, sb), (s, k)
+    a.close()
+    b.close()

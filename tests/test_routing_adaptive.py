@@ -10,8 +10,10 @@ nint(1 / tsRoutFactor)) times and averages Qmod (mo_mrm_routing.f90:
 CPU tests pin the oracle: C1/C2 against Chow's Muskingum coefficients for
 x = 0 (c0 = C2, c1 = C1 - C2, c2 = 1 - C1 with dt = TSrout), the routing
 step against the largest given_TS not above min K (equality included), the
-sub-step count from the factor, and the averaging against repeated
-one-loop calls.  GPU tests call mhmhip_set_routing_celerity and the per-step
+sub-step count from the factor, the averaging against repeated
+one-loop calls, and, for routing steps longer than the model step, each
+routing step's qOUT against the period's runoff the oracle reports step by
+step.  GPU tests call mhmhip_set_routing_celerity and the per-step
 entry points through the C-ABI against the oracle.
 """
 import numpy as np
@@ -241,3 +243,40 @@ def test_long_routing_steps_match_oracle(celerity, every, n_steps):
     assert compare_cells(model, sim, 1e-10, 1e-12) == []
     assert compare_routing(model, sim, 1e-10, 1e-12) == []
     model.close()
+
+
+@pytest.mark.parametrize("celerity,every,n_steps", [(0.1, 2, 24), (0.05, 4, 25)])
+def test_long_routing_steps_accumulate_the_period_runoff(celerity, every, n_steps):
+    """Routing steps longer than the model step in the oracle
+    (mo_mhm_interface_run.f90:494-512, 608-618): on a routing step the qOUT
+    of every node is the L1_total_runoff summed over the period's model
+    steps, times the cell area, converted with the routing step's length:
+    sum(q) [mm] * area [km2] * 1000 / (3600 s * timestep * steps in the
+    period) -- the last period of 25 steps by 4 is one step long.  Checked
+    from the per-step runoff the oracle reports, on a network without
+    inflow gauges (qOUT is then the accumulated runoff alone)."""
+    from mhm_amd import calendar, parameters as PM, synth
+    from oracle_sim import OracleSim
+    from pairs import START
+    dom = synth.make_domain(9, 8, seed=52)
+    synth.make_forcing(dom, calendar.n_forcing_days_needed(n_steps), 53)
+    sim = OracleSim([dom], n_sets=1, n_forcing_days=calendar.n_forcing_days_needed(n_steps))
+    g = PM.draw(np.random.default_rng(54), 1)[0]
+    sim.set_params(synth.cell_params(dom, g), 0)
+    sim.default_states()
+    sim.set_domain_forcing()
+    sim.set_routing_celerity(celerity)
+    assert sim.ds[0][0].adaptive[1] == float(every)
+    period = []
+    for r in calendar.step_records(START, n_steps, 1):
+        go = sim.step(r)
+        period.append(sim.ds[0][0].X[16].copy())  # this step's L1_total_runoff [mm]
+        tt = int(r["tt"])
+        if tt % every == 0 or tt == n_steps:
+            assert not np.isnan(go).any(), tt
+            want = np.sum(period, axis=0) * dom.area_km2 * 1000.0 / (3600.0 * len(period))
+            assert np.allclose(sim.ds[0][0].qOUT, want, rtol=1e-14, atol=0.0), tt
+            period = []
+        else:
+            assert np.isnan(go).all(), tt
+    assert period == []
