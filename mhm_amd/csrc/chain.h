@@ -11,7 +11,7 @@
 #include <float.h>
 #include <stdint.h>
 
-#include "logtab.h"
+#include "mathtab.h"
 
 namespace mhmhip {
 
@@ -70,26 +70,31 @@ struct DV {
 
 // Natural log of finite x > 0 (normal or subnormal), N values in lockstep,
 // table-driven (the method of Tang, ACM TOMS 16 (1990) 378-400): x = m 2^e
-// with m in [sqrt(1/2), sqrt(2)); F = j/128 the nearest table point, so
-// f = m - F is exact (|f| <= 1/256); u = f RN(1/F) and
+// with m in [1/2, 1) (frexp); F = j/256 the nearest table point (j = 128..256),
+// so f = m - F is exact (|f| <= 1/512); u = f RN(1/F) (|u| <= 1/256) and
 //   log x = e ln2 + log F + log1p(u),
-// log1p(u) = u + u^2 (-1/2 + u/3 - ... + u^5/7) (truncation below 1e-17
-// relative for |u| < 0.0056); within 1.25 ulp (near x = 1 the table term and
-// log1p(u) partly cancel).  log F comes from logtab.h as hi + lo with hi a
-// multiple of 2^-42, so e ln2_hi + hi is exact.  Every operation is an IEEE
-// +, -, *, fma, rint or an exact frexp, and the table holds the same numbers
-// on host and device (LDS, filled by logtab_fill), so the two builds agree
-// bit for bit.  Each step runs over all N values before the next one, which
-// gives the scheduler N independent dependency chains to interleave.
+// log1p(u) = u + u^2 (-1/2 + u/3 - ... + u^5/7) (truncation below 2e-18
+// relative).  log F comes from mathtab.h as hi + lo with hi a multiple of
+// 2^-42, so e ln2_hi + hi is exact; at F = 1/2 the table holds exactly
+// -(ln2_hi + ln2_lo), so just above x = 1 (e = 1, F = 1/2) the two terms
+// cancel with no rounding and log x is log1p(u) of u = 2m - 1 = x - 1.
+// Within 1.25 ulp.  Every operation is an IEEE +, -, *, fma, rint or an exact
+// frexp, and the table holds the same numbers on host and device (LDS,
+// filled by mathtab_fill), so the two builds agree bit for bit.  Each step
+// runs over all N values before the next one, which gives the scheduler N
+// independent dependency chains to interleave.  Rows are 4 doubles (32 B):
+// a row's address is one shift of j.
 #if defined(__HIP_DEVICE_COMPILE__)
-__shared__ double s_logtab[kLogTabN * 3];
+__shared__ __attribute__((aligned(16))) double s_logtab[kLogTabN * 4];
+__shared__ __attribute__((aligned(16))) double s_exptab[kExpTabN * 2];
 #endif
 
-// Every workgroup of a kernel that evaluates a power calls this first (all
-// threads, before any of them returns).
-__device__ inline void logtab_fill() {
+// Every workgroup of a kernel that evaluates a log or an exp calls this
+// first (all threads, before any of them returns).
+__device__ inline void mathtab_fill() {
 #if defined(__HIP_DEVICE_COMPILE__)
-  for (int i = threadIdx.x; i < kLogTabN * 3; i += blockDim.x) s_logtab[i] = kLogTab[i];
+  for (int i = threadIdx.x; i < kLogTabN * 4; i += blockDim.x) s_logtab[i] = kLogTab[i];
+  for (int i = threadIdx.x; i < kExpTabN * 2; i += blockDim.x) s_exptab[i] = kExpTab[i];
   __syncthreads();
 #endif
 }
@@ -102,6 +107,14 @@ __host__ __device__ inline const double *logtab() {
 #endif
 }
 
+__host__ __device__ inline const double *exptab() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return s_exptab;
+#else
+  return kExpTab;
+#endif
+}
+
 template <int N>
 __host__ __device__ inline DV<N> log_pos_v(const DV<N> &x) {
   DV<N> u, hi, lo, out;
@@ -109,14 +122,10 @@ __host__ __device__ inline DV<N> log_pos_v(const DV<N> &x) {
   const double *tab = logtab();
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    double m = frexp(x.v[i], &e[i]);  // [0.5, 1)
-    if (m < kSqrtHalf) {
-      m = m + m;
-      e[i] = e[i] - 1;
-    }
-    const double jd = rint(m * 128.0);  // 90..181
-    const double f = m - jd * 0.0078125;  // exact
-    const double *t = tab + ((int)jd - kLogTabJ0) * 3;
+    const double m = frexp(x.v[i], &e[i]);  // [1/2, 1)
+    const double jd = rint(m * 256.0);       // 128..256
+    const double f = fma(jd, -0.00390625, m);  // m - j/256, exact
+    const double *t = tab + ((int)jd - kLogTabJ0) * 4;
     u.v[i] = f * t[0];
     hi.v[i] = t[1];
     lo.v[i] = t[2];
@@ -138,35 +147,40 @@ __host__ __device__ inline DV<N> log_pos_v(const DV<N> &x) {
   return out;
 }
 
-// exp(z) for |z| < 1.4e6 (k ln2_hi exact for |k| < 2^21), N values in
-// lockstep: z = k ln2 + r with |r| <= ln2/2, exp(r) by its Taylor series
-// through r^13 (truncation below 5e-18 relative), scaled by 2^k; ldexp
-// gives 0 and +inf beyond the double range.  The chain's arguments
-// y log x stay within |z| < 1e4 (x >= 5e-324, y <= 10).
+// exp(z) for |z| < 5e6 (n fits an int; beyond |z| = 745 the result is 0 or
+// +inf), N values in lockstep, table-driven (the method of Tang, ACM TOMS 15 (1989) 144-157):
+// n = RN(z 256 / ln2), z = n ln2/256 + r with |r| <= ln2/512 (ln2/256 split
+// as ln2_hi/256 + ln2_lo/256; fma keeps r's rounding to its own), and
+//   exp(z) = 2^(n >> 8) 2^((n & 255)/256) exp(r),
+// 2^(j/256) from mathtab.h as hi + lo, exp(r) - 1 by its Taylor series
+// through r^4 (truncation below 4e-17 relative), the sum formed as
+// hi + (lo + hi (exp(r) - 1)); ldexp gives 0 and +inf beyond the double
+// range.  Within 1 ulp.  The chain's arguments y log x stay within |z| < 1e4
+// (x >= 5e-324, y <= 10).
 template <int N>
 __host__ __device__ inline DV<N> exp_fin_v(const DV<N> &zin) {
-  DV<N> k, r, p, out;
+  constexpr double kInvLn2x256 = 0x1.71547652b82fep+8;  // RN(256 / ln 2)
+  DV<N> r, th, tl, q, out;
+  int sc[N];
+  const double *tab = exptab();
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const double z = zin.v[i];
-    k.v[i] = rint(z * kInvLn2);
-    r.v[i] = fma(-k.v[i], kLn2Lo, fma(-k.v[i], kLn2Hi, z));
-    p.v[i] = 1.0 / 6227020800.0;  // 1/13!
+    const double n = rint(z * kInvLn2x256);
+    r.v[i] = fma(-n, kLn2Lo * 0.00390625, fma(-n, kLn2Hi * 0.00390625, z));
+    const int ni = (int)n;
+    sc[i] = ni >> 8;  // floor(n / 256)
+    const double *t = tab + (ni & 255) * 2;
+    th.v[i] = t[0];
+    tl.v[i] = t[1];
   }
-  constexpr double kE[10] = {1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0,
-                             1.0 / 362880.0,    1.0 / 40320.0,    1.0 / 5040.0,
-                             1.0 / 720.0,       1.0 / 120.0,      1.0 / 24.0,
-                             1.0 / 6.0};
 #pragma unroll
-  for (int j = 0; j < 10; ++j)
-#pragma unroll
-    for (int i = 0; i < N; ++i) p.v[i] = fma_k(p.v[i], r.v[i], kE[j]);
+  for (int i = 0; i < N; ++i) q.v[i] = fma_k(r.v[i], 1.0 / 24.0, 1.0 / 6.0);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    double q = fma(p.v[i], r.v[i], 0.5);
-    q = fma(q, r.v[i], 1.0);
-    q = fma(q, r.v[i], 1.0);
-    out.v[i] = ldexp(q, (int)k.v[i]);
+    q.v[i] = fma(q.v[i], r.v[i], 0.5);
+    const double pm = fma(r.v[i] * r.v[i], q.v[i], r.v[i]);  // exp(r) - 1
+    out.v[i] = ldexp(th.v[i] + fma(th.v[i], pm, tl.v[i]), sc[i]);
   }
   return out;
 }

@@ -416,7 +416,7 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
 // Plain path: one launch per model step; every state and flux is written.
 template <int NH, bool JV, bool OUT>
 __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, int64_t t) {
-  logtab_fill();  // the chain's powers read it (before any thread returns)
+  mathtab_fill();  // the chain's log and exp read it (before any thread returns)
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
   const uint32_t bc = c * 8u;
@@ -475,7 +475,7 @@ template <int NH, bool JV, bool OUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHMHIP_FUSED_WAVES)))
 void cell_fused_kernel(CellArgs a, int64_t t0, int32_t nsteps,
                                                          int32_t step0) {
-  logtab_fill();  // the chain's powers read it (before any thread returns)
+  mathtab_fill();  // the chain's log and exp read it (before any thread returns)
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
   const uint32_t bc = c * 8u;

@@ -30,7 +30,7 @@ __host__ __device__ inline double eval(int which, const double *x, const double 
 }
 
 __global__ void eval_kernel(int which, const double *x, const double *y, double *out, int64_t n) {
-  mhmhip::logtab_fill();
+  mhmhip::mathtab_fill();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = eval(which, x, y, i, n);
 }
