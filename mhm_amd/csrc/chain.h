@@ -218,8 +218,19 @@ __host__ __device__ inline double exp_ylogx(double y, double x) {
 __host__ __device__ inline double pow23(double x) {
   int e;
   const double m = frexp(x, &e);  // [1/2, 1)
-  const int q = (e + 1200) / 3 - 400;  // floor(e / 3) for e > -1200
-  const double a = ldexp(m, e - 3 * q);
+  // q = floor(e / 3) and e - 3q for the exponents of finite doubles: with
+  // u = e + 1200 in [126, 2224], floor(u / 3) = (u * 43691) >> 17 (exact for
+  // u < 2^16), so both are 24-bit products (single-pass on the device, where
+  // a 32-bit divide by 3 takes multi-pass multiplies)
+  const uint32_t u = (uint32_t)(e + 1200);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const int q = (int)(__umul24(u, 43691u) >> 17) - 400;
+  const int er = e - __mul24(q + 400, 3) + 1200;
+#else
+  const int q = (int)((u * 43691u) >> 17) - 400;
+  const int er = e - 3 * (q + 400) + 1200;
+#endif
+  const double a = ldexp(m, er);
 #if defined(__HIP_DEVICE_COMPILE__)
   const double t =
       (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)a) * (1.0f / 3.0f));
@@ -275,13 +286,38 @@ __device__ __forceinline__ bool le_eps(double a, double b) {
 //   if (s > e) then s = s - e else e = s; s = 0          ->  e = min(e, s), s = s - e
 // They give the same value for every non-NaN operand (a zero may come out
 // with the other sign, which no later operation of the chain tells apart).
-__device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
-__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
+// On the device they are the min / max instructions themselves: written as
+// fmin / fmax, the compiler first quiets a possible signalling NaN in each
+// operand it cannot prove canonical (an extra v_max_f64 per loaded value);
+// the chain holds no signalling NaN, and for every other operand the result
+// is the same.
+__device__ __forceinline__ double vmin(double a, double b) {
+  double d;
+  asm("v_min_f64 %0, %1, %2" : "=v"(d) : "v"(a), "v"(b));
+  return d;
+}
+__device__ __forceinline__ double vmax(double a, double b) {
+  double d;
+  asm("v_max_f64 %0, %1, %2" : "=v"(d) : "v"(a), "v"(b));
+  return d;
+}
+// max(a, 0) with the zero an inline constant, and max(a, k) for a constant k
+// passed in a scalar register pair
+__device__ __forceinline__ double vmax0(double a) {
+  double d;
+  asm("v_max_f64 %0, %1, 0" : "=v"(d) : "v"(a));
+  return d;
+}
+__device__ __forceinline__ double vmax_k(double a, double k) {
+  double d;
+  asm("v_max_f64 %0, %1, %2" : "=v"(d) : "v"(a), "s"(k));
+  return d;
+}
 
 __device__ __forceinline__ double canopy_fill(double interc_max, double rmax, double precip,
                                               double &interc, double &throughfall) {
   const double aux = interc + precip;
-  throughfall = vmax(aux - interc_max, 0.0);
+  throughfall = vmax0(aux - interc_max);
   interc = vmin(aux, interc_max);
   return interc_max > kEps ? div_rcp(interc, interc_max, rmax) : 1.0;
 }
@@ -293,7 +329,7 @@ __device__ __forceinline__ void canopy_evap(double pet, double interc_max, doubl
     evap_canopy = pet * pow_pos_from(ratio, pw);
   else
     evap_canopy = 0.0;
-  evap_canopy = vmin(vmax(evap_canopy, 0.0), interc);
+  evap_canopy = vmin(vmax0(evap_canopy), interc);
   interc = interc - evap_canopy;
 }
 
@@ -369,12 +405,12 @@ __device__ __forceinline__ void soil_moisture(double fS, double thrS, double rth
   aet_sealed = 0.0;
   if (fS > 0.0) {
     const double tmp = storage_sealed + peff;
-    runoff_sealed = vmax(tmp - thrS, 0.0);
+    runoff_sealed = vmax0(tmp - thrS);
     storage_sealed = vmin(tmp, thrS);
     if (thrS > kEps) {
       // revap = RN(1/evap): pet / evap as div_rcp, same bits (evap > 0)
       const double pe = div_rcp(pet, evap, revap);
-      aet_sealed = vmax((pe - ecan) * div_rcp(storage_sealed, thrS, rthrS), 0.0);
+      aet_sealed = vmax0((pe - ecan) * div_rcp(storage_sealed, thrS, rthrS));
     } else {
       aet_sealed = DBL_MAX;  // huge(1.0_dp)
     }
@@ -406,7 +442,7 @@ __device__ __forceinline__ void soil_moisture(double fS, double thrS, double rth
     if (h != 0) {
       double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < h; ++j) s = s + vmax(aet[j], 0.0);  // adds only the positive ones
+      for (int j = 0; j < h; ++j) s = s + vmax0(aet[j]);  // adds only the positive ones
       a = a - s;
     }
     double f;
@@ -414,14 +450,14 @@ __device__ __forceinline__ void soil_moisture(double fS, double thrS, double rth
       f = jarvis(sm[h], sat[h], wp[h], fr[h], c1);
     else
       f = feddes(sm[h], fc[h], wp[h], fr[h], rfw[h]);
-    a = vmax(a * f, 0.0);
+    a = vmax0(a * f);
     if (sm[h] > a) {
       sm[h] = sm[h] - a;
     } else {
       a = sm[h] - kEps;
       sm[h] = kEps;
     }
-    sm[h] = vmax(sm[h], kEps);
+    sm[h] = vmax_k(sm[h], kEps);
     aet[h] = a;
   }
 }
@@ -462,7 +498,7 @@ __device__ __forceinline__ void runoff_unsat_zone(double k1, double kp, double k
 
 // src/mHM/mo_runoff.f90:191-212
 __device__ __forceinline__ void runoff_sat_zone(double k2, double &sat, double &baseflow) {
-  const double s0 = vmax(sat, 0.0);  // an empty or negative store drains nothing, ends at 0
+  const double s0 = vmax0(sat);  // an empty or negative store drains nothing, ends at 0
   baseflow = k2 * s0;
   sat = s0 - baseflow;
 }

@@ -262,7 +262,7 @@ __device__ __forceinline__ double pet_day(const CellArgs &a, const StepDev &st, 
                                        st.cosd);
   if (a.pet_case >= 2) {
     // Priestley-Taylor and Penman-Monteith are not defined for netrad < 0
-    const double rn = vmax(ldr(frow_ptr(a, MHMHIP_F_NETRAD, st.day), bc), 0.0);
+    const double rn = vmax0(ldr(frow_ptr(a, MHMHIP_F_NETRAD, st.day), bc));
     if (a.pet_case == 2) return pet_priestly(P(pr_prietay(NH) + st.month0), rn, tavg);
     const double avp = ldr(frow_ptr(a, MHMHIP_F_ABSVAPPRESS, st.day), bc) / 1000.0;
     const double ws = ldr(frow_ptr(a, MHMHIP_F_WINDSPEED, st.day), bc);
