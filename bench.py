@@ -321,8 +321,10 @@ def bench_multi_basin(N, workloads, S):
             "setup_s": round(setup, 1), "kernels": kernel_table(mprof, S, 24)}
 
 
-def bench_serving(serve, n_req, qps, seed=11):
-    reqs = serve.poisson_requests(n_req, qps=qps, seed=seed)
+def bench_serving(serve, n_req, qps, pool, seed=11):
+    t0 = time.perf_counter()
+    reqs = serve.poisson_requests(n_req, qps=qps, seed=seed, pool=pool)
+    gen_s = time.perf_counter() - t0
     # warm the code paths (module loads, first context) outside the stream
     serve.run_alone(serve.poisson_requests(1, qps=1.0, seed=99, cells=(1000, 1000),
                                            horizon=(30, 30), sets=(1, 1))[0])
@@ -331,8 +333,10 @@ def bench_serving(serve, n_req, qps, seed=11):
     s = serve.summarize(res, reqs, srv.wall, srv.timings)
     s.update({"qps": qps, "peak_slots": srv.peak_slots,
               "peak_admitted_GiB": round(srv.peak_bytes / 2**30, 2),
-              "workload": f"{n_req} Poisson arrivals at {qps} req/s: log-uniform 1k..100k cells, "
-                          "30..365 daily steps, 1..64 parameter sets; budget 64 GiB, 8 slots"})
+              "basin_pool": pool, "generate_s": round(gen_s, 1),
+              "workload": f"{n_req} Poisson arrivals at {qps} req/s: log-uniform 1k..100k cells "
+                          f"(basins drawn from {pool} pre-built ones), 30..365 daily steps, "
+                          "1..64 parameter sets; budget 64 GiB, 8 slots"})
     return s
 
 
@@ -481,7 +485,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=241)
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=500)
+    ap.add_argument("--serve-pool", type=int, default=32,
+                    help="pre-built basins the serving requests draw from")
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--pmc-workload", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
@@ -583,7 +589,8 @@ def main():
         extra["multi_basin"] = bench_multi_basin(N, workloads, S)
         log(f"multi_basin: {extra['multi_basin']['value']:.4g}")
     if not args.no_serve:
-        extra["serving"] = bench_serving(serve, args.serve_requests, args.serve_qps)
+        extra["serving"] = bench_serving(serve, args.serve_requests, args.serve_qps,
+                                         args.serve_pool)
         log(f"serving: {json.dumps(extra['serving'])}")
 
     # The oracle last: its OpenMP runtime, started with OMP_PROC_BIND, pins

@@ -270,6 +270,61 @@ int mhmhip_set_forcing_l2(mhmhip_ctx *ctx, int32_t var, int32_t domain, int32_t 
 int mhmhip_set_meteo_weights(mhmhip_ctx *ctx, const double *pre_w, const double *temp_w,
                              const double *pet_w, int64_t ld);
 
+/* ---- MPR-lite on the device ---------------------------------------------------
+ * No counterpart in the reference, whose MPR (src/MPR/mo_multi_param_reg.f90:
+ * 67-654) builds the L1 fields from L0 maps and look-up tables and hands them
+ * to mHM() through the arrays mhmhip_set_cell_var fills.  For callers whose
+ * cells carry the synthetic domains' inputs (land-cover fractions, a soil
+ * texture index, aspect, latitude) and whose parameter sets are gamma vectors
+ * -- the serving loop -- this builds every parameter row of a set on the
+ * device from its gamma vector alone, the rows mhmhip_set_cell_var would
+ * store for the fields of the stand-in regionalisation (parameters.mpr_lite:
+ * the reference's transfer functions for the snow and sealed-storage fields,
+ * src/MPR/mo_multi_param_reg.f90:800-852, 883-894; documented stand-ins for
+ * the soil, recession and PET-aspect fields), so a set costs one launch
+ * instead of its fields' host computation and upload.  The gamma vector's
+ * order (mhm_parameter.nml's groups): */
+#define MHMHIP_MPRLITE_CANOPY 0               /* canopyInterceptionFactor        */
+#define MHMHIP_MPRLITE_SNOW_TEMP 1            /* snowTreshholdTemperature        */
+#define MHMHIP_MPRLITE_DDF_FOREST 2           /* degreeDayFactor_forest          */
+#define MHMHIP_MPRLITE_DDF_IMPERVIOUS 3       /* degreeDayFactor_impervious      */
+#define MHMHIP_MPRLITE_DDF_PERVIOUS 4         /* degreeDayFactor_pervious        */
+#define MHMHIP_MPRLITE_DDF_INCREASE 5         /* increaseDegreeDayFactorByPrecip */
+#define MHMHIP_MPRLITE_MAXDDF_FOREST 6        /* maxDegreeDayFactor_forest       */
+#define MHMHIP_MPRLITE_MAXDDF_IMPERVIOUS 7    /* maxDegreeDayFactor_impervious   */
+#define MHMHIP_MPRLITE_MAXDDF_PERVIOUS 8      /* maxDegreeDayFactor_pervious     */
+#define MHMHIP_MPRLITE_ROOT_FOREST 9          /* rootFractionCoefficient_forest  */
+#define MHMHIP_MPRLITE_ROOT_IMPERVIOUS 10     /* rootFractionCoefficient_impervious */
+#define MHMHIP_MPRLITE_ROOT_PERVIOUS 11       /* rootFractionCoefficient_pervious */
+#define MHMHIP_MPRLITE_INFIL_SHAPE 12         /* infiltrationShapeFactor         */
+#define MHMHIP_MPRLITE_JARVIS_C1 13           /* jarvis_sm_threshold_c1          */
+#define MHMHIP_MPRLITE_IMPERVIOUS_STORAGE 14  /* imperviousStorageCapacity       */
+#define MHMHIP_MPRLITE_PET_MIN 15             /* minCorrectionFactorPET          */
+#define MHMHIP_MPRLITE_PET_MAX 16             /* maxCorrectionFactorPET          */
+#define MHMHIP_MPRLITE_ASPECT_THRESH 17       /* aspectTresholdPET               */
+#define MHMHIP_MPRLITE_HARGREAVES 18          /* HargreavesSamaniCoeff           */
+#define MHMHIP_MPRLITE_INTERFLOW_CAPACITY 19  /* interflowStorageCapacityFactor  */
+#define MHMHIP_MPRLITE_INTERFLOW_SLOPE 20     /* interflowRecession_slope (unused) */
+#define MHMHIP_MPRLITE_FAST_FOREST 21         /* fastInterflowRecession_forest   */
+#define MHMHIP_MPRLITE_SLOW_KS 22             /* slowInterflowRecession_Ks       */
+#define MHMHIP_MPRLITE_EXP_SLOW 23            /* exponentSlowInterflow           */
+#define MHMHIP_MPRLITE_RECHARGE 24            /* rechargeCoefficient             */
+#define MHMHIP_MPRLITE_GEOPARAM 25            /* GeoParam                        */
+#define MHMHIP_MPRLITE_NGAMMA 26              /* (the 5 routing gammas follow:
+                                                 mhmhip_set_routing_params)     */
+/* The cells' inputs, kept by the context (host copies, laid out on the
+ * device as the cells are): cells is X(ld, 6) column-major over all cells:
+ * forest, impervious and pervious fractions, soil texture index in [0, 1),
+ * aspect [deg], latitude [deg]; horizon_depth[n_horizons] the horizons'
+ * lower boundaries [mm]; lai_class[3][12] the monthly leaf area index of the
+ * three cover classes. */
+int mhmhip_mprlite_inputs(mhmhip_ctx *ctx, const double *cells, int64_t ld,
+                          const double *horizon_depth, const double *lai_class);
+/* Every parameter row of one set from its gamma vector (the first
+ * MHMHIP_MPRLITE_NGAMMA entries, in the order above), on the context's
+ * stream; the process_pet 2 and 3 rows too when the context has that case. */
+int mhmhip_mprlite_params(mhmhip_ctx *ctx, int32_t set, const double *gamma);
+
 /* ---- one step ----------------------------------------------------------------
  * get_corrected_pet + get_temp + get_prec + mHM() for every cell and set:
  * src/mHM/mo_mhm_interface_run.f90:357-457 calling

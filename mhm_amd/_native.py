@@ -106,6 +106,8 @@ SIGNATURES = {
     "mhmhip_get_output_sum": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp,
                                         C.POINTER(C.c_int64)]),
     "mhmhip_set_meteo_weights": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int64]),
+    "mhmhip_mprlite_inputs": (C.c_int, [C.c_void_p, _dp, C.c_int64, _dp, _dp]),
+    "mhmhip_mprlite_params": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mhmhip_run": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts), _dp]),
     "mhmhip_run_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Step), C.c_int32, C.POINTER(RunOpts),
                                      _dp]),
@@ -113,6 +115,7 @@ SIGNATURES = {
     "mhmhip_get_profile": (C.c_int, [C.c_void_p, C.c_int32, _dp, _lp, _dp]),
 }
 K_NAMES = ["cell_step", "cell_fused", "route_diag", "route_wg", "route_tile"]
+MPRLITE_NGAMMA = 26  # gamma entries mhmhip_mprlite_params reads (the routing ones follow)
 
 
 class Plan(C.Structure):

@@ -82,6 +82,14 @@ struct mhmhip_ctx {
   int32_t frow[MHMHIP_F_COUNT] = {};
   int32_t n_frows = 0;
   double *W = nullptr;  // meteo weights [3][12 * 24][sstride], null: off
+  // MPR-lite inputs (mhmhip_mprlite_inputs): host copies in reference cell
+  // order, and the kernel's input rows laid out as the cells are (rebuilt
+  // from the host copies after a network binding moves the cells)
+  std::vector<double> mpr_cells;  // [6][n_cells]
+  double mpr_hd[MHMHIP_MAX_HORIZONS] = {};
+  double mpr_lai[36] = {};
+  double *d_mpr = nullptr;        // [8][sstride]
+  bool mpr_stale = true;
   // whether P, S, X and F hold anything written since creation (a network
   // bound before any data re-lays nothing)
   bool p_dirty = false, s_dirty = false, x_dirty = false, f_dirty = false;
@@ -508,6 +516,7 @@ extern "C" int mhmhip_destroy(mhmhip_ctx *x) {
     if (x->fstage_ev[i]) (void)hipEventDestroy(x->fstage_ev[i]);
   }
   (void)hipFree(x->d_pos);
+  (void)hipFree(x->d_mpr);
   for (hipEvent_t e : x->chunk_ev) (void)hipEventDestroy(e);
   if (x->cstream) (void)hipStreamDestroy(x->cstream);
   for (auto &p : x->pending) {
@@ -1025,6 +1034,63 @@ extern "C" int mhmhip_set_forcing_l2(mhmhip_ctx *x, int32_t var, int32_t domain,
   CHECK_HIP(x, launch_meteo_l2(b.data2, b.mask2, nr2, n2, b.cells, ncells, ndays, dst, x->sstride,
                                aggregate, x->stream), "set_forcing_l2 map");
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "set_forcing_l2");
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_mprlite_inputs(mhmhip_ctx *x, const double *cells, int64_t ld,
+                                    const double *hd, const double *lai_class) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!cells || !hd || !lai_class || ld < x->n_cells)
+    return x->fail(MHMHIP_EINVAL, "mprlite_inputs: bad pointer or leading dimension");
+  for (int h = 0; h < x->nh; ++h)
+    if (!(hd[h] > (h ? hd[h - 1] : 0.0)))
+      return x->fail(MHMHIP_EINVAL, "mprlite_inputs: horizon depths must increase from above 0");
+  x->mpr_cells.resize((size_t)6 * x->n_cells);
+  for (int k = 0; k < 6; ++k)
+    std::copy(cells + k * ld, cells + k * ld + x->n_cells, x->mpr_cells.begin() + k * x->n_cells);
+  std::copy(hd, hd + x->nh, x->mpr_hd);
+  std::copy(lai_class, lai_class + 36, x->mpr_lai);
+  x->mpr_stale = true;
+  return MHMHIP_OK;
+}
+
+extern "C" int mhmhip_mprlite_params(mhmhip_ctx *x, int32_t set, const double *gamma) {
+  if (!x) return MHMHIP_EINVAL;
+  if (!gamma || set < 0 || set >= x->n_sets)
+    return x->fail(MHMHIP_EINVAL, "mprlite_params: bad pointer or set");
+  if (x->mpr_cells.empty())
+    return x->fail(MHMHIP_ESTATE, "mprlite_params: no inputs (call mprlite_inputs first)");
+  CHECK_HIP(x, hipSetDevice(x->cfg.device), "mprlite_params");
+  const int64_t n = x->n_cells;
+  if (x->mpr_stale) {
+    if (!x->d_mpr) CHECK_RC(x->dalloc(&x->d_mpr, 8 * x->sstride, "alloc MPR-lite inputs"));
+    auto id = [](double v) { return v; };
+    for (int k = 0; k < 5; ++k)
+      CHECK_RC(put_row(x, x->d_mpr + k * x->sstride, x->mpr_cells.data() + k * n, id));
+    // the latitude as mhmhip_set_cell_var stores it (mo_pet.f90:113, 331-353)
+    const double *lat = x->mpr_cells.data() + 5 * n;
+    CHECK_RC(put_row(x, x->d_mpr + 5 * x->sstride, lat,
+                     [](double v) { return std::tan(kDeg2Rad * v); }));
+    CHECK_RC(put_row(x, x->d_mpr + 6 * x->sstride, lat,
+                     [](double v) { return std::sin(kDeg2Rad * v); }));
+    CHECK_RC(put_row(x, x->d_mpr + 7 * x->sstride, lat,
+                     [](double v) { return std::cos(kDeg2Rad * v); }));
+    x->mpr_stale = false;
+  }
+  MprLiteArgs a{};
+  a.P = x->P;
+  a.in = x->d_mpr;
+  a.ld = x->ld;
+  a.sstride = x->sstride;
+  a.n_cells = n;
+  a.nh = x->nh;
+  a.pet_case = x->cfg.process_pet;
+  a.c2 = x->c2TSTu;
+  std::copy(x->mpr_hd, x->mpr_hd + x->nh, a.hd);
+  std::copy(x->mpr_lai, x->mpr_lai + 36, a.lai);
+  std::copy(gamma, gamma + kMprNGamma, a.gamma);
+  x->p_dirty = true;
+  CHECK_HIP(x, launch_mprlite(a, set, x->stream), "mprlite_params");
   return MHMHIP_OK;
 }
 
@@ -1594,6 +1660,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   }
   x->pos = cpos;
   x->d_pos_stale = true;
+  if (moved) x->mpr_stale = true;
   const int64_t old_nodes = x->n_nodes;
 
   // per-link host copies for reg_rout, packed by domain

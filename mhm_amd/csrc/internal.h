@@ -262,6 +262,22 @@ struct RouteArgs {
   int32_t pad;
 };
 
+// MPR-lite on the device: the synthetic domains' stand-in regionalisation
+// (one set's parameter rows from its gamma vector and the cell inputs).
+constexpr int kMprNGamma = MHMHIP_MPRLITE_NGAMMA;
+struct MprLiteArgs {
+  double *P;               // parameter rows of set 0 (rows ld apart)
+  const double *in;        // input rows (sstride apart): forest, impervious,
+                           // pervious fraction, texture, aspect [deg], and the
+                           // tan, sin, cos of the latitude
+  int64_t ld, sstride, n_cells;
+  int32_t nh, pet_case;
+  double c2;               // c2TSTu (timestep / 24)
+  double hd[MHMHIP_MAX_HORIZONS];  // horizon lower boundaries [mm]
+  double lai[36];          // monthly LAI of forest, impervious, pervious
+  double gamma[kMprNGamma];
+};
+
 // Extra arguments of the tiled routing kernel.
 struct TileArgs {
   const SlotMeta *slot;    // [n_bins * kTileSlots]

@@ -1233,4 +1233,107 @@ hipError_t launch_route_wg(const RouteArgs &r, const int64_t *dom_start, const i
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// MPR-lite on the device: every parameter row of one set from its gamma
+// vector (the order of mhmhip.h's MHMHIP_MPRLITE_* indices) and the cells'
+// land-cover fractions, soil texture, aspect and latitude -- the synthetic
+// domains' stand-in regionalisation, the same operations in the same order
+// as its numpy statement (parameters.mpr_lite with static_fields and
+// pet_radiation_params), so every row but the root fractions has the host's
+// bits (those take c**(z/10) from the chain's power instead of libm's pow).
+// The rows hold what mhmhip_set_cell_var stores for those fields: the
+// degree-day rows times c2TSTu, the recession rows as c2TSTu / k.  The snow
+// and sealed-threshold fields follow the reference's transfer functions
+// (src/MPR/mo_multi_param_reg.f90:800-852, 883-894). One lane per cell.
+__global__ __launch_bounds__(256) void mprlite_kernel(MprLiteArgs a, int32_t set) {
+  mathtab_fill();  // the root fractions' power reads it (before any thread returns)
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n_cells) return;
+  const int64_t S = a.sstride;
+  const double fF = a.in[c], fI = a.in[S + c], fP = a.in[2 * S + c];
+  const double tx = a.in[3 * S + c], asp = a.in[4 * S + c];
+  double *P = a.P + (int64_t)set * S + c;
+  auto put = [&](int row, double v) { P[(int64_t)row * a.ld] = v; };
+  const double *g = a.gamma;
+  const int nh = a.nh;
+  const double c2 = a.c2;
+  put(PR_FS, fI);
+  // snow_acc_melt_param: degree-day factors per cover class, then the
+  // class-weighted cell values
+  const double ddf_f = g[MHMHIP_MPRLITE_DDF_FOREST];
+  const double ddf_p = ddf_f + g[MHMHIP_MPRLITE_DDF_PERVIOUS];
+  const double ddf_i = ddf_p + g[MHMHIP_MPRLITE_DDF_IMPERVIOUS];
+  const double max_f = ddf_f + g[MHMHIP_MPRLITE_MAXDDF_FOREST];
+  const double max_i = ddf_i + g[MHMHIP_MPRLITE_MAXDDF_IMPERVIOUS];
+  const double max_p = ddf_p + g[MHMHIP_MPRLITE_MAXDDF_PERVIOUS];
+  put(PR_TTH, g[MHMHIP_MPRLITE_SNOW_TEMP]);
+  put(PR_INCR, g[MHMHIP_MPRLITE_DDF_INCREASE]);
+  put(PR_DNPS, (ddf_f * fF + ddf_i * fI + ddf_p * fP) * c2);
+  put(PR_DMAXS, (max_f * fF + max_i * fI + max_p * fP) * c2);
+  put(PR_SEALTHR, g[MHMHIP_MPRLITE_IMPERVIOUS_STORAGE]);
+  // monthly leaf area index: interception capacity, PET-LAI factor and the
+  // radiation cases' fields
+  for (int m = 0; m < 12; ++m) {
+    const double lai = fF * a.lai[m] + fI * a.lai[12 + m] + fP * a.lai[24 + m];
+    put(pr_maxinter(nh) + m, g[MHMHIP_MPRLITE_CANOPY] * lai);
+    put(pr_petlai(nh) + m, fmin(fmax(0.6 + 0.08 * lai, 0.3), 1.3));
+    if (a.pet_case == 2) put(pr_prietay(nh) + m, 1.05 + 0.06 * lai);
+    if (a.pet_case == 3) {
+      put(pr_aero(nh) + m, 208.0 / (0.6 + 0.1 * lai));
+      put(pr_surf(nh) + m, 40.0 + 120.0 / (1.0 + lai));
+    }
+  }
+  // soil water contents per horizon
+  const double poro = 0.38 + 0.12 * tx, fcr = 0.55 + 0.25 * (1.0 - tx);
+  const double wpr = 0.30 + 0.25 * tx;
+  const double sexp = g[MHMHIP_MPRLITE_INFIL_SHAPE] * (1.0 + 0.3 * tx);
+  double rc = fF * g[MHMHIP_MPRLITE_ROOT_FOREST] + fI * g[MHMHIP_MPRLITE_ROOT_IMPERVIOUS] +
+              fP * g[MHMHIP_MPRLITE_ROOT_PERVIOUS];
+  rc = fmin(fmax(rc, 1e-3), 0.999);
+  double fr[MHMHIP_MAX_HORIZONS];
+  double cum_top = 0.0, fsum = 0.0, top = 0.0;
+  for (int h = 0; h < nh; ++h) {
+    const double dz = a.hd[h] - top;
+    top = a.hd[h];
+    const double sat = dz * poro, fc = sat * fcr;
+    put(pr_sat(nh) + h, sat);
+    put(pr_fc(nh) + h, fc);
+    put(pr_wp(nh) + h, fc * wpr);
+    put(pr_exp(nh) + h, sexp);
+    // root fraction: cumulative 1 - c**(z / 10 mm), differenced, normalised
+    const double cum = 1.0 - exp_ylogx(a.hd[h] / 10.0, rc);
+    fr[h] = fmax(cum - cum_top, 0.0);
+    cum_top = cum;
+    fsum = fsum + fr[h];
+  }
+  const double fden = fmax(fsum, 1e-12);
+  for (int h = 0; h < nh; ++h) put(pr_froots(nh) + h, fr[h] / fden);
+  put(PR_JARVIS, g[MHMHIP_MPRLITE_JARVIS_C1]);
+  // recession constants [d], stored as c2TSTu / k
+  const double k1 = g[MHMHIP_MPRLITE_SLOW_KS] * (0.5 + tx);
+  put(PR_K0S, c2 / (g[MHMHIP_MPRLITE_FAST_FOREST] * (1.0 + 0.5 * (1.0 - fF))));
+  put(PR_K1S, c2 / k1);
+  put(PR_KPS, c2 / fmax(g[MHMHIP_MPRLITE_RECHARGE] * (0.5 + tx), 2.0));
+  put(PR_K2S, c2 / fmax(g[MHMHIP_MPRLITE_GEOPARAM] * (0.5 + 0.5 * tx), k1));
+  put(PR_ALPHA, g[MHMHIP_MPRLITE_EXP_SLOW]);
+  put(PR_KARST, 1.0);
+  put(PR_UNSATTHR, g[MHMHIP_MPRLITE_INTERFLOW_CAPACITY] * (0.2 + 0.3 * tx));
+  // PET aspect correction
+  const double thr = g[MHMHIP_MPRLITE_ASPECT_THRESH], mn = g[MHMHIP_MPRLITE_PET_MIN];
+  const double mx = g[MHMHIP_MPRLITE_PET_MAX];
+  put(PR_FASP, asp < thr ? mn + mx * (thr - asp) / thr : mn + mx * (asp - thr) / (360.0 - thr));
+  put(PR_HARSAM, g[MHMHIP_MPRLITE_HARGREAVES]);
+  put(PR_TANLAT, a.in[5 * S + c]);
+  put(PR_SINLAT, a.in[6 * S + c]);
+  put(PR_COSLAT, a.in[7 * S + c]);
+}
+
+hipError_t launch_mprlite(const MprLiteArgs &a, int32_t set, hipStream_t s) {
+  if (a.n_cells < 1 || a.nh < 1 || a.nh > MHMHIP_MAX_HORIZONS || !a.P || !a.in || set < 0)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mprlite_kernel, dim3((unsigned)((a.n_cells + 255) / 256)), dim3(256), 0, s, a,
+                     set);
+  return hipGetLastError();
+}
+
 }  // namespace mhmhip

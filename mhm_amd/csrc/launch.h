@@ -31,6 +31,7 @@ hipError_t launch_runoff_agg(const double *tot, int64_t tot_rows, const int32_t 
                              const double *area, int64_t n_nodes, int64_t sstride, int64_t ld,
                              int n_sets, double *ring, const uint8_t *lag, int64_t ring_len,
                              int32_t rep, int64_t t0, int32_t nsteps, double tst, hipStream_t s);
+hipError_t launch_mprlite(const MprLiteArgs &a, int32_t set, hipStream_t s);
 hipError_t launch_route_wg(const RouteArgs &r, const int64_t *dom_start, const int32_t *dom_dmax,
                            int n_domains, int64_t d_lo, int64_t d_hi, int drain, int block,
                            hipStream_t s);

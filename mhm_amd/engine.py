@@ -150,6 +150,28 @@ class Model:
             self._ctx, N.F[name], int(domain), int(day0), nd, N.dptr(d2), nr2, nc2,
             float(cellsize2), m2.ctypes.data, nr1, nc1, float(cellsize1), m1.ctypes.data))
 
+    def set_mprlite_inputs(self):
+        """The packed domains' MPR-lite inputs (cover fractions, soil texture,
+        aspect, latitude; the first domain's horizon depths; the cover
+        classes' monthly LAI) for mprlite_params."""
+        from . import parameters as PM
+        d = self.domains
+        cols = [np.concatenate([x.fractions[k] for x in d]) for k in range(3)]
+        cols += [np.concatenate([getattr(x, f) for x in d]) for f in ("texture", "aspect", "lat")]
+        cells = np.ascontiguousarray(np.stack(cols), dtype=np.float64)  # X(n_cells, 6)
+        hd = np.ascontiguousarray(d[0].horizon_depth, dtype=np.float64)
+        lai = np.ascontiguousarray(PM.LAI, dtype=np.float64)
+        self._chk(N.lib().mhmhip_mprlite_inputs(self._ctx, N.dptr(cells), self.n_cells,
+                                                N.dptr(hd), N.dptr(lai)))
+
+    def mprlite_params(self, gamma, set_=0):
+        """Every parameter row of set_ built on the device from a gamma vector
+        (parameters.GAMMA order; the routing entries are not read)."""
+        g = np.ascontiguousarray(np.asarray(gamma, dtype=np.float64)[:N.MPRLITE_NGAMMA])
+        if g.shape[0] != N.MPRLITE_NGAMMA:
+            raise ValueError(f"gamma needs {N.MPRLITE_NGAMMA} entries")
+        self._chk(N.lib().mhmhip_mprlite_params(self._ctx, set_, N.dptr(g)))
+
     def set_meteo_weights(self, pre_w=None, temp_w=None, pet_w=None):
         """Disaggregation by weights (read_meteo_weights): each (n_cells, 12, 24)
         = L1_x_weights(cell, month, hour + 1) of all packed cells; no
@@ -163,11 +185,12 @@ class Model:
         self._chk(N.lib().mhmhip_set_meteo_weights(
             self._ctx, *[w.ctypes.data_as(N._dp) for w in ws], self.n_cells))
 
-    def set_domain_forcing(self, day0=1):
-        """Upload every domain's own forcing (synth.make_forcing) for all packed cells."""
+    def set_domain_forcing(self, day0=1, n_days=None):
+        """Upload every domain's own forcing (synth.make_forcing) for all packed
+        cells: all of its days, or its first n_days."""
         for name in ("PRE", "TAVG", "TMIN", "TMAX", "NETRAD", "ABSVAPPRESS", "WINDSPEED"):
             if all(name in d.forcing for d in self.domains):
-                data = np.concatenate([d.forcing[name] for d in self.domains], axis=1)
+                data = np.concatenate([d.forcing[name][:n_days] for d in self.domains], axis=1)
                 self.set_forcing(name, data, day0)
 
     # -- network -----------------------------------------------------------------

@@ -16,8 +16,6 @@ parameter-set counts arrive over time and are run with continuous batching.
 
 Latency = completion - arrival; p50/p99 are reported beside cell-steps/s.
 """
-from collections import deque
-from concurrent.futures import ThreadPoolExecutor
 import math
 import queue
 import threading
@@ -63,8 +61,6 @@ TILE_HEIGHT = 16  # levels per routing tile; the runoff ring has this many rows 
 
 
 STEP_BYTES = 104  # one step of the device step table (StepDev)
-PREP_WORKERS = 6  # threads building one request's parameter fields
-PREP_AHEAD = 8    # parameter sets built ahead of their upload (host memory bound)
 # forcing rows a context holds per PET case (pre, tavg and what the case reads)
 FORCING_ROWS = {-1: 3, 0: 3, 1: 4, 2: 3, 3: 5}
 
@@ -77,7 +73,9 @@ def estimate_bytes(n_cells, n_sets, n_horizons, n_forcing_days, plan, chunk_step
     sstride = (n_cells + 63) // 64 * 64
     ld = sstride * n_sets
     rows = (19 + 5 * n_horizons + 60) + (5 + n_horizons) + (17 + 2 * n_horizons)
-    cells = rows * ld * 8 + FORCING_ROWS[process_pet] * n_forcing_days * sstride * 8
+    # parameter, state and flux rows; forcing columns; MPR-lite's 8 input rows
+    cells = (rows * ld * 8 + FORCING_ROWS[process_pet] * n_forcing_days * sstride * 8
+             + 8 * sstride * 8)
     window = 8  # default routing window (mhmhip_run_opts)
     rows_run = min(chunk_steps, n_steps)  # steps of the longest call
     # ring rows of a call: (tile depth + 1) windows + S + 1, at most the
@@ -103,23 +101,15 @@ def prepare_model(req, device=0, timings=None):
     t1 = time.perf_counter()
     m.bind_network()
     t2 = time.perf_counter()
-    # the sets' parameter fields are independent host work (numpy releases the
-    # GIL in its array loops): built on worker threads, at most PREP_AHEAD sets
-    # ahead of the upload, and uploaded in order while the next ones are built
-    # (calls on one context are serialised)
-    n = len(req.gammas)
-    with ThreadPoolExecutor(max_workers=min(PREP_WORKERS, n)) as pool:
-        pending = deque()
-        nxt = 0
-        for s_ in range(n):
-            while nxt < n and len(pending) < PREP_AHEAD:
-                pending.append(pool.submit(synth.cell_params, dom, req.gammas[nxt]))
-                nxt += 1
-            m.set_params(pending.popleft().result(), s_)
-            m.set_routing_params(PM.routing_gamma(req.gammas[s_]), s_)
+    # the sets' parameter rows come from MPR-lite on the device: the cell
+    # inputs cross once, then each set is its gamma vector and one launch
+    m.set_mprlite_inputs()
+    for s_, g in enumerate(req.gammas):
+        m.mprlite_params(g, s_)
+        m.set_routing_params(PM.routing_gamma(g), s_)
     t3 = time.perf_counter()
     m.default_states()
-    m.set_domain_forcing()
+    m.set_domain_forcing(n_days=n_days)
     m.sync()
     if timings is not None:
         timings.update(create_s=t1 - t0, network_s=t2 - t1, params_s=t3 - t2,
@@ -261,21 +251,36 @@ class Server:
 
 
 def poisson_requests(n, qps, seed=0, cells=(1000, 100000), horizon=(30, 365), sets=(1, 64),
-                     timestep=24, start=None):
+                     timestep=24, start=None, pool=None):
     """Requests with Poisson arrivals: basin size log-uniform, horizon and
-    parameter-set count uniform within the given bounds."""
+    parameter-set count uniform within the given bounds.  pool: the requests
+    draw their basins (terrain, network, forcing over the longest horizon)
+    from that many pre-built ones, shared read-only, instead of one each --
+    long streams without a basin and a forcing year built per request."""
     import datetime as dt
     rng = np.random.default_rng(seed)
     start = start or dt.datetime(1990, 1, 1)
+
+    def basin(i, days):
+        nc = int(math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1]))))
+        nr = max(2, int(round(math.sqrt(nc))))
+        dom = synth.make_domain(nr, max(2, nc // nr), seed=seed * 10007 + i)
+        synth.make_forcing(dom, days, seed * 10007 + 5000 + i)
+        return dom
+
+    basins = None
+    if pool:
+        days = calendar.n_forcing_days_needed(horizon[1])
+        basins = [basin(i, days) for i in range(pool)]
     t = 0.0
     reqs = []
     for i in range(n):
         t += rng.exponential(1.0 / qps)
-        nc = int(math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1]))))
-        nr = max(2, int(round(math.sqrt(nc))))
-        dom = synth.make_domain(nr, max(2, nc // nr), seed=seed * 10007 + i)
         h = int(rng.integers(horizon[0], horizon[1] + 1))
-        synth.make_forcing(dom, calendar.n_forcing_days_needed(h), seed * 10007 + 5000 + i)
+        if basins is None:
+            dom = basin(i, calendar.n_forcing_days_needed(h))
+        else:
+            dom = basins[int(rng.integers(0, len(basins)))]
         k = int(rng.integers(sets[0], sets[1] + 1))
         gam = PM.draw(rng, k)
         reqs.append(Request(i, dom, gam, h, timestep, start, t))

@@ -77,3 +77,18 @@ def test_create_rejects_rows_beyond_32bit_offsets():
     cfg.evap_coeff[4] = 0.0  # and fails at the next check, before any device call
     assert L.mhmhip_create(C.byref(cfg), C.byref(ctx)) == N.EINVAL
     assert b"evap_coeff" in L.mhmhip_last_error(None)
+
+
+def test_mprlite_gamma_indices_follow_the_parameter_table():
+    """The header's MHMHIP_MPRLITE_* gamma indices name the entries of
+    parameters.GAMMA at those positions, and the count stops where the
+    routing group begins."""
+    from mhm_amd import parameters as PM
+    txt = open(os.path.join(ROOT, "include", "mhmhip.h")).read()
+    found = re.findall(r"#define MHMHIP_MPRLITE_(\w+) (\d+)\s+/\*\s*(\w+)", txt)
+    named = [(int(i), name) for key, i, name in found if key != "NGAMMA"]
+    assert len(named) == 26 and sorted(i for i, _ in named) == list(range(26))
+    for i, name in named:
+        assert PM.NAMES[i] == name, (i, name)
+    n = int(re.search(r"#define MHMHIP_MPRLITE_NGAMMA (\d+)", txt).group(1))
+    assert n == N.MPRLITE_NGAMMA == PM.NAMES.index(PM.ROUTING[0])

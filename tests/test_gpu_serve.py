@@ -14,7 +14,9 @@ pytestmark = pytest.mark.gpu
 
 def test_mix_equals_alone_and_oracle():
     print("case: serving mix", flush=True)
-    reqs = serve.poisson_requests(8, qps=40.0, seed=5, cells=(300, 3000), horizon=(20, 90),
+    # arrivals well inside one request's preparation time, so that requests
+    # are on the device together
+    reqs = serve.poisson_requests(8, qps=2000.0, seed=5, cells=(300, 3000), horizon=(20, 90),
                                   sets=(1, 6))
     srv = serve.Server(hbm_budget_bytes=2 << 30, max_slots=4, chunk_steps=30)
     res = srv.serve(reqs)
@@ -50,3 +52,18 @@ def test_footprint_estimate_matches_the_context():
         assert est == m.device_bytes, (r.rid, est, m.device_bytes)
         m.close()
 
+
+
+def test_mix_equals_alone_at_the_stream_bounds():
+    """Requests at the stream's upper bounds (50k..100k cells, 32..64
+    parameter sets) arriving together: each result in the mix is bitwise the
+    one it has alone."""
+    print("case: serving mix at 50k..100k cells x 32..64 sets", flush=True)
+    reqs = serve.poisson_requests(3, qps=1000.0, seed=17, cells=(50000, 100000),
+                                  horizon=(30, 45), sets=(32, 64))
+    srv = serve.Server(hbm_budget_bytes=64 << 30, max_slots=3, chunk_steps=73)
+    res = srv.serve(reqs)
+    assert srv.peak_slots >= 2, "no two requests were on the device together"
+    for r in reqs:
+        assert r.domain.n_cells >= 49000 and r.gammas.shape[0] >= 32
+        assert np.array_equal(res[r.rid].gauges, serve.run_alone(r, chunk_steps=73)), r.rid
