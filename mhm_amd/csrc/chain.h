@@ -85,33 +85,44 @@ struct DV {
 // independent dependency chains to interleave.  Rows are 4 doubles (32 B):
 // a row's address is one shift of j.
 #if defined(__HIP_DEVICE_COMPILE__)
-__shared__ __attribute__((aligned(16))) double s_logtab[kLogTabN * 4];
-__shared__ __attribute__((aligned(16))) double s_exptab[kExpTabN * 2];
+__shared__ __attribute__((aligned(16))) double s_mathtab[kLogTabN * 4 + kExpTabN * 2];
 #endif
 
 // Every workgroup of a kernel that evaluates a log or an exp calls this
-// first (all threads, before any of them returns).
+// before the first one (all threads, before any of them returns).  The
+// tables are copied in 16-byte pieces, a thread's first three loads issued
+// before its stores, so that the copy waits for memory once.
 __device__ inline void mathtab_fill() {
 #if defined(__HIP_DEVICE_COMPILE__)
-  for (int i = threadIdx.x; i < kLogTabN * 4; i += blockDim.x) s_logtab[i] = kLogTab[i];
-  for (int i = threadIdx.x; i < kExpTabN * 2; i += blockDim.x) s_exptab[i] = kExpTab[i];
+  constexpr int n = (kLogTabN * 4 + kExpTabN * 2) / 2;
+  const double2 *src = reinterpret_cast<const double2 *>(kMathTab);
+  double2 *dst = reinterpret_cast<double2 *>(s_mathtab);
+  const int i0 = threadIdx.x, bs = blockDim.x;
+  // pieces past the table copy its last piece again (no branch between the
+  // loads: the stores of one value to one place)
+  double2 v[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] = src[min(i0 + k * bs, n - 1)];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) dst[min(i0 + k * bs, n - 1)] = v[k];
+  for (int i = i0 + 3 * bs; i < n; i += bs) dst[i] = src[i];
   __syncthreads();
 #endif
 }
 
 __host__ __device__ inline const double *logtab() {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return s_logtab;
+  return s_mathtab;
 #else
-  return kLogTab;
+  return kMathTab;
 #endif
 }
 
 __host__ __device__ inline const double *exptab() {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return s_exptab;
+  return s_mathtab + kExpTabOff;
 #else
-  return kExpTab;
+  return kMathTab + kExpTabOff;
 #endif
 }
 

@@ -414,11 +414,13 @@ __device__ __forceinline__ void chain_step(const CellArgs &a, CellRegs<NH> &c, F
 
 // ---------------------------------------------------------------------------
 // Plain path: one launch per model step; every state and flux is written.
+// The cell's loads are issued before the log / exp tables are filled, so the
+// fill's reads and barrier wait behind them rather than in front.
 template <int NH, bool JV, bool OUT>
 __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, int64_t t) {
-  mathtab_fill();  // the chain's log and exp read it (before any thread returns)
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.n_cells) return;
+  const uint32_t c0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = c0 < a.n_cells;
+  const uint32_t c = act ? c0 : 0u;  // an idle lane loads cell 0 and leaves after the fill
   const uint32_t bc = c * 8u;
   const uint32_t bv = (uint32_t)(blockIdx.y * a.sstride + c) * 8u;
   CellRegs<NH> r;
@@ -429,6 +431,8 @@ __global__ __launch_bounds__(256) void cell_step_kernel(CellArgs a, StepDev st, 
   const double tavg = ldr(frow_ptr(a, MHMHIP_F_TAVG, st.day), bc);
   const double tmax = a.pet_case == 1 ? ldr(frow_ptr(a, MHMHIP_F_TMAX, st.day), bc) : 0.0;
   const double tmin = a.pet_case == 1 ? ldr(frow_ptr(a, MHMHIP_F_TMIN, st.day), bc) : 0.0;
+  mathtab_fill();  // the chain's log and exp read it (every thread, before any returns)
+  if (!act) return;
   const double pd = pet_day<NH>(a, st, bv, bc, tavg, tmax, tmin);
   disagg_step<NH>(a, st, bc, pd, tavg, pre, x.pet, x.temp, x.prec);
   if (st.init_sm) {

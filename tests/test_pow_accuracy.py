@@ -149,21 +149,26 @@ def _fma(a, b, c):
 
 
 def _table(name):
-    """The rows of kLogTab (4 values each) or kExpTab (2 each) in mathtab.h."""
+    """The log rows (4 values each) or the exp rows (2 each) of mathtab.h's
+    kMathTab."""
     import re
     path = os.path.join(ROOT, "mhm_amd", "csrc", "mathtab.h")
     text = open(path).read()
-    body = text[text.index(name + "["):]
+    n_log = int(re.search(r"kLogTabN = (\d+);", text).group(1))
+    body = text[text.index("kMathTab["):]
     body = body[body.index("{") + 1:body.index("};")]
+    body = re.sub(r"//[^\n]*", "", body)
     vals = [float.fromhex(v) if "x" in v else float(v)
             for v in re.findall(r"-?0x[0-9a-fp.+-]+|0\.0", body)]
-    w = 4 if name == "kLogTab" else 2
-    return [vals[w * k:w * k + w] for k in range(len(vals) // w)]
+    if name == "log":
+        return [vals[4 * k:4 * k + 4] for k in range(n_log)]
+    e = vals[4 * n_log:]
+    return [e[2 * k:2 * k + 2] for k in range(len(e) // 2)]
 
 
 def _log_copy(x, degree=7):
     """log_pos restated in Python (every operation correctly rounded)."""
-    tab = _table("kLogTab")
+    tab = _table("log")
     m, e = math.frexp(x)
     jd = float(round(m * 256.0))
     f = _fma(jd, -0.00390625, m)
@@ -178,7 +183,7 @@ def _log_copy(x, degree=7):
 
 def _exp_copy(z, top=4):
     """exp_fin restated in Python; top: the last Taylor term of exp(r) - 1."""
-    tab = _table("kExpTab")
+    tab = _table("exp")
     n = float(round(z * float.fromhex("0x1.71547652b82fep+8")))
     r = _fma(-n, LN2_HI / 256.0, z)
     r = _fma(-n, LN2_LO / 256.0, r)
