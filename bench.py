@@ -477,6 +477,8 @@ def main():
                     help="routing window of the headline loop (ring memory grows with it)")
     ap.add_argument("--no-graph", action="store_true", help="headline loop without hipGraph")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-block", type=int, default=0, choices=(0, 64, 128, 256),
+                    help="threads per workgroup of the plain path's cell kernel (0: 256)")
     ap.add_argument("--no-year", action="store_true")
     ap.add_argument("--no-per-step", action="store_true")
     ap.add_argument("--per-step-steps", type=int, default=48)
@@ -558,6 +560,8 @@ def main():
     extra = {"roofline_hbm": hbm}
     if not args.no_plain:
         pkw = dict(mode=N.RUN_PLAIN, want_gauges=True, route_window=args.route_window)
+        if args.plain_block:
+            pkw["block"] = args.plain_block
         model.run(steps[:48], **pkw)
         model.set_profiling(True, PROF_STRIDE)
         tp = timed_runs(model, steps, 1, **pkw)

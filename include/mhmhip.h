@@ -182,8 +182,9 @@ typedef struct {
 /* Options of mhmhip_run. */
 typedef struct {
   int32_t mode;          /* MHMHIP_RUN_PLAIN or MHMHIP_RUN_FUSED                 */
-  int32_t steps_per_launch; /* fused path: S (0 = autotuned)                   */
-  int32_t block;         /* cell-kernel block size (0 = autotuned)               */
+  int32_t steps_per_launch; /* fused path: S (0 = 24)                          */
+  int32_t block;         /* cell-kernel block size 64/128/256 (0 = 256 fused,
+                            128 plain)                                          */
   int32_t use_graph;     /* capture the tile sequence in a hipGraph              */
   int32_t route_mode;    /* 0 auto (= 3), 1 grid diagonals, 2 workgroup-resident
                             trees, 3 tiles (windows of up to 64 steps per launch) */
