@@ -216,16 +216,16 @@ __host__ __device__ inline double exp_ylogx(double y, double x) {
 
 // x**(2/3) for finite x > 0 (the canopy's (I / Imax)**(2/3),
 // mo_canopy_interc.f90:121), without a log and an exp: x = m 2^e with
-// e = 3q + r (r in 0..2), a = m 2^r in [1/2, 4), and c = cbrt(a) from a
-// single-precision seed 2^(log2(a) / 3) (relative error below 2^-20),
-// refined by one Halley step written as a small correction,
-//   c' = c - c (c^3 - a) / (2 c^3 + a),
-// with c^2 split exactly (fma) so that c^3 - a keeps its digits: the seed's
-// error becomes 2^-60 before rounding; x**(2/3) = c'^2 2^(2q).  Within 3 ulp
-// of the exact power (against mpmath in the tests).  The seed comes from the
-// hardware's single-precision log2 / exp2 on the device and from libm on the
-// host, and the divisor's reciprocal from v_rcp_f64 and one Newton step, so
-// the two builds may differ in the last bit.
+// e = 3q + r (r in 0..2), a = m 2^r in [1/2, 4), y = a^(-1/3) from a
+// single-precision seed 2^(-log2(a) / 3) (relative error below 2^-20),
+// refined by one Halley step written as a series in the residual
+// r = 1 - a y^3 (the binomial series of (1 - r)^(-1/3)),
+//   y' = y + y r (1/3 + 2 r / 9)      (error 14 r^3 / 81, 2^-62, before rounding),
+// and x**(2/3) = a y' 2^(2q): no division.  Within 3 ulp of the exact power
+// (against mpmath in the tests), exact on exact cubes such as powers of 8,
+// 27 and 125.  The seed comes from the hardware's single-precision log2 /
+// exp2 on the device and from libm on the host, so the two builds may
+// differ in the last bit.
 __host__ __device__ inline double pow23(double x) {
   int e;
   const double m = frexp(x, &e);  // [1/2, 1)
@@ -243,22 +243,14 @@ __host__ __device__ inline double pow23(double x) {
 #endif
   const double a = ldexp(m, er);
 #if defined(__HIP_DEVICE_COMPILE__)
-  const double t =
-      (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)a) * (1.0f / 3.0f));
+  const double y =
+      (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)a) * (-1.0f / 3.0f));
 #else
-  const double t = (double)exp2f(log2f((float)a) * (1.0f / 3.0f));
+  const double y = (double)exp2f(log2f((float)a) * (-1.0f / 3.0f));
 #endif
-  const double t2h = t * t, t2l = fma(t, t, -t2h);     // t^2 exactly
-  const double d = fma(t2l, t, fma(t2h, t, -a));       // t^3 - a
-  const double den = fma(2.0 * t2h, t, a);             // 2 t^3 + a (to ~1 ulp)
-#if defined(__HIP_DEVICE_COMPILE__)
-  double rr = __builtin_amdgcn_rcp(den);
-  rr = fma(rr, fma(-den, rr, 1.0), rr);
-#else
-  const double rr = 1.0 / den;
-#endif
-  const double c = fma(-(t * d), rr, t);
-  return ldexp(c * c, 2 * q);
+  const double r = fma(-a, (y * y) * y, 1.0);
+  const double y1 = fma(y * r, fma(r, 2.0 / 9.0, 1.0 / 3.0), y);
+  return ldexp(a * y1, 2 * q);
 }
 
 // x / d for a divisor whose correctly rounded reciprocal r = RN(1/d) is known:
