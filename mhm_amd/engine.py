@@ -152,10 +152,13 @@ class Model:
 
     def set_mprlite_inputs(self):
         """The packed domains' MPR-lite inputs (cover fractions, soil texture,
-        aspect, latitude; the first domain's horizon depths; the cover
-        classes' monthly LAI) for mprlite_params."""
+        aspect, latitude; the horizon depths, which the packed domains share;
+        the cover classes' monthly LAI) for mprlite_params."""
         from . import parameters as PM
         d = self.domains
+        if any(not np.array_equal(x.horizon_depth, d[0].horizon_depth) for x in d[1:]):
+            raise ValueError("MPR-lite on the device takes one set of horizon depths: the "
+                             "packed domains differ")
         cols = [np.concatenate([x.fractions[k] for x in d]) for k in range(3)]
         cols += [np.concatenate([getattr(x, f) for x in d]) for f in ("texture", "aspect", "lat")]
         cells = np.ascontiguousarray(np.stack(cols), dtype=np.float64)  # X(n_cells, 6)
