@@ -20,8 +20,12 @@ def _params_for(domains, gammas):
 
 
 def plain_domain(nrows, ncols, n_steps, timestep=1, seed=1, n_sets=1, forcing_seed=None,
-                 soil=1, pet=1, start=START):
-    """Model + step table for one basin (n_sets parameter sets drawn within the bounds)."""
+                 soil=1, pet=1, start=START, device_params=False):
+    """Model + step table for one basin (n_sets parameter sets drawn within the
+    bounds).  device_params: each set's parameter rows built on the device
+    from its gamma vector (Model.mprlite_params, one launch per set) instead
+    of in numpy and uploaded; the two agree bit for bit but for the root
+    fractions' power."""
     dom = synth.make_domain(nrows, ncols, seed=seed)
     n_days = calendar.n_forcing_days_needed(n_steps)
     synth.make_forcing(dom, n_days, seed + 1000 if forcing_seed is None else forcing_seed,
@@ -31,8 +35,13 @@ def plain_domain(nrows, ncols, n_steps, timestep=1, seed=1, n_sets=1, forcing_se
     model.bind_network()
     rng = np.random.default_rng(seed + 2000)
     gam = PM.draw(rng, n_sets)
+    if device_params:
+        model.set_mprlite_inputs()
     for s in range(n_sets):
-        model.set_params(_params_for([dom], [gam[s]]), s)
+        if device_params:
+            model.mprlite_params(gam[s], s)
+        else:
+            model.set_params(_params_for([dom], [gam[s]]), s)
         model.set_routing_params(PM.routing_gamma(gam[s]), s)
     model.default_states()
     model.set_domain_forcing()

@@ -135,3 +135,25 @@ def test_device_mprlite_rejects_missing_inputs_and_bad_sets():
         m.mprlite_params(g[:10], 0)
     m.mprlite_params(g, 1)
     m.close()
+
+
+def test_ensemble_builder_with_device_parameter_sets():
+    """workloads.plain_domain(device_params=True), as the ensemble benchmark
+    builds its sets, gives the run of the host-built rows set by set."""
+    print("case: ensemble builder, 8 sets x 12x10 cells, daily x 40, device parameter sets",
+          flush=True)
+    from mhm_amd import workloads
+    power = _chain_pow()
+    kw = dict(timestep=24, seed=5, n_sets=8)
+    dev, steps, dom, gam = workloads.plain_domain(12, 10, 40, device_params=True, **kw)
+    host, _, _, _ = workloads.plain_domain(12, 10, 40, **kw)
+    for s in range(8):
+        host.set_params(_host_params(dom, gam[s], 1, power), s)
+    gd = dev.run(steps, mode=1, steps_per_launch=20)
+    gh = host.run(steps, mode=1, steps_per_launch=20)
+    assert np.array_equal(gd, gh)
+    for s in (0, 7):
+        for name in STATE_NAMES:
+            assert np.array_equal(dev.get(name, s), host.get(name, s)), (name, s)
+    dev.close()
+    host.close()
