@@ -303,12 +303,13 @@ def bench_year_basin(N, workloads, autotune):
     return out, dom, gam
 
 
-def bench_multi_basin(N, workloads, S):
+def bench_multi_basin(N, workloads, S, window):
     t0 = time.perf_counter()
     mm, msteps, mdoms, _ = workloads.multi_basin(64, 500, 200000, 720, seed=7)
     setup = time.perf_counter() - t0
     cells = sum(d.n_cells for d in mdoms)
-    kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=True)
+    kw = dict(mode=N.RUN_FUSED, steps_per_launch=S, want_gauges=True, use_graph=True,
+              route_window=window)
     mm.run(msteps, **kw)
     mm.set_profiling(True, PROF_STRIDE)
     mm.run(msteps, **kw)
@@ -319,8 +320,9 @@ def bench_multi_basin(N, workloads, S):
     return {"value": cells * len(msteps) / tm, "unit": "cell-steps/s",
             "workload": f"64 basins, log-uniform 500..200k cells ({cells} in all, largest "
                         f"{max(d.n_cells for d in mdoms)}, max depth {max(d.dmax for d in mdoms)}),"
-                        f" own DEM/network/forcing/parameters each, hourly x 720, fused S={S}",
-            "setup_s": round(setup, 1), "kernels": kernel_table(mprof, S, 24)}
+                        f" own DEM/network/forcing/parameters each, hourly x 720, fused S={S},"
+                        f" tiled routing window {window}",
+            "setup_s": round(setup, 1), "kernels": kernel_table(mprof, S, 24, window)}
 
 
 def bench_serving(serve, n_req, qps, pool, seed=11):
@@ -592,7 +594,7 @@ def main():
         extra["ensemble"] = bench_ensemble(N, workloads, autotune, traffic)
         log(f"ensemble: {extra['ensemble']['value']:.4g} {extra['ensemble']['settings']}")
     if not args.no_multi:
-        extra["multi_basin"] = bench_multi_basin(N, workloads, S)
+        extra["multi_basin"] = bench_multi_basin(N, workloads, S, args.route_window)
         log(f"multi_basin: {extra['multi_basin']['value']:.4g}")
     if not args.no_serve:
         extra["serving"] = bench_serving(serve, args.serve_requests, args.serve_qps,
