@@ -516,6 +516,10 @@ __device__ __forceinline__ double total_runoff(double fS, double fast, double sl
 // latitude terms tan/sin/cos(deg2rad*lat) are per-cell constants and the
 // day-of-year terms dr, tan/sin/cos(delta) per-step constants; both are
 // evaluated once on the host with the same libm calls the reference makes.
+// sin(omega) of omega = acos(arg) is sqrt((1 - arg)(1 + arg)), the same
+// value within a few ulp (the factored form keeps its relative accuracy as
+// arg nears +-1), for a square root instead of a sine's range reduction and
+// series.
 __device__ MHMHIP_OUTLINE double pet_hargreaves(double coeff, double tavg, double tmax,
                                                  double tmin, double tanlat, double sinlat,
                                                  double coslat, double dr, double tand,
@@ -526,7 +530,8 @@ __device__ MHMHIP_OUTLINE double pet_hargreaves(double coeff, double tavg, doubl
   if (arg < -1.0) arg = -1.0;
   if (arg > 1.0) arg = 1.0;
   const double omega = acos(arg);
-  const double ra = kRadConst * dr * (omega * sinlat * sind + coslat * cosd * sin(omega));
+  const double sin_omega = sqrt((1.0 - arg) * (1.0 + arg));
+  const double ra = kRadConst * dr * (omega * sinlat * sind + coslat * cosd * sin_omega);
   return coeff * ra * (tavg + kHarSamConst) * sqrt(dT);
 }
 
