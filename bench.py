@@ -233,7 +233,7 @@ def bench_ensemble(N, workloads, autotune, traffic):
             "kernels": kernel_table(eprof, S, 24, best.get("route_window") or 8, 256)}
 
 
-def bench_per_step(model, steps, dom, n):
+def bench_per_step(model, steps, dom, n, tile_height):
     """The drop-in as the reference's time loop calls it: per step one
     mhmhip_mhm_step (meteo + mHM(), every flux written) and one
     mhmhip_mrm_step (mRM_routing) handing the gauge discharge back to the
@@ -255,7 +255,7 @@ def bench_per_step(model, steps, dom, n):
     return {"value": dom.n_cells * n / dt, "unit": "cell-steps/s", "ms_per_step": dt / n * 1e3,
             "workload": f"same basin, {n} hourly steps, each one mhm_step + one mrm_step "
                         "(gauges to the host every step), as mhm_interface_run_do_time_step "
-                        "would call them",
+                        f"would call them; routing tiles of {tile_height} levels",
             "one_step": {"cell_step_us": round(prof["cell_step"][0] * 1e3, 1),
                          "route_tile_launches": plan_launches,
                          "route_tile_us": round(prof["route_tile"][0] * 1e3, 1)}}
@@ -486,6 +486,8 @@ def main():
     ap.add_argument("--no-year", action="store_true")
     ap.add_argument("--no-per-step", action="store_true")
     ap.add_argument("--per-step-steps", type=int, default=48)
+    ap.add_argument("--per-step-tile-height", type=int, default=32,
+                    help="levels per routing tile of the per-step context (16: the headline's)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -578,11 +580,16 @@ def main():
                                                              24, traffic),
                           "kernels": kernel_table(pprof, 1, 24, args.route_window)}
         log(f"plain: {extra['plain']['value']:.4g} cell-steps/s")
+    model.close()
     if not args.no_per_step:
-        extra["per_step"] = bench_per_step(model, steps, dom, args.per_step_steps)
+        # the drop-in's own context: tiles of per_step_tile_height levels
+        pm, psteps = workloads.model_on(dom, args.per_step_steps + 2, gam,
+                                        tile_height=args.per_step_tile_height)
+        extra["per_step"] = bench_per_step(pm, psteps, dom, args.per_step_steps,
+                                           args.per_step_tile_height)
+        pm.close()
         log(f"per step: {extra['per_step']['value']:.4g} cell-steps/s, "
             f"{extra['per_step']['ms_per_step']:.3f} ms per step")
-    model.close()
 
     year = None
     if not args.no_year:

@@ -365,6 +365,15 @@ int mhmhip_plan_network(int64_t n_nodes, int32_t n_outlets, const int32_t *netPe
  * src/mRM/mo_mrm_routing.f90:104-110); resets routing states to 0
  * (src/mRM/mo_mrm_init.f90:453-494). */
 int mhmhip_bind_network(mhmhip_ctx *ctx, const mhmhip_network *net);
+
+/* Levels per routing tile of the partition the next mhmhip_bind_network
+ * builds: 2..64, 0 = 16 (the default).  No counterpart in the reference:
+ * the results are the same at every height; the height trades the number
+ * of tiled routing launches (tile depth + 1 for one step) against each
+ * launch's length.  16 suits the time loop (mhmhip_run), 32 a caller that
+ * routes one step per call (mhmhip_mrm_step: 0.94 against 1.16 ms per step
+ * on a 1024 x 1024 basin).  Only before the first mhmhip_bind_network. */
+int mhmhip_set_tile_height(mhmhip_ctx *ctx, int32_t height);
 /* The five routing1 parameters (mhm_parameter.nml:193-201) of one parameter
  * set for one domain (domain < 0: every domain); C1/C2 follow from reg_rout
  * (src/mRM/mo_mrm_mpr.f90:61-119) over that domain's links. */

@@ -93,6 +93,7 @@ SIGNATURES = {
     "mhmhip_plan_network": (C.c_int, [C.c_int64, C.c_int32, _ip, _ip, _ip, C.c_void_p, _lp, _ip,
                                       C.c_int64]),
     "mhmhip_bind_network": (C.c_int, [C.c_void_p, C.POINTER(Network)]),
+    "mhmhip_set_tile_height": (C.c_int, [C.c_void_p, C.c_int32]),
     "mhmhip_set_routing_params": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
     "mhmhip_set_routing_celerity": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "mhmhip_set_routing_celerity_links": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),

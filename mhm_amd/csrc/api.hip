@@ -107,6 +107,7 @@ struct mhmhip_ctx {
   std::vector<int64_t> npos;
   std::vector<int64_t> dom_node_start;
   bool coarse = false;
+  int32_t tile_h = kTileHeight;  // levels per routing tile (mhmhip_set_tile_height)
   int32_t *d_agg_start = nullptr;
   double *d_cring = nullptr;
   int64_t cring_rows = 0;
@@ -1513,7 +1514,7 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
   if (route) {
     if (x->t_route != x->t_cell)
       return x->fail(MHMHIP_ESTATE, "mhm_step: previous step not routed (call mrm_step)");
-    CHECK_RC(ensure_ring(x, 1 + kTileHeight));
+    CHECK_RC(ensure_ring(x, 1 + x->tile_h));
   }
   // the step is step 0 of a one-step ring, routed by the tiles (mrm_step); on
   // a coarser routing grid mrm_step aggregates the step's L1_total_runoff
@@ -1533,6 +1534,19 @@ extern "C" int mhmhip_mhm_step(mhmhip_ctx *x, const mhmhip_step *st) {
 
 // ---------------------------------------------------------------------------
 // network and routing
+
+extern "C" int mhmhip_set_tile_height(mhmhip_ctx *x, int32_t height) {
+  if (!x) return MHMHIP_EINVAL;
+  // a tile's proxies for its upstream tiles sit one level below its root
+  if (height < 0 || height == 1 || height > kTileHeightMax)
+    return x->fail(MHMHIP_EINVAL, "set_tile_height: height must be 0 or 2.." +
+                                      std::to_string(kTileHeightMax));
+  // the ring's rows past its length and the bound tiles follow the height
+  if (x->net_bound)
+    return x->fail(MHMHIP_ESTATE, "set_tile_height: call it before bind_network");
+  x->tile_h = height ? height : kTileHeight;
+  return MHMHIP_OK;
+}
 
 extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   if (!x) return MHMHIP_EINVAL;
@@ -1576,7 +1590,7 @@ extern "C" int mhmhip_bind_network(mhmhip_ctx *x, const mhmhip_network *net) {
   CHECK_HIP(x, hipStreamSynchronize(x->stream), "bind_network");
 
   LayoutIn in{net->n_nodes, x->n_domains, ds.data(), net->n_outlets, net->netPerm, net->fromN,
-              net->toN};
+              net->toN, x->tile_h};
   LayoutOut lay;
   std::string e;
   int rc = build_layout(in, lay, e);
@@ -2563,7 +2577,7 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
     const int64_t gper = (int64_t)x->n_sets * std::max(1, x->n_gauges);
     const size_t need = (size_t)round_up((int64_t)sizeof(StepDev) * nsteps, 256) +
                         (route ? (size_t)round_up(8 * nr * gper, 256) +
-                                     (size_t)(8 * (ring_rows + kTileHeight) * x->ld)
+                                     (size_t)(8 * (ring_rows + x->tile_h) * x->ld)
                                : 0);
     if (need > kScratchBytes)
       return x->fail(MHMHIP_EINVAL,
@@ -2573,17 +2587,17 @@ int run_impl(mhmhip_ctx *x, const mhmhip_step *steps, int32_t nsteps,
                      "smaller route_window, or mhmhip_run)");
     StepDev *st = reinterpret_cast<StepDev *>(carve(sizeof(StepDev) * nsteps));
     double *gb = route ? reinterpret_cast<double *>(carve(8 * (size_t)nr * gper)) : nullptr;
-    double *rg = route ? reinterpret_cast<double *>(carve(8 * (ring_rows + kTileHeight) * x->ld))
+    double *rg = route ? reinterpret_cast<double *>(carve(8 * (ring_rows + x->tile_h) * x->ld))
                        : nullptr;
     swap.swap_in(st, nsteps, route ? gb : x->d_gauge_buf, route ? nr : x->gauge_buf_rows,
-                 route ? rg : x->d_ring, route ? ring_rows + kTileHeight : x->ring_len);
+                 route ? rg : x->d_ring, route ? ring_rows + x->tile_h : x->ring_len);
   }
   CHECK_RC(ensure_steps(x, nsteps));
   CHECK_HIP(x, hipMemcpyAsync(x->d_steps, hs.data(), sizeof(StepDev) * nsteps,
                               hipMemcpyHostToDevice, x->stream),
             "upload steps");
   if (route) {
-    CHECK_RC(ensure_ring(x, ring_rows + kTileHeight));  // ring_row's rows past R
+    CHECK_RC(ensure_ring(x, ring_rows + x->tile_h));  // ring_row's rows past R
     CHECK_RC(ensure_gauge_buf(x, nr));
     if (rep > 1 && !x->d_qsub) CHECK_RC(x->dalloc(&x->d_qsub, x->ld, "alloc Qmod sub-step sums"));
   }

@@ -190,9 +190,11 @@ struct MeteoL2Cell {
 #define MHMHIP_TILE_WINDOW 64
 #endif
 constexpr int kTileSlots = MHMHIP_TILE_SLOTS;
-constexpr int kTileHeight = MHMHIP_TILE_HEIGHT;
+constexpr int kTileHeight = MHMHIP_TILE_HEIGHT;  // default levels per tile
+constexpr int kTileHeightMax = 64;                // mhmhip_set_tile_height's limit
 constexpr int kTileWindowMax = MHMHIP_TILE_WINDOW;  // steps per window, at most
-static_assert(kTileHeight <= 256, "slot lags are stored in 8 bits");
+static_assert(kTileHeight >= 1 && kTileHeight <= kTileHeightMax && kTileHeightMax <= 256,
+              "slot lags are stored in 8 bits");
 
 // Ring row of step t (relative to the call) of a node with lag g, in a ring
 // of ring_len logical rows (plus kTileHeight rows past them).  The tiled

@@ -86,8 +86,8 @@ namespace mhmhip {
 // their qTR sum runs in the order of L11_routing's accumulation.
 //
 // Tile partition, bottom-up (reverse breadth-first order): a node's open
-// tile takes upstream open tiles within kTileSlots slots and kTileHeight
-// levels; an upstream open tile that is closed becomes a tile of its own
+// tile takes upstream open tiles within kTileSlots slots and
+// in.tile_height levels; an upstream open tile that is closed becomes a tile of its own
 // and a proxy slot in the node's tile.  Which ones are closed is chosen to
 // keep the longest chain of tiles short (the rule is at the partition).
 // Tiles are numbered breadth-first over the tile tree (tile depth tau
@@ -96,6 +96,10 @@ namespace mhmhip {
 // slots, lag-major inside a bin; device indices follow the real slots of the
 // bins in order.
 int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
+  if (in.tile_height < 2 || in.tile_height > kTileHeightMax) {
+    err = "bind_network: tile height out of range";
+    return MHMHIP_EINVAL;
+  }
   const int64_t nN = in.n_nodes;
   out.pos.assign(nN, -1);
   out.node_at.assign(nN, -1);
@@ -210,7 +214,7 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
         const int64_t c = child_node(i);
         if (oc[c] == M) hc = std::max(hc, oh[c]);
       }
-      bool open = M < 0 || 1 + hc <= kTileHeight;
+      bool open = M < 0 || 1 + hc <= in.tile_height;
       if (open) {
         for (int64_t i : up[n]) {
           const int64_t c = child_node(i);
@@ -285,7 +289,7 @@ int build_layout(const LayoutIn &in, LayoutOut &out, std::string &err) {
       }
       T.height = 0;
       for (const TSlot &u : T.s) T.height = std::max(T.height, u.ldepth + 1);
-      if ((int)T.s.size() > kTileSlots || T.height > kTileHeight) {
+      if ((int)T.s.size() > kTileSlots || T.height > in.tile_height) {
         err = "bind_network: tile partition failed";
         return MHMHIP_EINVAL;
       }

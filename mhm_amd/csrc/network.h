@@ -17,6 +17,7 @@ struct LayoutIn {
   const int32_t *netPerm;         // packed per-domain links, 1-based local
   const int32_t *fromN;
   const int32_t *toN;
+  int32_t tile_height = kTileHeight;  // levels per tile, 2..kTileHeightMax
 };
 
 struct LayoutOut {

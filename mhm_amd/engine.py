@@ -197,6 +197,11 @@ class Model:
                 self.set_forcing(name, data, day0)
 
     # -- network -----------------------------------------------------------------
+    def set_tile_height(self, height):
+        """Levels per routing tile of the partition bind_network builds (0 = 16;
+        32 for callers that route one step per call).  Before bind_network."""
+        self._chk(N.lib().mhmhip_set_tile_height(self._ctx, int(height)))
+
     def bind_network(self, inflow=None):
         doms = self.domains
         cat = lambda f, dt: np.ascontiguousarray(np.concatenate([f(d) for d in doms]).astype(dt))

@@ -67,7 +67,7 @@ module mo_mhmhip
             mhmhip_set_forcing, mhmhip_stream_forcing, mhmhip_set_forcing_l2, &
             mhmhip_set_meteo_weights, &
             mhmhip_set_routing_celerity, mhmhip_set_routing_celerity_links, &
-            mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, &
+            mhmhip_mhm_step, mhmhip_routing_order, mhmhip_bind_network, mhmhip_set_tile_height, &
             mhmhip_set_routing_params, mhmhip_set_inflow, mhmhip_get_routing_state, &
             mhmhip_set_routing_state, mhmhip_mrm_step, mhmhip_run, mhmhip_run_noalloc, &
             mhmhip_set_profiling, &
@@ -168,6 +168,12 @@ module mo_mhmhip
       import :: c_int, c_ptr, mhmhip_network
       type(c_ptr), value :: ctx
       type(mhmhip_network), intent(in) :: net
+    end function
+    integer(c_int) function mhmhip_set_tile_height(ctx, height) &
+        bind(C, name="mhmhip_set_tile_height")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: height
     end function
     integer(c_int) function mhmhip_set_routing_params(ctx, set, domain, gamma) &
         bind(C, name="mhmhip_set_routing_params")

@@ -20,21 +20,33 @@ def _params_for(domains, gammas):
 
 
 def plain_domain(nrows, ncols, n_steps, timestep=1, seed=1, n_sets=1, forcing_seed=None,
-                 soil=1, pet=1, start=START, device_params=False):
+                 soil=1, pet=1, start=START, device_params=False, tile_height=0):
     """Model + step table for one basin (n_sets parameter sets drawn within the
     bounds).  device_params: each set's parameter rows built on the device
     from its gamma vector (Model.mprlite_params, one launch per set) instead
     of in numpy and uploaded; the two agree bit for bit but for the root
-    fractions' power."""
+    fractions' power.  tile_height: levels per routing tile (0: 16)."""
     dom = synth.make_domain(nrows, ncols, seed=seed)
     n_days = calendar.n_forcing_days_needed(n_steps)
     synth.make_forcing(dom, n_days, seed + 1000 if forcing_seed is None else forcing_seed,
                        doy0=start.timetuple().tm_yday)
+    gam = PM.draw(np.random.default_rng(seed + 2000), n_sets)
+    model, steps = model_on(dom, n_steps, gam, timestep, soil, pet, start, device_params,
+                            tile_height)
+    return model, steps, dom, gam
+
+
+def model_on(dom, n_steps, gam, timestep=1, soil=1, pet=1, start=START, device_params=False,
+             tile_height=0):
+    """Model + step table on a built basin with its forcing, one parameter set
+    per row of gam."""
+    n_sets = len(gam)
+    n_days = calendar.n_forcing_days_needed(n_steps)
     model = engine.Model([dom], n_sets=n_sets, timestep=timestep, process_soil=soil,
                          process_pet=pet, route=True, n_forcing_days=n_days)
+    if tile_height:
+        model.set_tile_height(tile_height)
     model.bind_network()
-    rng = np.random.default_rng(seed + 2000)
-    gam = PM.draw(rng, n_sets)
     if device_params:
         model.set_mprlite_inputs()
     for s in range(n_sets):
@@ -44,9 +56,9 @@ def plain_domain(nrows, ncols, n_steps, timestep=1, seed=1, n_sets=1, forcing_se
             model.set_params(_params_for([dom], [gam[s]]), s)
         model.set_routing_params(PM.routing_gamma(gam[s]), s)
     model.default_states()
-    model.set_domain_forcing()
+    model.set_domain_forcing(n_days=n_days)
     steps = calendar.step_table(start, n_steps, timestep)
-    return model, steps, dom, gam
+    return model, steps
 
 
 def multi_basin(n_basins, lo, hi, n_steps, timestep=1, seed=7, start=START):

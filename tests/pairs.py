@@ -11,13 +11,14 @@ START = dt.datetime(1990, 1, 1)
 
 def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, route=True,
               forcing_seed=100, pet_forcing=False, start=START, inflow=None,
-              forcing_columns=None, weights=False):
+              forcing_columns=None, weights=False, tile_height=0):
     """HIP model and oracle on the same inputs.  inflow: {"domain", "node" (1-based),
     "headwater", "q" (n_steps, n_inflow)} inflow gauges, given to both.
     forcing_columns: the model gets that many forcing columns and no forcing
     (the caller streams it); the oracle gets all of it.
     pet 2, 3: the radiation-based PET cases get their daily inputs and
-    monthly parameters; weights: meteo weights instead of day/night factors."""
+    monthly parameters; weights: meteo weights instead of day/night factors.
+    tile_height: levels per routing tile of the model's partition (0: 16)."""
     n_days = calendar.n_forcing_days_needed(n_steps)
     doy0 = start.timetuple().tm_yday
     for i, d in enumerate(domains):
@@ -34,6 +35,8 @@ def make_pair(domains, n_sets=1, timestep=1, n_steps=24, soil=1, pet=1, seed=0, 
     sim = OracleSim(domains, n_sets=n_sets, timestep=timestep, process_soil=soil, process_pet=pet,
                     n_forcing_days=n_days, route=route)
     if route:
+        if tile_height:
+            model.set_tile_height(tile_height)
         model.bind_network(inflow=inflow)
         if inflow is not None:
             model.set_inflow(inflow["q"])
