@@ -164,6 +164,8 @@ contains
     net%n_gauges = n_gauges
     net%gauge_domain = c_loc(gdom(1))
     net%gauge_node = c_loc(gnode(1))
+    ! one routing step per call: tiles of 32 levels (fewer tile launches per step)
+    call mhmhip_check(mhmhip_set_tile_height(ctx, 32_c_int32_t), ctx, "set_tile_height")
     call mhmhip_check(mhmhip_bind_network(ctx, net), ctx, "bind_network")
     if (with_routing_params) &
       call mhmhip_check(mhmhip_set_routing_params(ctx, 0, -1, gam), ctx, "routing params")
