@@ -341,8 +341,9 @@ int mhmhip_routing_order(int64_t n_nodes, int64_t n_links, const int32_t *fromN,
                          const int32_t *toN, int32_t *rOrder, int32_t *netPerm);
 
 /* Device layout of one domain's network as mhmhip_bind_network builds it
- * (no counterpart in the reference; for footprint estimates before a
- * context exists and for inspection).  Host-only.  netPerm/fromN/toN as in
+ * at the default tile height of 16 levels (no counterpart in the reference;
+ * for footprint estimates before a context exists and for inspection).
+ * Host-only.  netPerm/fromN/toN as in
  * mhmhip_network for one domain (n_nodes - n_outlets links).  Optional
  * outputs: pos[n_nodes] = device index of each node; slots[4 * n_slots] =
  * per bin slot: device index (-1: proxy or padding), boundary row (-1: none),
