@@ -485,6 +485,8 @@ def main():
                     help="threads per workgroup of the plain path's cell kernel (0: 256)")
     ap.add_argument("--no-year", action="store_true")
     ap.add_argument("--no-per-step", action="store_true")
+    ap.add_argument("--tile-height", type=int, default=0,
+                    help="levels per routing tile of the headline context (0: 16)")
     ap.add_argument("--per-step-steps", type=int, default=48)
     ap.add_argument("--per-step-tile-height", type=int, default=32,
                     help="levels per routing tile of the per-step context (16: the headline's)")
@@ -532,7 +534,8 @@ def main():
     flops = load_json(os.path.join(ROOT, "profiles", "fp64_flops.json"))
     n_steps = args.days * 24
     t0 = time.perf_counter()
-    model, steps, dom, gam = workloads.plain_domain(args.grid, args.grid, n_steps, seed=1)
+    model, steps, dom, gam = workloads.plain_domain(args.grid, args.grid, n_steps, seed=1,
+                                                    tile_height=args.tile_height)
     log(f"setup {time.perf_counter() - t0:.1f}s: cells={dom.n_cells} dmax={dom.dmax} "
         f"steps={n_steps} device_bytes={model.device_bytes / 2**30:.2f} GiB")
     S = args.steps_per_launch
